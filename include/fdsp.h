@@ -1,0 +1,235 @@
+/*
+ * fdsp.h -- C ABI of the MI355X (gfx950) streaming-channelizer library.
+ *
+ * Every entry point takes plain pointers and sizes and returns an int status
+ * (FDSP_OK or a negative FDSP_E* code; fdsp_last_error() describes the last
+ * failure on the calling thread). Nothing here ever calls exit().
+ *
+ * Two calling forms per operation:
+ *   *_apply        host pointers, synchronous: the drop-in form that the
+ *                  reference's std::vector based classes map onto.
+ *   *_apply_device device pointers, asynchronous on `stream` (a hipStream_t;
+ *                  NULL means the handle's own stream). Output counts are
+ *                  exact and known on return, before the kernels finish.
+ *
+ * Complex samples are interleaved float pairs {re, im} (std::complex<float>).
+ * Handles are not thread-safe: one owner per handle.
+ */
+#ifndef FDSP_H
+#define FDSP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FDSP_OK        0
+#define FDSP_EINVAL   -1  /* bad argument */
+#define FDSP_ENOTINIT -2  /* handle not initialised */
+#define FDSP_EBUSY    -3  /* resource in use */
+#define FDSP_ENOMEM   -4  /* device or host allocation failed / over budget */
+#define FDSP_EHIP     -5  /* HIP runtime error */
+#define FDSP_ERANGE   -6  /* size above a limit, or output capacity too small */
+#define FDSP_EAGAIN   -7  /* serving: not admitted yet, retry later */
+
+typedef struct fdsp_cf32 { float re, im; } fdsp_cf32;
+typedef void *fdsp_stream_t; /* hipStream_t */
+typedef void *fdsp_event_t;  /* hipEvent_t */
+
+const char *fdsp_last_error(void);
+int fdsp_version(void);
+
+/* ---- device plumbing (plain HIP calls behind a C ABI; used by hosts that
+ *      have no HIP headers, e.g. ctypes) ---- */
+int fdsp_device_count(int *count);
+int fdsp_set_device(int device);
+int fdsp_malloc(void **ptr, size_t bytes);
+int fdsp_free(void *ptr);
+int fdsp_memcpy_h2d(void *dst, const void *src, size_t bytes, fdsp_stream_t stream);
+int fdsp_memcpy_d2h(void *dst, const void *src, size_t bytes, fdsp_stream_t stream);
+int fdsp_memcpy_d2d(void *dst, const void *src, size_t bytes, fdsp_stream_t stream);
+int fdsp_memset(void *dst, int value, size_t bytes, fdsp_stream_t stream);
+int fdsp_host_alloc_pinned(void **ptr, size_t bytes);
+int fdsp_host_free_pinned(void *ptr);
+int fdsp_stream_create(fdsp_stream_t *stream);
+int fdsp_stream_destroy(fdsp_stream_t stream);
+int fdsp_stream_sync(fdsp_stream_t stream);
+int fdsp_device_sync(void);
+int fdsp_event_create(fdsp_event_t *ev);
+int fdsp_event_destroy(fdsp_event_t ev);
+int fdsp_event_record(fdsp_event_t ev, fdsp_stream_t stream);
+int fdsp_event_sync(fdsp_event_t ev);
+int fdsp_event_elapsed_ms(fdsp_event_t start, fdsp_event_t stop, float *ms);
+
+/* ---- rate helpers ----
+ * fdsp_gcd / fdsp_lcm replace calculate_gcd / calculate_lcm
+ *   (reference src/utilities/falcon_dsp_utils.cc:123-137).
+ * fdsp_rat_approx replaces rat_approx (falcon_dsp_utils.cc:231-294).
+ * fdsp_freq_shift_params replaces falcon_dsp_freq_shift::get_freq_shift_params
+ *   (reference src/transform/falcon_dsp_freq_shift.cc:194-235); f_hz == 0 is
+ *   defined here as the identity shift (rollover 1, omega 0). */
+uint64_t fdsp_gcd(uint64_t a, uint64_t b);
+uint64_t fdsp_lcm(uint64_t a, uint64_t b);
+void fdsp_rat_approx(double f, int64_t max_denom, int64_t *num, int64_t *denom);
+int fdsp_freq_shift_params(uint32_t fs, int32_t f_hz, uint32_t *rollover, double *omega);
+
+/* ---- NCO frequency shift ----
+ * Replaces falcon_dsp_freq_shift::{initialize, apply(cf32), reset_state}
+ * (reference hdr/transform/falcon_dsp_freq_shift.h:98-121, src .cc:107-192) and
+ * falcon_dsp_freq_shift_cuda::{initialize(fs, vector<int32>),
+ * apply(in, vector<vector<cf32>>)} (hdr/transform/falcon_dsp_freq_shift_cuda.h:187-212,
+ * src/transform/falcon_dsp_freq_shift_cuda.cu:355-488): one input, n_shifts
+ * outputs, y_c[n] = x[n] * exp(j*w_c*((s_c + n) mod R_c)), s_c carried. */
+typedef struct fdsp_freq_shift_s *fdsp_freq_shift_t;
+int fdsp_freq_shift_create(uint32_t fs, const int32_t *shifts_hz, uint32_t n_shifts,
+                           fdsp_freq_shift_t *out);
+int fdsp_freq_shift_destroy(fdsp_freq_shift_t h);
+int fdsp_freq_shift_reset(fdsp_freq_shift_t h);
+/* samples handled modulo the rollover index for shift `chan` */
+int fdsp_freq_shift_get_state(fdsp_freq_shift_t h, uint32_t chan, uint64_t *samples_handled);
+/* outs[c] receives n samples for each shift c */
+int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *const *outs);
+/* output of shift c starts at d_out + c * out_stride (in samples) */
+int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t n,
+                                 fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream);
+
+/* ---- polyphase rational resampler ----
+ * Replaces falcon_dsp_polyphase_resampler::{initialize, apply(cf32),
+ * needed_out_count, coeffs_per_phase, reset_state}
+ * (reference hdr/transform/falcon_dsp_polyphase_resampler.h:168-195,
+ * src .cc:114-346) and falcon_dsp_polyphase_resampler_cuda::apply
+ * (src/transform/falcon_dsp_polyphase_resampler_cuda.cu:459-619). Only the
+ * real part of each tap is used, as in the reference (.cc:288,298). */
+typedef struct fdsp_resampler_s *fdsp_resampler_t;
+int fdsp_resampler_create(uint32_t up, uint32_t down, const fdsp_cf32 *taps, uint32_t n_taps,
+                          fdsp_resampler_t *out);
+int fdsp_resampler_destroy(fdsp_resampler_t h);
+int fdsp_resampler_reset(fdsp_resampler_t h);
+int fdsp_resampler_needed_out_count(fdsp_resampler_t h, uint32_t in_count, uint32_t *out_count);
+int fdsp_resampler_coeffs_per_phase(fdsp_resampler_t h, uint32_t *cpp);
+/* coeff_phase and xOffset exactly as the reference carries them, plus the
+ * totals of inputs consumed and outputs produced since the last reset */
+int fdsp_resampler_get_state(fdsp_resampler_t h, uint32_t *coeff_phase, uint32_t *x_offset,
+                             uint64_t *in_total, uint64_t *out_total);
+int fdsp_resampler_apply(fdsp_resampler_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out,
+                         size_t cap, size_t *n_out);
+int fdsp_resampler_apply_device(fdsp_resampler_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                size_t cap, size_t *n_out, fdsp_stream_t stream);
+
+/* ---- FIR filter (complex taps) ----
+ * Replaces falcon_dsp_fir_filter(coeffs) / apply(cf32) / reset_state
+ * (reference hdr/transform/falcon_dsp_fir_filter.h:119-140, src .cc:136-284)
+ * and falcon_dsp_fir_filter_cuda (src/transform/falcon_dsp_fir_filter_cuda.cu:256-373):
+ * y[n] = sum_k h[k] x[n-k], history carried. Empty input is FDSP_EINVAL. */
+typedef struct fdsp_fir_s *fdsp_fir_t;
+int fdsp_fir_create(const fdsp_cf32 *coeffs, uint32_t n_coeffs, fdsp_fir_t *out);
+int fdsp_fir_destroy(fdsp_fir_t h);
+int fdsp_fir_reset(fdsp_fir_t h);
+int fdsp_fir_apply(fdsp_fir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out);
+int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                          fdsp_stream_t stream);
+
+/* ---- FIR bank: many independent channels, real taps each ----
+ * The multi-channel form of falcon_dsp_fir_filter::apply (reference
+ * src/transform/falcon_dsp_fir_filter.cc:177-243) with real taps: channel c
+ * filters d_in + c*in_stride with taps[c*n_taps ...], history carried per
+ * channel. One launch covers all channels. */
+typedef struct fdsp_fir_bank_s *fdsp_fir_bank_t;
+int fdsp_fir_bank_create(uint32_t n_chan, uint32_t n_taps, const float *taps, fdsp_fir_bank_t *out);
+int fdsp_fir_bank_destroy(fdsp_fir_bank_t h);
+int fdsp_fir_bank_reset(fdsp_fir_bank_t h);
+int fdsp_fir_bank_apply_device(fdsp_fir_bank_t h, const fdsp_cf32 *d_in, size_t n, size_t in_stride,
+                               fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream);
+
+/* ---- polar (FM) discriminator ----
+ * Replaces falcon_dsp_polar_discriminator::{apply(cf32), reset_state}
+ * (reference hdr/transform/falcon_dsp_polar_discriminator.h:97-114, src .cc:108-171):
+ * out = atan2(Im z, Re z), z = x[n]*conj(x[n-1]); the first call after a reset
+ * emits n-1 values, later calls n. */
+typedef struct fdsp_discriminator_s *fdsp_discriminator_t;
+int fdsp_discriminator_create(fdsp_discriminator_t *out);
+int fdsp_discriminator_destroy(fdsp_discriminator_t h);
+int fdsp_discriminator_reset(fdsp_discriminator_t h);
+int fdsp_discriminator_apply(fdsp_discriminator_t h, const fdsp_cf32 *in, size_t n, float *out,
+                             size_t *n_out);
+int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fdsp_cf32 *d_in, size_t n, float *d_out,
+                                    size_t *n_out, fdsp_stream_t stream);
+
+/* ---- multi-rate channelizer ----
+ * Replaces falcon_dsp_multi_rate_channelizer{,_cuda}::{initialize, apply, reset_state}
+ * (reference hdr/transform/falcon_dsp_multi_rate_channelizer.h:70-134,
+ * src .cc:81-173; hdr/transform/falcon_dsp_multi_rate_channelizer_cuda.h:102-167,
+ * src/transform/falcon_dsp_multi_rate_channelizer_cuda.cu:87-577). The channel
+ * descriptor mirrors multi_rate_channelizer_channel_s (.h:70-77); the shift is
+ * narrowed to int32 as the reference does.
+ *
+ * FDSP_CHAN_PLAIN keeps the reference's structure on the GPU: one multi-shift
+ * NCO launch writing a full-rate copy per channel, then one resampler launch
+ * per channel. FDSP_CHAN_BATCHED runs every channel in one fused launch that
+ * reads the input once (shift folded into modulated taps, input staged in LDS).
+ * Both produce the reference's samples and counts. */
+#define FDSP_CHAN_PLAIN   0
+#define FDSP_CHAN_BATCHED 1
+
+#define FDSP_TAIL_NONE          0
+#define FDSP_TAIL_DISCRIMINATOR 1 /* channel output is float angles (polar discriminator) */
+
+typedef struct fdsp_channel_desc {
+    uint32_t output_sample_rate_in_sps;
+    int64_t freq_shift_in_hz;
+    uint32_t up_rate;
+    uint32_t down_rate;
+    const fdsp_cf32 *taps;
+    uint32_t n_taps;
+    uint32_t tail; /* FDSP_TAIL_* (batched mode only) */
+} fdsp_channel_desc;
+
+typedef struct fdsp_channelizer_s *fdsp_channelizer_t;
+int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *chans, uint32_t n_chan, int mode,
+                            fdsp_channelizer_t *out);
+int fdsp_channelizer_destroy(fdsp_channelizer_t h);
+int fdsp_channelizer_reset(fdsp_channelizer_t h);
+int fdsp_channelizer_num_channels(fdsp_channelizer_t h, uint32_t *n_chan);
+/* exact count channel `chan` emits for the next in_count inputs */
+int fdsp_channelizer_needed_out_count(fdsp_channelizer_t h, uint32_t chan, uint32_t in_count,
+                                      uint32_t *out_count);
+/* per-channel carried state: NCO samples handled (mod R), resampler
+ * coeff_phase / xOffset, totals since reset */
+int fdsp_channelizer_get_state(fdsp_channelizer_t h, uint32_t chan, uint64_t *nco_samples_handled,
+                               uint32_t *coeff_phase, uint32_t *x_offset, uint64_t *in_total,
+                               uint64_t *out_total);
+/* outs[c] holds caps[c] samples (float for discriminator channels); counts[c]
+ * receives the number written */
+int fdsp_channelizer_apply(fdsp_channelizer_t h, const fdsp_cf32 *in, size_t n, void *const *outs,
+                           const size_t *caps, size_t *counts);
+int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t n,
+                                  void *const *d_outs, const size_t *caps, size_t *counts,
+                                  fdsp_stream_t stream);
+
+/* ---- batched channelizer as a replayable HIP graph ----
+ * Captures one chunk of chunk_len samples from d_in into d_outs (fixed
+ * buffers) as a graph whose last node advances the carried state on the
+ * device, so consecutive launches process consecutive chunks with no host
+ * work. Counts of the last launch are read back with ..._graph_counts. */
+int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_len, const fdsp_cf32 *d_in,
+                                  void *const *d_outs, const size_t *caps);
+int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t stream);
+int fdsp_channelizer_graph_counts(fdsp_channelizer_t h, size_t *counts);
+int fdsp_channelizer_graph_destroy(fdsp_channelizer_t h);
+/* Optional kernel-level timing: events recorded right before and right after
+ * the fused kernel inside the graph, so fdsp_event_elapsed_ms(start, stop)
+ * after a launch is that launch's kernel time. Set before graph_create;
+ * NULLs remove them. */
+int fdsp_channelizer_graph_set_timing_events(fdsp_channelizer_t h, fdsp_event_t start, fdsp_event_t stop);
+
+/* tile parameters of the batched kernel (set by the autotuner) */
+int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads);
+int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDSP_H */

@@ -1,0 +1,70 @@
+"""In-tree build of libfdsp_hip.so (gfx950) and the C++ drop-in library.
+
+    python -m libfalcon_dsp_amd.build
+
+Packed FP32 instructions are switched off for device code
+(-target-feature -packed-fp32-ops): the kernels are written for scalar FP32
+FMAs only.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIP_SOURCES = [os.path.join(CSRC, f) for f in ("fdsp_kernels.hip", "fdsp_capi.hip")]
+HIP_LIB = os.path.join(HERE, "libfdsp_hip.so")
+DROPIN_SOURCES_DIR = os.path.join(HERE, "src")
+DROPIN_LIB = os.path.join(HERE, "libfalcon_dsp.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # hipcc's host pass does not know the device-only feature; that note is noise
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip():
+        print(out)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_hip():
+    _run([HIPCC, *HIP_FLAGS, *HIP_SOURCES, "-o", HIP_LIB])
+
+
+def build_dropin():
+    """C++ drop-in classes (reference class names and signatures) over the C ABI."""
+    if not os.path.isdir(DROPIN_SOURCES_DIR):
+        return
+    srcs = sorted(os.path.join(dp, f) for dp, _, fs in os.walk(DROPIN_SOURCES_DIR) for f in fs if f.endswith(".cc"))
+    if not srcs:
+        return
+    _run(["g++", "-O2", "-std=c++14", "-fPIC", "-shared", "-Wall", f"-I{os.path.join(HERE, 'hdr')}",
+          f"-I{os.path.join(ROOT, 'include')}", *srcs, "-o", DROPIN_LIB, f"-L{HERE}", "-lfdsp_hip",
+          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+
+
+def build_oracle():
+    """The CPU checker (oracle/) and, where the reference sources exist, oracle/_ref."""
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
+
+
+def build_all():
+    build_hip()
+    build_dropin()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

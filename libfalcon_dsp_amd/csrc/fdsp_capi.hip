@@ -1,0 +1,1566 @@
+// fdsp_capi.hip -- the C ABI (include/fdsp.h) over the gfx950 kernels.
+//
+// Handles own their device buffers and carry the reference's stream state in
+// closed form: the only per-stream integers are the inputs consumed since the
+// last reset (p) and the ping-pong parity of the history buffers; coeff_phase,
+// xOffset, output counts and NCO indices are all derived from p exactly
+// (reference src/transform/falcon_dsp_polyphase_resampler.cc:216-228,256-319,
+// src/transform/falcon_dsp_freq_shift.cc:177-189).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fdsp.h"
+#include "fdsp_internal.h"
+
+using namespace fdsp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err(FDSP_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                           __LINE__);                                                       \
+    } while (0)
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// device buffer grown on demand
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want)
+    {
+        if (want <= bytes)
+            return FDSP_OK;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (want == 0)
+            return FDSP_OK;
+        if (hipMalloc(&p, want) != hipSuccess)
+            return set_err(FDSP_ENOMEM, "hipMalloc(%zu) failed", want);
+        bytes = want;
+        return FDSP_OK;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// outputs a chunk of n inputs starting at global input p emits
+uint32_t chunk_count(uint64_t p, uint64_t n, uint32_t up, uint32_t down)
+{
+    return (uint32_t)(ceil_div((p + n) * up, down) - ceil_div(p * up, down));
+}
+
+// reference coeff_phase / xOffset after p inputs
+void resampler_fields(uint64_t p, uint32_t up, uint32_t down, uint32_t *phase, uint32_t *xoff, uint64_t *n_next)
+{
+    const uint64_t nn = ceil_div(p * up, down);
+    const uint64_t num = nn * down;
+    *phase = (uint32_t)(num % up);
+    *xoff = (uint32_t)(num / up - p);
+    *n_next = nn;
+}
+
+float2 phasor64(double revolutions)
+{
+    const double t = revolutions - std::floor(revolutions);
+    return make_float2((float)std::cos(kTwoPi * t), (float)std::sin(kTwoPi * t));
+}
+
+int check_chunk(size_t n, uint32_t up)
+{
+    // chunk-local index arithmetic runs in 32 bits
+    if (n >= (1ull << 31) || (uint64_t)n * up >= (1ull << 32))
+        return set_err(FDSP_ERANGE, "chunk of %zu samples is above the per-call limit (n < 2^31, n*up < 2^32)", n);
+    return FDSP_OK;
+}
+
+hipStream_t pick(hipStream_t own, fdsp_stream_t s) { return s ? static_cast<hipStream_t>(s) : own; }
+
+}  // namespace
+
+// ===========================================================================
+// misc + plumbing
+// ===========================================================================
+extern "C" const char *fdsp_last_error(void) { return g_err.c_str(); }
+extern "C" int fdsp_version(void) { return 100; }
+
+extern "C" int fdsp_device_count(int *count)
+{
+    if (!count)
+        return set_err(FDSP_EINVAL, "null count");
+    HIP_TRY(hipGetDeviceCount(count));
+    return FDSP_OK;
+}
+extern "C" int fdsp_set_device(int device)
+{
+    HIP_TRY(hipSetDevice(device));
+    return FDSP_OK;
+}
+extern "C" int fdsp_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr)
+        return set_err(FDSP_EINVAL, "null ptr");
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess)
+        return set_err(FDSP_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    return FDSP_OK;
+}
+extern "C" int fdsp_free(void *ptr)
+{
+    if (ptr)
+        HIP_TRY(hipFree(ptr));
+    return FDSP_OK;
+}
+extern "C" int fdsp_memcpy_h2d(void *dst, const void *src, size_t bytes, fdsp_stream_t s)
+{
+    if (s)
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)s));
+    else
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return FDSP_OK;
+}
+extern "C" int fdsp_memcpy_d2h(void *dst, const void *src, size_t bytes, fdsp_stream_t s)
+{
+    if (s)
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
+    else
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return FDSP_OK;
+}
+extern "C" int fdsp_memcpy_d2d(void *dst, const void *src, size_t bytes, fdsp_stream_t s)
+{
+    if (s)
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    else
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return FDSP_OK;
+}
+extern "C" int fdsp_memset(void *dst, int value, size_t bytes, fdsp_stream_t s)
+{
+    if (s)
+        HIP_TRY(hipMemsetAsync(dst, value, bytes, (hipStream_t)s));
+    else
+        HIP_TRY(hipMemset(dst, value, bytes));
+    return FDSP_OK;
+}
+extern "C" int fdsp_host_alloc_pinned(void **ptr, size_t bytes)
+{
+    if (!ptr)
+        return set_err(FDSP_EINVAL, "null ptr");
+    if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+        return set_err(FDSP_ENOMEM, "hipHostMalloc(%zu) failed", bytes);
+    return FDSP_OK;
+}
+extern "C" int fdsp_host_free_pinned(void *ptr)
+{
+    if (ptr)
+        HIP_TRY(hipHostFree(ptr));
+    return FDSP_OK;
+}
+extern "C" int fdsp_stream_create(fdsp_stream_t *stream)
+{
+    if (!stream)
+        return set_err(FDSP_EINVAL, "null stream");
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return FDSP_OK;
+}
+extern "C" int fdsp_stream_destroy(fdsp_stream_t stream)
+{
+    if (stream)
+        HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return FDSP_OK;
+}
+extern "C" int fdsp_stream_sync(fdsp_stream_t stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return FDSP_OK;
+}
+extern "C" int fdsp_device_sync(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return FDSP_OK;
+}
+extern "C" int fdsp_event_create(fdsp_event_t *ev)
+{
+    if (!ev)
+        return set_err(FDSP_EINVAL, "null event");
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    *ev = e;
+    return FDSP_OK;
+}
+extern "C" int fdsp_event_destroy(fdsp_event_t ev)
+{
+    if (ev)
+        HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return FDSP_OK;
+}
+extern "C" int fdsp_event_record(fdsp_event_t ev, fdsp_stream_t stream)
+{
+    HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return FDSP_OK;
+}
+extern "C" int fdsp_event_sync(fdsp_event_t ev)
+{
+    HIP_TRY(hipEventSynchronize((hipEvent_t)ev));
+    return FDSP_OK;
+}
+extern "C" int fdsp_event_elapsed_ms(fdsp_event_t a, fdsp_event_t b, float *ms)
+{
+    if (!ms)
+        return set_err(FDSP_EINVAL, "null ms");
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return FDSP_OK;
+}
+
+// ===========================================================================
+// rate helpers (reference src/utilities/falcon_dsp_utils.cc:123-137,231-294;
+// src/transform/falcon_dsp_freq_shift.cc:194-235)
+// ===========================================================================
+extern "C" uint64_t fdsp_gcd(uint64_t a, uint64_t b)
+{
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+extern "C" uint64_t fdsp_lcm(uint64_t a, uint64_t b)
+{
+    const uint64_t g = fdsp_gcd(a, b);
+    return g ? (a / g) * b : 0;
+}
+
+// Best rational approximation num/denom of f with denom < max_denom, by the
+// continued-fraction expansion of f's binary expansion; when the next
+// convergent would reach the bound, the largest admissible semiconvergent is
+// taken if it is at least half the next partial quotient.
+extern "C" void fdsp_rat_approx(double f, int64_t max_denom, int64_t *num, int64_t *denom)
+{
+    if (!num || !denom)
+        return;
+    if (max_denom <= 1) {
+        *denom = 1;
+        *num = (int64_t)f;
+        return;
+    }
+    const bool neg = f < 0;
+    if (neg)
+        f = -f;
+    int64_t scale = 1;  // f = d / scale exactly, scale a power of two
+    while (f != std::floor(f)) {
+        scale <<= 1;
+        f *= 2;
+    }
+    int64_t d = (int64_t)f, n = scale;
+    int64_t hm2 = 0, hm1 = 1;  // numerators of the last two convergents
+    int64_t km2 = 1, km1 = 0;  // denominators
+    for (int i = 0; i < 64; ++i) {
+        const int64_t a = n ? d / n : 0;
+        if (i && !a)
+            break;
+        const int64_t rem = n ? d % n : 0;
+        d = n;
+        n = rem;
+        int64_t x = a;
+        bool last = false;
+        if (km1 * a + km2 >= max_denom) {
+            x = (max_denom - km2) / km1;
+            if (x * 2 >= a || km1 >= max_denom)
+                last = true;
+            else
+                break;
+        }
+        const int64_t h = x * hm1 + hm2, k = x * km1 + km2;
+        hm2 = hm1;
+        hm1 = h;
+        km2 = km1;
+        km1 = k;
+        if (last)
+            break;
+    }
+    *denom = km1;
+    *num = neg ? -hm1 : hm1;
+}
+
+extern "C" int fdsp_freq_shift_params(uint32_t fs, int32_t f_hz, uint32_t *rollover, double *omega)
+{
+    if (!rollover || !omega || fs == 0)
+        return set_err(FDSP_EINVAL, "fs must be nonzero and outputs non-null");
+    if (f_hz == 0) {
+        *rollover = 1;
+        *omega = 0.0;
+        return FDSP_OK;
+    }
+    // the period fs/|f| in samples, scaled by ten until it is at least 1e5
+    double r = (double)fs / std::fabs((double)f_hz);
+    while (r < 1e5)
+        r *= 10;
+    *rollover = (uint32_t)std::llround(r);
+    *omega = ((double)f_hz / (double)fs) * 2.0 * M_PI;
+    return FDSP_OK;
+}
+
+namespace {
+// per-shift NCO parameters shared by the freq-shift and plain channelizer paths
+NcoChan make_nco(uint32_t fs, int32_t f)
+{
+    NcoChan c{};
+    uint32_t r = 1;
+    double w = 0;
+    fdsp_freq_shift_params(fs, f, &r, &w);
+    if (f == 0) {
+        c.nu = 0.0;
+        c.r = 0xFFFFFFFFu;  // identity shift: the index never matters
+    } else {
+        c.nu = (double)f / (double)fs;
+        c.r = r;
+    }
+    c.inv_r = 1.0 / (double)c.r;
+    for (int i = 0; i < 4; ++i)
+        c.step[i] = phasor64(c.nu * i);
+    return c;
+}
+}  // namespace
+
+// ===========================================================================
+// NCO frequency shift
+// ===========================================================================
+struct fdsp_freq_shift_s {
+    uint32_t fs = 0;
+    std::vector<NcoChan> chans;
+    DevBuf d_chans, d_in, d_out;
+    uint64_t start = 0;  // samples since reset (the NCO index is start mod R)
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int fdsp_freq_shift_create(uint32_t fs, const int32_t *shifts_hz, uint32_t n_shifts,
+                                      fdsp_freq_shift_t *out)
+{
+    if (!out || !shifts_hz || n_shifts == 0 || fs == 0)
+        return set_err(FDSP_EINVAL, "freq_shift_create: need fs > 0 and at least one shift");
+    auto *h = new (std::nothrow) fdsp_freq_shift_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->fs = fs;
+    for (uint32_t i = 0; i < n_shifts; ++i)
+        h->chans.push_back(make_nco(fs, shifts_hz[i]));
+    int rc = h->d_chans.ensure(sizeof(NcoChan) * n_shifts);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMemcpy(h->d_chans.p, h->chans.data(), sizeof(NcoChan) * n_shifts, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        h->d_chans.release();
+        delete h;
+        return set_err(FDSP_EHIP, "freq_shift_create: HIP setup failed");
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_freq_shift_destroy(fdsp_freq_shift_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    (void)hipStreamSynchronize(h->stream);
+    h->d_chans.release();
+    h->d_in.release();
+    h->d_out.release();
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_freq_shift_reset(fdsp_freq_shift_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    h->start = 0;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_freq_shift_get_state(fdsp_freq_shift_t h, uint32_t chan, uint64_t *samples_handled)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (chan >= h->chans.size() || !samples_handled)
+        return set_err(FDSP_EINVAL, "bad channel");
+    *samples_handled = h->chans[chan].nu == 0.0 ? 0 : h->start % h->chans[chan].r;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                            size_t out_stride, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!d_in || !d_out || (h->chans.size() > 1 && out_stride < n))
+        return set_err(FDSP_EINVAL, "bad buffers");
+    if (int rc = check_chunk(n, 1))
+        return rc;
+    HIP_TRY(launch_nco_multi(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, h->start,
+                             h->d_chans.as<NcoChan>(), (uint32_t)h->chans.size(), reinterpret_cast<float2 *>(d_out),
+                             out_stride, pick(h->stream, stream)));
+    h->start += n;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *const *outs)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!in || !outs)
+        return set_err(FDSP_EINVAL, "null buffers");
+    const size_t C = h->chans.size();
+    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+        return rc;
+    if (int rc = h->d_out.ensure(n * C * sizeof(float2)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    if (int rc = fdsp_freq_shift_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
+                                              h->d_out.as<fdsp_cf32>(), n, h->stream))
+        return rc;
+    for (size_t c = 0; c < C; ++c)
+        HIP_TRY(hipMemcpyAsync(outs[c], h->d_out.as<float2>() + c * n, n * sizeof(float2), hipMemcpyDeviceToHost,
+                               h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+// ===========================================================================
+// polyphase resampler
+// ===========================================================================
+struct fdsp_resampler_s {
+    uint32_t up = 1, down = 1, cpp = 1;
+    DevBuf d_taps, d_hist[2], d_in, d_out;
+    uint32_t parity = 0;
+    uint64_t p = 0;  // inputs consumed since reset
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+// transposed, flipped real bank: bank[ph*cpp + (cpp-1-j)] = Re h[j*up + ph]
+// (reference src/transform/falcon_dsp_polyphase_resampler.cc:131-157)
+std::vector<float> real_bank(const fdsp_cf32 *taps, uint32_t n_taps, uint32_t up, uint32_t *cpp_out)
+{
+    uint32_t padded = n_taps;
+    while (padded % up)
+        ++padded;
+    const uint32_t cpp = padded / up;
+    std::vector<float> bank((size_t)up * cpp, 0.f);
+    for (uint32_t ph = 0; ph < up; ++ph)
+        for (uint32_t j = 0; j < cpp; ++j)
+            if ((size_t)j * up + ph < n_taps)
+                bank[(size_t)ph * cpp + (cpp - 1 - j)] = taps[(size_t)j * up + ph].re;
+    *cpp_out = cpp;
+    return bank;
+}
+}  // namespace
+
+extern "C" int fdsp_resampler_create(uint32_t up, uint32_t down, const fdsp_cf32 *taps, uint32_t n_taps,
+                                     fdsp_resampler_t *out)
+{
+    if (!out || !taps || up == 0 || down == 0 || n_taps == 0)
+        return set_err(FDSP_EINVAL, "resampler_create: need up, down > 0 and taps");
+    auto *h = new (std::nothrow) fdsp_resampler_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->up = up;
+    h->down = down;
+    auto bank = real_bank(taps, n_taps, up, &h->cpp);
+    const size_t hbytes = (size_t)(h->cpp > 1 ? h->cpp - 1 : 1) * sizeof(float2);
+    int rc = h->d_taps.ensure(bank.size() * sizeof(float));
+    if (!rc)
+        rc = h->d_hist[0].ensure(hbytes);
+    if (!rc)
+        rc = h->d_hist[1].ensure(hbytes);
+    if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipMemcpy(h->d_taps.p, bank.data(), bank.size() * sizeof(float), hipMemcpyHostToDevice) !=
+                    hipSuccess ||
+                hipMemset(h->d_hist[0].p, 0, hbytes) != hipSuccess ||
+                hipMemset(h->d_hist[1].p, 0, hbytes) != hipSuccess))
+        rc = set_err(FDSP_EHIP, "resampler_create: HIP setup failed");
+    if (rc) {
+        fdsp_resampler_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_destroy(fdsp_resampler_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    h->d_taps.release();
+    h->d_hist[0].release();
+    h->d_hist[1].release();
+    h->d_in.release();
+    h->d_out.release();
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_reset(fdsp_resampler_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    h->p = 0;
+    h->parity = 0;
+    HIP_TRY(hipMemsetAsync(h->d_hist[0].p, 0, h->d_hist[0].bytes, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_needed_out_count(fdsp_resampler_t h, uint32_t in_count, uint32_t *out_count)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!out_count)
+        return set_err(FDSP_EINVAL, "null output");
+    *out_count = chunk_count(h->p, in_count, h->up, h->down);
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_coeffs_per_phase(fdsp_resampler_t h, uint32_t *cpp)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!cpp)
+        return set_err(FDSP_EINVAL, "null output");
+    *cpp = h->cpp;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_get_state(fdsp_resampler_t h, uint32_t *coeff_phase, uint32_t *x_offset,
+                                        uint64_t *in_total, uint64_t *out_total)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    uint32_t ph, xo;
+    uint64_t nn;
+    resampler_fields(h->p, h->up, h->down, &ph, &xo, &nn);
+    if (coeff_phase)
+        *coeff_phase = ph;
+    if (x_offset)
+        *x_offset = xo;
+    if (in_total)
+        *in_total = h->p;
+    if (out_total)
+        *out_total = nn;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_apply_device(fdsp_resampler_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                           size_t cap, size_t *n_out, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!n_out)
+        return set_err(FDSP_EINVAL, "null n_out");
+    *n_out = 0;
+    if (n == 0)
+        return FDSP_OK;  // the reference emits nothing and keeps its state
+    if (!d_in)
+        return set_err(FDSP_EINVAL, "null input");
+    if (int rc = check_chunk(n, h->up))
+        return rc;
+    const uint32_t count = chunk_count(h->p, n, h->up, h->down);
+    if (count > cap || (count && !d_out))
+        return set_err(FDSP_ERANGE, "output capacity %zu below the %u samples this chunk emits", cap, count);
+    HIP_TRY(launch_resampler(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, h->p, h->up, h->down, h->cpp,
+                             h->d_taps.as<float>(), h->d_hist[h->parity].as<float2>(),
+                             h->d_hist[h->parity ^ 1].as<float2>(), reinterpret_cast<float2 *>(d_out), count,
+                             pick(h->stream, stream)));
+    h->parity ^= 1;
+    h->p += n;
+    *n_out = count;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_apply(fdsp_resampler_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out, size_t cap,
+                                    size_t *n_out)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!n_out)
+        return set_err(FDSP_EINVAL, "null n_out");
+    *n_out = 0;
+    if (n == 0)
+        return FDSP_OK;
+    if (!in)
+        return set_err(FDSP_EINVAL, "null input");
+    if (int rc = check_chunk(n, h->up))
+        return rc;
+    const uint32_t count = chunk_count(h->p, n, h->up, h->down);
+    if (count > cap || (count && !out))
+        return set_err(FDSP_ERANGE, "output capacity %zu below the %u samples this chunk emits", cap, count);
+    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+        return rc;
+    if (int rc = h->d_out.ensure((size_t)(count ? count : 1) * sizeof(float2)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    size_t got = 0;
+    if (int rc = fdsp_resampler_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
+                                             h->d_out.as<fdsp_cf32>(), count, &got, h->stream))
+        return rc;
+    if (got)
+        HIP_TRY(hipMemcpyAsync(out, h->d_out.p, got * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *n_out = got;
+    return FDSP_OK;
+}
+
+// ===========================================================================
+// FIR (complex taps)
+// ===========================================================================
+struct fdsp_fir_s {
+    uint32_t m = 0;
+    DevBuf d_taps, d_hist[2], d_in, d_out;
+    uint32_t parity = 0;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int fdsp_fir_create(const fdsp_cf32 *coeffs, uint32_t n_coeffs, fdsp_fir_t *out)
+{
+    if (!out || !coeffs || n_coeffs == 0)
+        return set_err(FDSP_EINVAL, "fir_create: need at least one coefficient");
+    auto *h = new (std::nothrow) fdsp_fir_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->m = n_coeffs;
+    const size_t hbytes = (size_t)(n_coeffs > 1 ? n_coeffs - 1 : 1) * sizeof(float2);
+    int rc = h->d_taps.ensure(n_coeffs * sizeof(float2));
+    if (!rc)
+        rc = h->d_hist[0].ensure(hbytes);
+    if (!rc)
+        rc = h->d_hist[1].ensure(hbytes);
+    if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipMemcpy(h->d_taps.p, coeffs, n_coeffs * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(h->d_hist[0].p, 0, hbytes) != hipSuccess ||
+                hipMemset(h->d_hist[1].p, 0, hbytes) != hipSuccess))
+        rc = set_err(FDSP_EHIP, "fir_create: HIP setup failed");
+    if (rc) {
+        fdsp_fir_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_destroy(fdsp_fir_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    h->d_taps.release();
+    h->d_hist[0].release();
+    h->d_hist[1].release();
+    h->d_in.release();
+    h->d_out.release();
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_reset(fdsp_fir_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    h->parity = 0;
+    HIP_TRY(hipMemsetAsync(h->d_hist[0].p, 0, h->d_hist[0].bytes, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                     fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!d_in || !d_out)
+        return set_err(FDSP_EINVAL, "null buffers");
+    if (int rc = check_chunk(n, 1))
+        return rc;
+    hipError_t e = launch_fir(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, h->d_taps.as<float2>(), h->m,
+                              h->d_hist[h->parity].as<float2>(), h->d_hist[h->parity ^ 1].as<float2>(),
+                              reinterpret_cast<float2 *>(d_out), pick(h->stream, stream));
+    if (e == hipErrorInvalidValue)
+        return set_err(FDSP_ERANGE, "fir: %u taps exceed the LDS tile", h->m);
+    HIP_TRY(e);
+    h->parity ^= 1;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_apply(fdsp_fir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!in || !out)
+        return set_err(FDSP_EINVAL, "null buffers");
+    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+        return rc;
+    if (int rc = h->d_out.ensure(n * sizeof(float2)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    if (int rc = fdsp_fir_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
+                                       h->d_out.as<fdsp_cf32>(), h->stream))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, h->d_out.p, n * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+// ===========================================================================
+// FIR bank (real taps)
+// ===========================================================================
+struct fdsp_fir_bank_s {
+    uint32_t n_chan = 0, m = 0;
+    DevBuf d_taps, d_hist[2];
+    uint32_t parity = 0;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int fdsp_fir_bank_create(uint32_t n_chan, uint32_t n_taps, const float *taps, fdsp_fir_bank_t *out)
+{
+    if (!out || !taps || n_chan == 0 || n_taps == 0 || n_chan > 65535)
+        return set_err(FDSP_EINVAL, "fir_bank_create: need 1..65535 channels and taps");
+    auto *h = new (std::nothrow) fdsp_fir_bank_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->n_chan = n_chan;
+    h->m = n_taps;
+    const size_t hbytes = (size_t)n_chan * (n_taps > 1 ? n_taps - 1 : 1) * sizeof(float2);
+    int rc = h->d_taps.ensure((size_t)n_chan * n_taps * sizeof(float));
+    if (!rc)
+        rc = h->d_hist[0].ensure(hbytes);
+    if (!rc)
+        rc = h->d_hist[1].ensure(hbytes);
+    if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipMemcpy(h->d_taps.p, taps, (size_t)n_chan * n_taps * sizeof(float), hipMemcpyHostToDevice) !=
+                    hipSuccess ||
+                hipMemset(h->d_hist[0].p, 0, hbytes) != hipSuccess ||
+                hipMemset(h->d_hist[1].p, 0, hbytes) != hipSuccess))
+        rc = set_err(FDSP_EHIP, "fir_bank_create: HIP setup failed");
+    if (rc) {
+        fdsp_fir_bank_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_bank_destroy(fdsp_fir_bank_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    h->d_taps.release();
+    h->d_hist[0].release();
+    h->d_hist[1].release();
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_bank_reset(fdsp_fir_bank_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    h->parity = 0;
+    HIP_TRY(hipMemsetAsync(h->d_hist[0].p, 0, h->d_hist[0].bytes, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_fir_bank_apply_device(fdsp_fir_bank_t h, const fdsp_cf32 *d_in, size_t n, size_t in_stride,
+                                          fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!d_in || !d_out || (h->n_chan > 1 && (in_stride < n || out_stride < n)))
+        return set_err(FDSP_EINVAL, "bad buffers or strides");
+    if (int rc = check_chunk(n, 1))
+        return rc;
+    hipError_t e = launch_fir_bank(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, in_stride,
+                                   h->d_taps.as<float>(), h->m, h->n_chan, h->d_hist[h->parity].as<float2>(),
+                                   h->d_hist[h->parity ^ 1].as<float2>(), reinterpret_cast<float2 *>(d_out),
+                                   out_stride, pick(h->stream, stream));
+    if (e == hipErrorInvalidValue)
+        return set_err(FDSP_ERANGE, "fir_bank: %u taps exceed the LDS tile", h->m);
+    HIP_TRY(e);
+    h->parity ^= 1;
+    return FDSP_OK;
+}
+
+// ===========================================================================
+// polar discriminator
+// ===========================================================================
+struct fdsp_discriminator_s {
+    DevBuf d_prev[2], d_in, d_out;
+    uint32_t parity = 0;
+    bool has_prev = false;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int fdsp_discriminator_create(fdsp_discriminator_t *out)
+{
+    if (!out)
+        return set_err(FDSP_EINVAL, "null output");
+    auto *h = new (std::nothrow) fdsp_discriminator_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    int rc = h->d_prev[0].ensure(sizeof(float2));
+    if (!rc)
+        rc = h->d_prev[1].ensure(sizeof(float2));
+    if (!rc && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = set_err(FDSP_EHIP, "stream creation failed");
+    if (rc) {
+        fdsp_discriminator_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_discriminator_destroy(fdsp_discriminator_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    h->d_prev[0].release();
+    h->d_prev[1].release();
+    h->d_in.release();
+    h->d_out.release();
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_discriminator_reset(fdsp_discriminator_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    h->has_prev = false;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fdsp_cf32 *d_in, size_t n, float *d_out,
+                                               size_t *n_out, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!n_out)
+        return set_err(FDSP_EINVAL, "null n_out");
+    *n_out = 0;
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!d_in || !d_out)
+        return set_err(FDSP_EINVAL, "null buffers");
+    if (int rc = check_chunk(n, 1))
+        return rc;
+    HIP_TRY(launch_discriminator(reinterpret_cast<const float2 *>(d_in), (uint32_t)n,
+                                 h->d_prev[h->parity].as<float2>(), h->has_prev ? 1 : 0,
+                                 h->d_prev[h->parity ^ 1].as<float2>(), d_out, pick(h->stream, stream)));
+    *n_out = h->has_prev ? n : n - 1;
+    h->has_prev = true;
+    h->parity ^= 1;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_discriminator_apply(fdsp_discriminator_t h, const fdsp_cf32 *in, size_t n, float *out,
+                                        size_t *n_out)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!n_out)
+        return set_err(FDSP_EINVAL, "null n_out");
+    *n_out = 0;
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!in || !out)
+        return set_err(FDSP_EINVAL, "null buffers");
+    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+        return rc;
+    if (int rc = h->d_out.ensure(n * sizeof(float)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    size_t got = 0;
+    if (int rc = fdsp_discriminator_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
+                                                 h->d_out.as<float>(), &got, h->stream))
+        return rc;
+    if (got)
+        HIP_TRY(hipMemcpyAsync(out, h->d_out.p, got * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *n_out = got;
+    return FDSP_OK;
+}
+
+// ===========================================================================
+// multi-rate channelizer
+// ===========================================================================
+namespace {
+struct ChanHost {
+    int32_t shift;
+    uint32_t out_rate, up, down, cpp, tail;
+    NcoChan nco;
+    std::vector<float> bank;    // real transposed bank (plain mode)
+    DevBuf d_bank;              // plain mode
+    DevBuf d_hist[2];           // plain mode: shifted-sample history
+    DevBuf d_scratch;           // complex outputs before the discriminator tail
+    DevBuf d_prev[2];           // discriminator previous sample
+    uint32_t prev_parity = 0;
+    bool has_prev = false;
+};
+}  // namespace
+
+struct fdsp_channelizer_s {
+    uint32_t fs = 0;
+    int mode = FDSP_CHAN_BATCHED;
+    std::vector<ChanHost> ch;
+    uint64_t p = 0;             // inputs consumed since reset
+    uint32_t parity = 0;        // plain-mode history parity (all channels together)
+    hipStream_t stream = nullptr;
+    // plain mode
+    DevBuf d_nco, d_full;       // NCO params, full-rate shifted copies (C x n)
+    // batched mode
+    std::vector<ChanStatic> stat;
+    DevBuf d_stat, d_taps, d_hist[2], d_pos;
+    ChanStatic *pinned_stat = nullptr;
+    hipEvent_t stat_ev = nullptr;   // last table upload
+    hipEvent_t use_ev = nullptr;    // last fused launch that read the table
+    bool stat_valid = false;
+    uint32_t taps_total = 0, hist_len = 0;
+    uint32_t seg_len = 4096, block = 256;
+    // host-pointer path
+    DevBuf d_in;
+    std::vector<DevBuf> d_outs;
+    // graph mode
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    size_t graph_chunk = 0;
+    std::vector<size_t> graph_counts;
+    hipEvent_t time_start = nullptr, time_stop = nullptr;
+};
+
+namespace {
+
+int chan_upload_outs(fdsp_channelizer_s *h, void *const *outs)
+{
+    // batched mode: output pointers live in the device channel table
+    bool same = h->stat_valid;
+    for (size_t c = 0; c < h->ch.size(); ++c) {
+        void *o = h->ch[c].tail ? h->ch[c].d_scratch.p : outs[c];
+        if (h->stat[c].out != o) {
+            h->stat[c].out = o;
+            same = false;
+        }
+    }
+    if (same)
+        return FDSP_OK;
+    // the pinned staging copy may still be in flight from the previous upload,
+    // and a launch on another stream may still be reading the device table
+    HIP_TRY(hipEventSynchronize(h->stat_ev));
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->use_ev, 0));
+    std::memcpy(h->pinned_stat, h->stat.data(), sizeof(ChanStatic) * h->stat.size());
+    HIP_TRY(hipMemcpyAsync(h->d_stat.p, h->pinned_stat, sizeof(ChanStatic) * h->stat.size(),
+                           hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipEventRecord(h->stat_ev, h->stream));
+    h->stat_valid = true;
+    return FDSP_OK;
+}
+
+uint32_t fit_seg_len(const fdsp_channelizer_s *h, uint32_t want)
+{
+    uint32_t s = want;
+    while (s > 256 && chan_fused_lds_bytes((uint32_t)h->ch.size(), h->taps_total, s, h->hist_len) > 64 * 1024)
+        s >>= 1;
+    return s;
+}
+
+}  // namespace
+
+extern "C" int fdsp_channelizer_destroy(fdsp_channelizer_t h);
+
+extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *chans, uint32_t n_chan, int mode,
+                                       fdsp_channelizer_t *out)
+{
+    if (!out || !chans || n_chan == 0 || fs == 0)
+        return set_err(FDSP_EINVAL, "channelizer_create: need fs > 0 and at least one channel");
+    if (mode != FDSP_CHAN_PLAIN && mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "unknown channelizer mode %d", mode);
+    for (uint32_t c = 0; c < n_chan; ++c)
+        if (chans[c].up_rate == 0 || chans[c].down_rate == 0 || !chans[c].taps || chans[c].n_taps == 0 ||
+            chans[c].tail > FDSP_TAIL_DISCRIMINATOR)
+            return set_err(FDSP_EINVAL, "channel %u: need up, down > 0, taps and a known tail", c);
+    auto *h = new (std::nothrow) fdsp_channelizer_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->fs = fs;
+    h->mode = mode;
+    h->ch.resize(n_chan);
+    int rc = FDSP_OK;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = set_err(FDSP_EHIP, "stream creation failed");
+    uint32_t max_cpp = 1;
+    for (uint32_t c = 0; c < n_chan && !rc; ++c) {
+        ChanHost &k = h->ch[c];
+        k.shift = (int32_t)chans[c].freq_shift_in_hz;  // narrowed as the reference does
+        k.out_rate = chans[c].output_sample_rate_in_sps;
+        k.up = chans[c].up_rate;
+        k.down = chans[c].down_rate;
+        k.tail = chans[c].tail;
+        k.nco = make_nco(fs, k.shift);
+        k.bank = real_bank(chans[c].taps, chans[c].n_taps, k.up, &k.cpp);
+        max_cpp = std::max(max_cpp, k.cpp);
+        if (k.tail) {
+            rc = k.d_prev[0].ensure(sizeof(float2));
+            if (!rc)
+                rc = k.d_prev[1].ensure(sizeof(float2));
+        }
+        if (!rc && mode == FDSP_CHAN_PLAIN) {
+            const size_t hb = (size_t)(k.cpp > 1 ? k.cpp - 1 : 1) * sizeof(float2);
+            rc = k.d_bank.ensure(k.bank.size() * sizeof(float));
+            if (!rc)
+                rc = k.d_hist[0].ensure(hb);
+            if (!rc)
+                rc = k.d_hist[1].ensure(hb);
+            if (!rc && (hipMemcpy(k.d_bank.p, k.bank.data(), k.bank.size() * sizeof(float),
+                                  hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemset(k.d_hist[0].p, 0, hb) != hipSuccess ||
+                        hipMemset(k.d_hist[1].p, 0, hb) != hipSuccess))
+                rc = set_err(FDSP_EHIP, "channel %u: HIP setup failed", c);
+        }
+    }
+    if (!rc && mode == FDSP_CHAN_PLAIN) {
+        std::vector<NcoChan> nco;
+        for (auto &k : h->ch)
+            nco.push_back(k.nco);
+        rc = h->d_nco.ensure(sizeof(NcoChan) * n_chan);
+        if (!rc && hipMemcpy(h->d_nco.p, nco.data(), sizeof(NcoChan) * n_chan, hipMemcpyHostToDevice) != hipSuccess)
+            rc = set_err(FDSP_EHIP, "NCO table upload failed");
+    }
+    if (!rc && mode == FDSP_CHAN_BATCHED) {
+        // modulated taps: g[ph][t] = Re(bank[ph][t]) * e^{-j 2 pi nu d}, d = cpp-1-t
+        std::vector<float2> taps;
+        h->stat.resize(n_chan);
+        for (uint32_t c = 0; c < n_chan; ++c) {
+            ChanHost &k = h->ch[c];
+            ChanStatic &s = h->stat[c];
+            s.nu = k.nco.nu;
+            s.r = k.nco.r;
+            s.inv_r = k.nco.inv_r;
+            s.up = k.up;
+            s.down = k.down;
+            s.cpp = k.cpp;
+            s.taps_off = (uint32_t)taps.size();
+            s.tail = k.tail;
+            s.wrap = phasor64(s.nu * (double)s.r);
+            s.out = nullptr;
+            for (uint32_t ph = 0; ph < k.up; ++ph)
+                for (uint32_t t = 0; t < k.cpp; ++t) {
+                    const double d = (double)(k.cpp - 1 - t);
+                    const double g = k.bank[(size_t)ph * k.cpp + t];
+                    const double rev = -s.nu * d;
+                    const double tt = rev - std::floor(rev);
+                    taps.push_back(make_float2((float)(g * std::cos(kTwoPi * tt)), (float)(g * std::sin(kTwoPi * tt))));
+                }
+        }
+        h->taps_total = (uint32_t)taps.size();
+        h->hist_len = (max_cpp - 1 + 1) & ~1u;  // even, so staged segments stay 16-byte aligned
+        h->seg_len = fit_seg_len(h, 4096);
+        if (chan_fused_lds_bytes(n_chan, h->taps_total, h->seg_len, h->hist_len) > 64 * 1024)
+            rc = set_err(FDSP_ERANGE, "batched channelizer: %u channels with %u taps in total do not fit one "
+                                      "block's LDS; use FDSP_CHAN_PLAIN",
+                         n_chan, h->taps_total);
+        const size_t hb = (size_t)(h->hist_len ? h->hist_len : 1) * sizeof(float2);
+        if (!rc)
+            rc = h->d_taps.ensure(taps.size() * sizeof(float2));
+        if (!rc)
+            rc = h->d_stat.ensure(sizeof(ChanStatic) * n_chan);
+        if (!rc)
+            rc = h->d_hist[0].ensure(hb);
+        if (!rc)
+            rc = h->d_hist[1].ensure(hb);
+        if (!rc)
+            rc = h->d_pos.ensure(sizeof(ChanPos));
+        if (!rc && (hipMemcpy(h->d_taps.p, taps.data(), taps.size() * sizeof(float2), hipMemcpyHostToDevice) !=
+                        hipSuccess ||
+                    hipMemset(h->d_hist[0].p, 0, hb) != hipSuccess || hipMemset(h->d_hist[1].p, 0, hb) != hipSuccess ||
+                    hipMemset(h->d_pos.p, 0, sizeof(ChanPos)) != hipSuccess ||
+                    hipHostMalloc((void **)&h->pinned_stat, sizeof(ChanStatic) * n_chan, hipHostMallocDefault) !=
+                        hipSuccess ||
+                    hipEventCreateWithFlags(&h->stat_ev, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&h->use_ev, hipEventDisableTiming) != hipSuccess ||
+                    hipEventRecord(h->stat_ev, h->stream) != hipSuccess ||
+                    hipEventRecord(h->use_ev, h->stream) != hipSuccess))
+            rc = set_err(FDSP_EHIP, "batched channelizer: HIP setup failed");
+    }
+    if (rc) {
+        fdsp_channelizer_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_graph_destroy(fdsp_channelizer_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->graph_exec)
+        (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph)
+        (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_chunk = 0;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_destroy(fdsp_channelizer_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    fdsp_channelizer_graph_destroy(h);
+    for (auto &k : h->ch) {
+        k.d_bank.release();
+        k.d_hist[0].release();
+        k.d_hist[1].release();
+        k.d_scratch.release();
+        k.d_prev[0].release();
+        k.d_prev[1].release();
+    }
+    h->d_nco.release();
+    h->d_full.release();
+    h->d_stat.release();
+    h->d_taps.release();
+    h->d_hist[0].release();
+    h->d_hist[1].release();
+    h->d_pos.release();
+    h->d_in.release();
+    for (auto &b : h->d_outs)
+        b.release();
+    if (h->pinned_stat)
+        (void)hipHostFree(h->pinned_stat);
+    if (h->stat_ev)
+        (void)hipEventDestroy(h->stat_ev);
+    if (h->use_ev)
+        (void)hipEventDestroy(h->use_ev);
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_reset(fdsp_channelizer_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    h->p = 0;
+    h->parity = 0;
+    for (auto &k : h->ch) {
+        k.has_prev = false;
+        k.prev_parity = 0;
+        if (h->mode == FDSP_CHAN_PLAIN)
+            HIP_TRY(hipMemsetAsync(k.d_hist[0].p, 0, k.d_hist[0].bytes, h->stream));
+    }
+    if (h->mode == FDSP_CHAN_BATCHED) {
+        HIP_TRY(hipMemsetAsync(h->d_hist[0].p, 0, h->d_hist[0].bytes, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_hist[1].p, 0, h->d_hist[1].bytes, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_pos.p, 0, sizeof(ChanPos), h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_num_channels(fdsp_channelizer_t h, uint32_t *n_chan)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!n_chan)
+        return set_err(FDSP_EINVAL, "null output");
+    *n_chan = (uint32_t)h->ch.size();
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_needed_out_count(fdsp_channelizer_t h, uint32_t chan, uint32_t in_count,
+                                                 uint32_t *out_count)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (chan >= h->ch.size() || !out_count)
+        return set_err(FDSP_EINVAL, "bad channel");
+    const ChanHost &k = h->ch[chan];
+    uint32_t cnt = chunk_count(h->p, in_count, k.up, k.down);
+    if (k.tail && cnt && !k.has_prev)
+        --cnt;
+    *out_count = cnt;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_get_state(fdsp_channelizer_t h, uint32_t chan, uint64_t *nco_samples_handled,
+                                          uint32_t *coeff_phase, uint32_t *x_offset, uint64_t *in_total,
+                                          uint64_t *out_total)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (chan >= h->ch.size())
+        return set_err(FDSP_EINVAL, "bad channel");
+    const ChanHost &k = h->ch[chan];
+    uint32_t ph, xo;
+    uint64_t nn;
+    resampler_fields(h->p, k.up, k.down, &ph, &xo, &nn);
+    if (nco_samples_handled)
+        *nco_samples_handled = k.nco.nu == 0.0 ? 0 : h->p % k.nco.r;
+    if (coeff_phase)
+        *coeff_phase = ph;
+    if (x_offset)
+        *x_offset = xo;
+    if (in_total)
+        *in_total = h->p;
+    if (out_total)
+        *out_total = nn;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t n,
+                                             void *const *d_outs, const size_t *caps, size_t *counts,
+                                             fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!d_outs || !caps || !counts)
+        return set_err(FDSP_EINVAL, "null output arrays");
+    const size_t C = h->ch.size();
+    for (size_t c = 0; c < C; ++c)
+        counts[c] = 0;
+    if (n == 0)
+        return FDSP_OK;
+    if (!d_in)
+        return set_err(FDSP_EINVAL, "null input");
+    uint32_t max_up = 1;
+    for (auto &k : h->ch)
+        max_up = std::max(max_up, k.up);
+    if (int rc = check_chunk(n, max_up))
+        return rc;
+    hipStream_t s = pick(h->stream, stream);
+    // exact per-channel counts, checked against the capacities up front
+    std::vector<uint32_t> raw(C);
+    for (size_t c = 0; c < C; ++c) {
+        const ChanHost &k = h->ch[c];
+        raw[c] = chunk_count(h->p, n, k.up, k.down);
+        size_t emit = raw[c];
+        if (k.tail && emit && !k.has_prev)
+            --emit;
+        if (emit > caps[c] || (emit && !d_outs[c]))
+            return set_err(FDSP_ERANGE, "channel %zu: capacity %zu below the %zu samples this chunk emits", c,
+                           caps[c], emit);
+        counts[c] = emit;
+    }
+    for (size_t c = 0; c < C; ++c)
+        if (h->ch[c].tail)
+            if (int rc = h->ch[c].d_scratch.ensure((size_t)(raw[c] ? raw[c] : 1) * sizeof(float2)))
+                return rc;
+
+    if (h->mode == FDSP_CHAN_PLAIN) {
+        // reference structure: one multi-shift NCO pass writing a full-rate copy
+        // per channel, then one resampler launch per channel
+        if (int rc = h->d_full.ensure(n * C * sizeof(float2)))
+            return rc;
+        HIP_TRY(launch_nco_multi(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, h->p, h->d_nco.as<NcoChan>(),
+                                 (uint32_t)C, h->d_full.as<float2>(), n, s));
+        for (size_t c = 0; c < C; ++c) {
+            ChanHost &k = h->ch[c];
+            float2 *dst = k.tail ? k.d_scratch.as<float2>() : static_cast<float2 *>(d_outs[c]);
+            HIP_TRY(launch_resampler(h->d_full.as<float2>() + c * n, (uint32_t)n, h->p, k.up, k.down, k.cpp,
+                                     k.d_bank.as<float>(), k.d_hist[h->parity].as<float2>(),
+                                     k.d_hist[h->parity ^ 1].as<float2>(), dst, raw[c], s));
+        }
+        h->parity ^= 1;
+    } else {
+        if (s != h->stream) {
+            // the channel table upload runs on the handle's stream; order it before `s`
+            if (int rc = chan_upload_outs(h, d_outs))
+                return rc;
+            HIP_TRY(hipStreamWaitEvent(s, h->stat_ev, 0));
+        } else if (int rc = chan_upload_outs(h, d_outs)) {
+            return rc;
+        }
+        FusedArgs f{};
+        f.in = reinterpret_cast<const float2 *>(d_in);
+        f.n = (uint32_t)n;
+        f.hist_len = h->hist_len;
+        f.hist[0] = h->d_hist[0].as<float2>();
+        f.hist[1] = h->d_hist[1].as<float2>();
+        f.hist_w[0] = h->d_hist[0].as<float2>();
+        f.hist_w[1] = h->d_hist[1].as<float2>();
+        f.chans = h->d_stat.as<ChanStatic>();
+        f.n_chan = (uint32_t)C;
+        f.taps = h->d_taps.as<float2>();
+        f.taps_total = h->taps_total;
+        f.seg_len = h->seg_len;
+        f.block = h->block;
+        f.pos = h->d_pos.as<ChanPos>();
+        HIP_TRY(launch_chan_fused(f, s));
+        HIP_TRY(launch_chan_advance(h->d_pos.as<ChanPos>(), (uint32_t)n, s));
+        HIP_TRY(hipEventRecord(h->use_ev, s));
+    }
+    // discriminator tails
+    for (size_t c = 0; c < C; ++c) {
+        ChanHost &k = h->ch[c];
+        if (!k.tail || raw[c] == 0)
+            continue;
+        HIP_TRY(launch_discriminator(k.d_scratch.as<float2>(), raw[c], k.d_prev[k.prev_parity].as<float2>(),
+                                     k.has_prev ? 1 : 0, k.d_prev[k.prev_parity ^ 1].as<float2>(),
+                                     static_cast<float *>(d_outs[c]), s));
+        k.has_prev = true;
+        k.prev_parity ^= 1;
+    }
+    h->p += n;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_apply(fdsp_channelizer_t h, const fdsp_cf32 *in, size_t n, void *const *outs,
+                                      const size_t *caps, size_t *counts)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!outs || !caps || !counts)
+        return set_err(FDSP_EINVAL, "null output arrays");
+    const size_t C = h->ch.size();
+    if (n == 0) {
+        for (size_t c = 0; c < C; ++c)
+            counts[c] = 0;
+        return FDSP_OK;
+    }
+    if (!in)
+        return set_err(FDSP_EINVAL, "null input");
+    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+        return rc;
+    h->d_outs.resize(C);
+    std::vector<void *> dptr(C);
+    std::vector<size_t> dcap(C);
+    for (size_t c = 0; c < C; ++c) {
+        const ChanHost &k = h->ch[c];
+        const size_t need = chunk_count(h->p, n, k.up, k.down) + 1;
+        if (int rc = h->d_outs[c].ensure(need * sizeof(float2)))
+            return rc;
+        dptr[c] = h->d_outs[c].p;
+        dcap[c] = need;
+    }
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    std::vector<size_t> got(C);
+    // validate the caller's capacities first so a failure leaves the state untouched
+    for (size_t c = 0; c < C; ++c) {
+        uint32_t cnt = 0;
+        fdsp_channelizer_needed_out_count(h, (uint32_t)c, (uint32_t)n, &cnt);
+        if (cnt > caps[c] || (cnt && !outs[c]))
+            return set_err(FDSP_ERANGE, "channel %zu: capacity %zu below the %u samples this chunk emits", c,
+                           caps[c], cnt);
+    }
+    if (int rc = fdsp_channelizer_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n, dptr.data(),
+                                               dcap.data(), got.data(), h->stream))
+        return rc;
+    for (size_t c = 0; c < C; ++c) {
+        const size_t bytes = got[c] * (h->ch[c].tail ? sizeof(float) : sizeof(float2));
+        if (bytes)
+            HIP_TRY(hipMemcpyAsync(outs[c], dptr[c], bytes, hipMemcpyDeviceToHost, h->stream));
+        counts[c] = got[c];
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph mode
+// ---------------------------------------------------------------------------
+extern "C" int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_len, const fdsp_cf32 *d_in,
+                                             void *const *d_outs, const size_t *caps)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "graph mode needs FDSP_CHAN_BATCHED");
+    if (!d_in || !d_outs || !caps || chunk_len == 0)
+        return set_err(FDSP_EINVAL, "null buffers or empty chunk");
+    const size_t C = h->ch.size();
+    uint32_t max_up = 1;
+    for (size_t c = 0; c < C; ++c) {
+        const ChanHost &k = h->ch[c];
+        if (k.tail)
+            return set_err(FDSP_EINVAL, "graph mode does not take discriminator-tail channels");
+        max_up = std::max(max_up, k.up);
+        // any chunk of chunk_len inputs emits at most ceil(chunk_len*up/down) outputs
+        const uint64_t most = ceil_div((uint64_t)chunk_len * k.up, k.down);
+        if (caps[c] < most || !d_outs[c])
+            return set_err(FDSP_ERANGE, "channel %zu: graph capacity %zu below %llu", c, caps[c],
+                           (unsigned long long)most);
+    }
+    if (int rc = check_chunk(chunk_len, max_up))
+        return rc;
+    fdsp_channelizer_graph_destroy(h);
+    if (int rc = chan_upload_outs(h, d_outs))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    FusedArgs f{};
+    f.in = reinterpret_cast<const float2 *>(d_in);
+    f.n = (uint32_t)chunk_len;
+    f.hist_len = h->hist_len;
+    f.hist[0] = h->d_hist[0].as<float2>();
+    f.hist[1] = h->d_hist[1].as<float2>();
+    f.hist_w[0] = h->d_hist[0].as<float2>();
+    f.hist_w[1] = h->d_hist[1].as<float2>();
+    f.chans = h->d_stat.as<ChanStatic>();
+    f.n_chan = (uint32_t)C;
+    f.taps = h->d_taps.as<float2>();
+    f.taps_total = h->taps_total;
+    f.seg_len = h->seg_len;
+    f.block = h->block;
+    f.pos = h->d_pos.as<ChanPos>();
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e0 = h->time_start ? hipEventRecord(h->time_start, h->stream) : hipSuccess;
+    hipError_t e1 = launch_chan_fused(f, h->stream);
+    hipError_t e4 = h->time_stop ? hipEventRecord(h->time_stop, h->stream) : hipSuccess;
+    hipError_t e2 = launch_chan_advance(h->d_pos.as<ChanPos>(), (uint32_t)chunk_len, h->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e3 = hipStreamEndCapture(h->stream, &g);
+    HIP_TRY(e0);
+    HIP_TRY(e1);
+    HIP_TRY(e4);
+    HIP_TRY(e2);
+    HIP_TRY(e3);
+    h->graph = g;
+    HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    h->graph_chunk = chunk_len;
+    h->graph_counts.assign(C, 0);
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!h->graph_exec)
+        return set_err(FDSP_ENOTINIT, "no graph: call fdsp_channelizer_graph_create first");
+    for (size_t c = 0; c < h->ch.size(); ++c)
+        h->graph_counts[c] = chunk_count(h->p, h->graph_chunk, h->ch[c].up, h->ch[c].down);
+    HIP_TRY(hipGraphLaunch(h->graph_exec, pick(h->stream, stream)));
+    h->p += h->graph_chunk;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_graph_counts(fdsp_channelizer_t h, size_t *counts)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!counts)
+        return set_err(FDSP_EINVAL, "null counts");
+    for (size_t c = 0; c < h->graph_counts.size(); ++c)
+        counts[c] = h->graph_counts[c];
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_graph_set_timing_events(fdsp_channelizer_t h, fdsp_event_t start,
+                                                        fdsp_event_t stop)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if ((start == nullptr) != (stop == nullptr))
+        return set_err(FDSP_EINVAL, "pass both events or neither");
+    h->time_start = static_cast<hipEvent_t>(start);
+    h->time_stop = static_cast<hipEvent_t>(stop);
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "tiling applies to the batched mode");
+    if (segment_len < 256 || segment_len > (1u << 16) || (segment_len & (segment_len - 1)))
+        return set_err(FDSP_EINVAL, "segment_len must be a power of two in [256, 65536]");
+    if (block_threads != 128 && block_threads != 256 && block_threads != 512 && block_threads != 1024)
+        return set_err(FDSP_EINVAL, "block_threads must be 128, 256, 512 or 1024");
+    if (chan_fused_lds_bytes((uint32_t)h->ch.size(), h->taps_total, segment_len, h->hist_len) > 64 * 1024)
+        return set_err(FDSP_ERANGE, "segment_len %u does not fit one block's LDS", segment_len);
+    if (h->graph_exec)
+        return set_err(FDSP_EBUSY, "destroy the graph before retiling");
+    h->seg_len = segment_len;
+    h->block = block_threads;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (segment_len)
+        *segment_len = h->seg_len;
+    if (block_threads)
+        *block_threads = h->block;
+    return FDSP_OK;
+}

@@ -1,0 +1,636 @@
+// fdsp_kernels.hip -- gfx950 kernels of the streaming channelizer.
+//
+// Every streaming kernel works from the closed form of the reference's
+// carried state (reference src/transform/falcon_dsp_polyphase_resampler.cc:256-319):
+// output n of a stream reads input A(n) = floor(n*down/up) (the "x_idx") with
+// filter phase (n*down) mod up, and is emitted in the chunk whose input range
+// holds A(n). A chunk starting at global input p therefore emits outputs
+// n0 = ceil(p*up/down) ... ceil((p+N)*up/down)-1, and the reference's
+// coeff_phase / xOffset are functions of (p, n0). The NCO index of input i is
+// i mod R (reference src/transform/falcon_dsp_freq_shift.cc:177-189).
+#include "fdsp_internal.h"
+
+namespace fdsp {
+
+__device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// NCO, one input -> n_chan shifted outputs
+// (reference kernels __freq_shift / __freq_shift_multi_chan,
+//  src/transform/falcon_dsp_freq_shift_cuda.cu:124-216)
+// Each thread owns 4 consecutive samples: one exact phasor per thread and
+// channel, the other three by a product with e^{j 2 pi nu i} (two roundings).
+// ---------------------------------------------------------------------------
+constexpr int kNcoL = 4;
+
+template <bool kVec>
+__global__ __launch_bounds__(256) void nco_multi_kernel(const float2 *__restrict__ in, uint32_t n, uint64_t start,
+                                                        const NcoChan *__restrict__ chans, uint32_t n_chan,
+                                                        float2 *__restrict__ out, uint64_t out_stride)
+{
+    const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kNcoL;
+    if (i0 >= n)
+        return;
+    const bool full = (i0 + kNcoL) <= n;
+    float2 x[kNcoL];
+    if (kVec && full) {
+        const float4 a = *reinterpret_cast<const float4 *>(in + i0);
+        const float4 b = *reinterpret_cast<const float4 *>(in + i0 + 2);
+        x[0] = make_float2(a.x, a.y);
+        x[1] = make_float2(a.z, a.w);
+        x[2] = make_float2(b.x, b.y);
+        x[3] = make_float2(b.z, b.w);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kNcoL; ++i)
+            x[i] = (i0 + i < n) ? in[i0 + i] : make_float2(0.f, 0.f);
+    }
+    for (uint32_t c = 0; c < n_chan; ++c) {
+        const double nu = chans[c].nu;
+        const uint32_t r = chans[c].r;
+        const uint64_t k0 = mod_u64(start + i0, r, chans[c].inv_r);
+        const float2 base = nco_phasor((int64_t)k0, nu);
+        float2 y[kNcoL];
+        if (k0 + (kNcoL - 1) < r) {
+#pragma unroll
+            for (int i = 0; i < kNcoL; ++i)
+                y[i] = cmul(x[i], cmul(base, chans[c].step[i]));
+        } else {
+            // the rollover falls inside this thread's samples: index restarts at 0
+            const float2 basew = nco_phasor((int64_t)k0 - (int64_t)r, nu);
+#pragma unroll
+            for (int i = 0; i < kNcoL; ++i)
+                y[i] = cmul(x[i], cmul((k0 + i < r) ? base : basew, chans[c].step[i]));
+        }
+        float2 *o = out + (uint64_t)c * out_stride;
+        if (kVec && full) {
+            *reinterpret_cast<float4 *>(o + i0) = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+            *reinterpret_cast<float4 *>(o + i0 + 2) = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+        } else {
+#pragma unroll
+            for (int i = 0; i < kNcoL; ++i)
+                if (i0 + i < n)
+                    o[i0 + i] = y[i];
+        }
+    }
+}
+
+hipError_t launch_nco_multi(const float2 *in, uint32_t n, uint64_t start, const NcoChan *chans, uint32_t n_chan,
+                            float2 *out, uint64_t out_stride, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    const uint32_t threads = (n + kNcoL - 1) / kNcoL;
+    const uint32_t blocks = (threads + 255) / 256;
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16 == 0) &&
+                     (out_stride % 2 == 0);
+    if (vec)
+        hipLaunchKernelGGL(nco_multi_kernel<true>, dim3(blocks), dim3(256), 0, s, in, n, start, chans, n_chan, out,
+                           out_stride);
+    else
+        hipLaunchKernelGGL(nco_multi_kernel<false>, dim3(blocks), dim3(256), 0, s, in, n, start, chans, n_chan, out,
+                           out_stride);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Polyphase resampler, one stream
+// (reference falcon_dsp_polyphase_resampler::apply, .cc:256-319, and the
+//  __polyphase_resampler_single_out kernel, _cuda.cu:165-235; here the
+//  per-output (x_idx, phase) pairs are computed on the device in closed form)
+// A block owns a tile of consecutive outputs. Dense tiles stage their input
+// span and the real tap bank in LDS; sparse tiles (long decimation) read each
+// window straight from global memory.
+// ---------------------------------------------------------------------------
+template <int kBlock, bool kStaged>
+__global__ __launch_bounds__(kBlock) void resampler_kernel(const float2 *__restrict__ in, uint32_t n, uint64_t n0,
+                                                           uint64_t p, uint32_t up, uint32_t down, uint32_t cpp,
+                                                           const float *__restrict__ taps,
+                                                           const float2 *__restrict__ hist_in,
+                                                           float2 *__restrict__ hist_out, float2 *__restrict__ out,
+                                                           uint32_t count, uint32_t tile)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t hlen = cpp - 1;
+    // new history = last cpp-1 samples of [history | chunk]
+    if (blockIdx.x == 0) {
+        for (uint32_t t = threadIdx.x; t < hlen; t += kBlock) {
+            const int64_t i = (int64_t)n - hlen + t;
+            hist_out[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+        }
+    }
+    const uint32_t j0 = blockIdx.x * tile;
+    if (j0 >= count)
+        return;
+    const uint32_t nt = min(tile, count - j0);
+    const uint64_t num0 = (n0 + j0) * (uint64_t)down;
+    const uint64_t q0 = num0 / up;
+    const uint32_t r0 = (uint32_t)(num0 - q0 * up);
+    const int64_t a_first = (int64_t)(q0 - p);  // chunk-local anchor of the tile's first output
+
+    if (kStaged) {
+        const uint32_t last_num = r0 + (nt - 1) * down;
+        const int64_t a_last = a_first + last_num / up;
+        const int64_t lo = a_first - hlen;
+        const uint32_t span = (uint32_t)(a_last - lo + 1);
+        float2 *xs = reinterpret_cast<float2 *>(smem);
+        float *ts = reinterpret_cast<float *>(xs + span);
+        for (uint32_t t = threadIdx.x; t < span; t += kBlock) {
+            const int64_t i = lo + t;
+            xs[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+        }
+        for (uint32_t t = threadIdx.x; t < up * cpp; t += kBlock)
+            ts[t] = taps[t];
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < nt; q += kBlock) {
+            const uint32_t num = r0 + q * down;
+            const uint32_t adv = (up == 1) ? num : num / up;
+            const uint32_t ph = num - adv * up;
+            const float2 *xw = xs + adv;  // window start, relative to lo
+            const float *g = ts + ph * cpp;
+            float2 acc = make_float2(0.f, 0.f);
+            for (uint32_t t = 0; t < cpp; ++t) {
+                const float2 x = xw[t];
+                acc.x = fmaf(x.x, g[t], acc.x);
+                acc.y = fmaf(x.y, g[t], acc.y);
+            }
+            out[j0 + q] = acc;
+        }
+    } else {
+        for (uint32_t q = threadIdx.x; q < nt; q += kBlock) {
+            const uint32_t num = r0 + q * down;
+            const uint32_t adv = (up == 1) ? num : num / up;
+            const uint32_t ph = num - adv * up;
+            const int64_t a = a_first + adv;
+            const float *g = taps + ph * cpp;
+            float2 acc = make_float2(0.f, 0.f);
+            const int64_t lo = a - hlen;
+            if (lo >= 0) {
+                const float2 *xw = in + lo;
+                for (uint32_t t = 0; t < cpp; ++t) {
+                    const float2 x = xw[t];
+                    acc.x = fmaf(x.x, g[t], acc.x);
+                    acc.y = fmaf(x.y, g[t], acc.y);
+                }
+            } else {
+                for (uint32_t t = 0; t < cpp; ++t) {
+                    const int64_t i = lo + t;
+                    const float2 x = i >= 0 ? in[i] : hist_in[hlen + i];
+                    acc.x = fmaf(x.x, g[t], acc.x);
+                    acc.y = fmaf(x.y, g[t], acc.y);
+                }
+            }
+            out[j0 + q] = acc;
+        }
+    }
+}
+
+// LDS budget for a staged resampler tile (samples + taps)
+constexpr uint32_t kResamplerLdsBytes = 48 * 1024;
+
+hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t up, uint32_t down, uint32_t cpp,
+                            const float *taps, const float2 *hist_in, float2 *hist_out, float2 *out, uint32_t count,
+                            hipStream_t s)
+{
+    constexpr int kBlock = 256;
+    const uint64_t n0 = (p * up + down - 1) / down;
+    // staged tile: outputs whose input span (tile*down/up + cpp) and taps fit the LDS budget
+    const uint64_t tap_bytes = (uint64_t)up * cpp * sizeof(float);
+    uint64_t tile = 0;
+    if (tap_bytes + (uint64_t)(cpp + 2) * sizeof(float2) < kResamplerLdsBytes) {
+        const uint64_t span_cap = (kResamplerLdsBytes - tap_bytes) / sizeof(float2) - cpp - 1;
+        tile = span_cap * up / down;
+        if (tile > 4096)
+            tile = 4096;
+    }
+    const bool staged = tile >= kBlock;
+    if (!staged) {
+        tile = 1024;
+        // keep r0 + q*down inside 32 bits
+        while (tile > 1 && (uint64_t)tile * down + up >= (1ull << 31))
+            tile >>= 1;
+    }
+    const uint32_t blocks = count ? (uint32_t)((count + tile - 1) / tile) : 1;
+    if (staged) {
+        const uint32_t span_max = (uint32_t)(((tile - 1) * down + up) / up + cpp + 1);
+        const size_t lds = (size_t)span_max * sizeof(float2) + tap_bytes;
+        hipLaunchKernelGGL((resampler_kernel<kBlock, true>), dim3(blocks), dim3(kBlock), lds, s, in, n, n0, p, up,
+                           down, cpp, taps, hist_in, hist_out, out, count, (uint32_t)tile);
+    } else {
+        hipLaunchKernelGGL((resampler_kernel<kBlock, false>), dim3(blocks), dim3(kBlock), 0, s, in, n, n0, p, up,
+                           down, cpp, taps, hist_in, hist_out, out, count, (uint32_t)tile);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FIR with complex taps, one stream
+// (reference falcon_dsp_fir_filter::apply, .cc:177-243; CUDA __fir_filter,
+//  _cuda.cu:102-187). y[n] = sum_k h[k] x[n-k], accumulated k = 0..M-1.
+// Input window and taps staged in LDS.
+// ---------------------------------------------------------------------------
+template <int kBlock, bool kTapsLds>
+__global__ __launch_bounds__(kBlock) void fir_kernel(const float2 *__restrict__ in, uint32_t n,
+                                                     const float2 *__restrict__ taps, uint32_t m,
+                                                     const float2 *__restrict__ hist_in,
+                                                     float2 *__restrict__ hist_out, float2 *__restrict__ out,
+                                                     uint32_t tile)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t hlen = m - 1;
+    if (blockIdx.x == 0) {
+        for (uint32_t t = threadIdx.x; t < hlen; t += kBlock) {
+            const int64_t i = (int64_t)n - hlen + t;
+            hist_out[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+        }
+    }
+    const uint32_t o0 = blockIdx.x * tile;
+    const uint32_t nt = min(tile, n - o0);
+    float2 *xs = reinterpret_cast<float2 *>(smem);
+    float2 *ts = xs + tile + hlen;
+    const int64_t lo = (int64_t)o0 - hlen;
+    for (uint32_t t = threadIdx.x; t < nt + hlen; t += kBlock) {
+        const int64_t i = lo + t;
+        xs[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+    }
+    if (kTapsLds)
+        for (uint32_t t = threadIdx.x; t < m; t += kBlock)
+            ts[t] = taps[t];
+    __syncthreads();
+    const float2 *g = kTapsLds ? ts : taps;
+    for (uint32_t q = threadIdx.x; q < nt; q += kBlock) {
+        const float2 *xw = xs + q + hlen;  // xw[-k] = x[o0 + q - k]
+        float2 acc = make_float2(0.f, 0.f);
+        for (uint32_t k = 0; k < m; ++k) {
+            const float2 x = xw[-(int32_t)k];
+            const float2 h = g[k];
+            // complex<float> product added to the accumulator, reference order
+            acc.x += x.x * h.x - x.y * h.y;
+            acc.y += x.x * h.y + x.y * h.x;
+        }
+        out[o0 + q] = acc;
+    }
+}
+
+hipError_t launch_fir(const float2 *in, uint32_t n, const float2 *taps, uint32_t m, const float2 *hist_in,
+                      float2 *hist_out, float2 *out, hipStream_t s)
+{
+    constexpr int kBlock = 256;
+    const uint32_t tile = 2048;
+    const bool taps_lds = m <= 2048;
+    const size_t lds = (size_t)(tile + m - 1) * sizeof(float2) + (taps_lds ? m * sizeof(float2) : 0);
+    if (lds > 64 * 1024)
+        return hipErrorInvalidValue;
+    const uint32_t blocks = (n + tile - 1) / tile;
+    if (taps_lds)
+        hipLaunchKernelGGL((fir_kernel<kBlock, true>), dim3(blocks), dim3(kBlock), lds, s, in, n, taps, m, hist_in,
+                           hist_out, out, tile);
+    else
+        hipLaunchKernelGGL((fir_kernel<kBlock, false>), dim3(blocks), dim3(kBlock), lds, s, in, n, taps, m, hist_in,
+                           hist_out, out, tile);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FIR bank, real taps, many channels (compute-bound FP32 VALU kernel).
+// blockIdx.y is the channel. Each thread keeps kR consecutive outputs in
+// registers and slides a kR+kR-1 sample register window down the input, so
+// every tap costs one LDS read and 2*kR FMAs; taps are channel-uniform and
+// come through the scalar cache. The LDS window is padded by one sample per
+// kR so the stride-kR reads of a wavefront fall in distinct banks.
+// ---------------------------------------------------------------------------
+constexpr int kBankR = 8;
+__host__ __device__ __forceinline__ uint32_t bank_pad(uint32_t i) { return i + i / kBankR; }
+
+template <int kBlock>
+__global__ __launch_bounds__(kBlock) void fir_bank_kernel(const float2 *__restrict__ in, uint32_t n,
+                                                          uint64_t in_stride, const float *__restrict__ taps,
+                                                          uint32_t m, const float2 *__restrict__ hist_in,
+                                                          float2 *__restrict__ hist_out,
+                                                          float2 *__restrict__ out, uint64_t out_stride)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float2 *xs = reinterpret_cast<float2 *>(smem);
+    const uint32_t c = blockIdx.y;
+    const uint32_t hlen = m - 1;
+    const float2 *x = in + c * in_stride;
+    const float2 *hi = hist_in + (uint64_t)c * hlen;
+    const float *h = taps + (uint64_t)c * m;
+    float2 *y = out + c * out_stride;
+    if (blockIdx.x == 0) {
+        float2 *ho = hist_out + (uint64_t)c * hlen;
+        for (uint32_t t = threadIdx.x; t < hlen; t += kBlock) {
+            const int64_t i = (int64_t)n - hlen + t;
+            ho[t] = i >= 0 ? x[i] : hi[hlen + i];
+        }
+    }
+    constexpr uint32_t tile = kBlock * kBankR;
+    const uint32_t o0 = blockIdx.x * tile;
+    // window: x[o0 - mpad .. o0 + tile), mpad = m rounded up to kR (extra zero-tap slack)
+    const uint32_t mpad = (m + kBankR - 1) / kBankR * kBankR;
+    const int64_t lo = (int64_t)o0 - mpad;
+    const uint32_t span = tile + mpad;
+    for (uint32_t t = threadIdx.x; t < span; t += kBlock) {
+        const int64_t i = lo + t;
+        float2 v = make_float2(0.f, 0.f);
+        if (i >= 0) {
+            if (i < n)
+                v = x[i];
+        } else if (i >= -(int64_t)hlen) {
+            v = hi[hlen + i];
+        }
+        xs[bank_pad(t)] = v;
+    }
+    __syncthreads();
+    // outputs o0 + tid*kR + r; sample x[o0 + tid*kR + r - k] lives at window
+    // index mpad + tid*kR + r - k
+    const uint32_t wb = mpad + threadIdx.x * kBankR;
+    float2 acc[kBankR];
+#pragma unroll
+    for (int r = 0; r < kBankR; ++r)
+        acc[r] = make_float2(0.f, 0.f);
+    // w[i] = x at window index wb - k0 - (kR-1) + i, i in [0, 2kR-1). Taps
+    // k0..k0+kR-1 need all of it; the next block of taps reuses the lowest
+    // kR-1 entries (moved up by kR) and loads kR new ones.
+    float2 w[2 * kBankR - 1];
+#pragma unroll
+    for (int i = 1; i < kBankR; ++i)
+        w[kBankR - 1 + i] = xs[bank_pad(wb + i)];
+    for (uint32_t k0 = 0; k0 < mpad; k0 += kBankR) {
+#pragma unroll
+        for (int i = 0; i < kBankR; ++i)
+            w[i] = xs[bank_pad(wb - k0 - (kBankR - 1) + i)];
+        float hk[kBankR];
+#pragma unroll
+        for (int kk = 0; kk < kBankR; ++kk)
+            hk[kk] = (k0 + kk < m) ? h[k0 + kk] : 0.f;
+#pragma unroll
+        for (int kk = 0; kk < kBankR; ++kk) {
+#pragma unroll
+            for (int r = 0; r < kBankR; ++r) {
+                const float2 v = w[kBankR - 1 + r - kk];
+                acc[r].x = fmaf(hk[kk], v.x, acc[r].x);
+                acc[r].y = fmaf(hk[kk], v.y, acc[r].y);
+            }
+        }
+        // slide: the lowest kR-1 samples become the upper part for the next taps
+#pragma unroll
+        for (int i = 0; i < kBankR - 1; ++i)
+            w[kBankR + i] = w[i];
+    }
+    const uint32_t ob = o0 + threadIdx.x * kBankR;
+#pragma unroll
+    for (int r = 0; r < kBankR; ++r)
+        if (ob + r < n)
+            y[ob + r] = acc[r];
+}
+
+hipError_t launch_fir_bank(const float2 *in, uint32_t n, uint64_t in_stride, const float *taps, uint32_t m,
+                           uint32_t n_chan, const float2 *hist_in, float2 *hist_out, float2 *out,
+                           uint64_t out_stride, hipStream_t s)
+{
+    constexpr int kBlock = 256;
+    constexpr uint32_t tile = kBlock * kBankR;
+    const uint32_t mpad = (m + kBankR - 1) / kBankR * kBankR;
+    const size_t lds = (size_t)(bank_pad(tile + mpad) + 1) * sizeof(float2);
+    if (lds > 64 * 1024)
+        return hipErrorInvalidValue;
+    dim3 grid((n + tile - 1) / tile, n_chan);
+    hipLaunchKernelGGL((fir_bank_kernel<kBlock>), grid, dim3(kBlock), lds, s, in, n, in_stride, taps, m, hist_in,
+                       hist_out, out, out_stride);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Polar discriminator
+// (reference falcon_dsp_polar_discriminator::apply, .cc:134-171)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void discriminator_kernel(const float2 *__restrict__ in, uint32_t n,
+                                                            const float2 *__restrict__ prev, int has_prev,
+                                                            float2 *__restrict__ prev_out, float *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    if (i == n - 1)
+        *prev_out = in[i];
+    if (i == 0 && !has_prev)
+        return;
+    const float2 a = in[i];
+    const float2 b = (i == 0) ? *prev : in[i - 1];
+    const float2 z = cmul(a, make_float2(b.x, -b.y));
+    out[has_prev ? i : i - 1] = atan2f(z.y, z.x);
+}
+
+hipError_t launch_discriminator(const float2 *in, uint32_t n, const float2 *prev, int has_prev, float2 *prev_out,
+                                float *out, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(discriminator_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in, n, prev, has_prev,
+                       prev_out, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused multi-rate channelizer (all channels, one launch, input read once)
+// (replaces falcon_dsp_multi_rate_channelizer{,_cuda}::apply,
+//  reference src/transform/falcon_dsp_multi_rate_channelizer.cc:146-164,
+//  _cuda.cu:375-514)
+//
+// Shift-then-filter equals filter-with-modulated-taps then one phasor per
+// output: with k(a) the NCO index of the output's anchor sample a and
+// d = a - i the distance of window sample i,
+//   y = e^{j2pi nu k(a)} * sum_t [h_t e^{-j2pi nu d_t}] x[i_t]
+// as long as no rollover (k restarting at 0) falls inside the window; when one
+// does (k(a) < cpp-1, rare: R >= 1e5), the samples before it carry an extra
+// e^{j2pi nu R}. Block b stages input segment b (plus history) in LDS once and
+// computes every channel's outputs whose anchors lie in the segment; the
+// (channel, output) pairs are flattened so all threads stay busy.
+// ---------------------------------------------------------------------------
+struct FusedDev {
+    const float2 *in;
+    uint32_t n;
+    uint32_t hist_len;
+    const float2 *hist0, *hist1;
+    float2 *histw0, *histw1;
+    const ChanStatic *chans;
+    uint32_t n_chan;
+    const float2 *taps;
+    uint32_t taps_total;
+    uint32_t seg_len;
+    const ChanPos *pos;
+};
+
+template <int kBlock>
+__global__ __launch_bounds__(kBlock) void chan_fused_kernel(FusedDev a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t C = a.n_chan;
+    const uint32_t H = a.hist_len;
+    const uint32_t S = a.seg_len;
+    ChanStatic *sc = reinterpret_cast<ChanStatic *>(smem);
+    float2 *tp = reinterpret_cast<float2 *>(sc + C);
+    float2 *xs = tp + a.taps_total;
+    uint32_t *s_pref = reinterpret_cast<uint32_t *>(xs + S + H);  // C+1
+    uint32_t *s_qb = s_pref + C + 1;
+    uint32_t *s_rb = s_qb + C;
+    uint32_t *s_jb = s_rb + C;
+    uint32_t *s_ks = s_jb + C;
+
+    const uint64_t P = a.pos->p;
+    const uint32_t par = a.pos->parity;
+    const float2 *hist = par ? a.hist1 : a.hist0;
+    const uint32_t seg0 = blockIdx.x * S;
+    const uint32_t seg1 = min(seg0 + S, a.n);
+
+    if (blockIdx.x == 0) {
+        float2 *hw = par ? a.histw0 : a.histw1;
+        for (uint32_t t = threadIdx.x; t < H; t += kBlock) {
+            const int64_t i = (int64_t)a.n - H + t;
+            hw[t] = i >= 0 ? a.in[i] : hist[H + i];
+        }
+    }
+
+    // stage input segment with H samples of history before it
+    const uint32_t span = seg1 - seg0 + H;
+    for (uint32_t t = threadIdx.x; t < span; t += kBlock) {
+        const int64_t i = (int64_t)seg0 - H + t;
+        xs[t] = i >= 0 ? a.in[i] : hist[H + i];
+    }
+    for (uint32_t t = threadIdx.x; t < a.taps_total; t += kBlock)
+        tp[t] = a.taps[t];
+    // per-channel output range of this segment
+    for (uint32_t c = threadIdx.x; c < C; c += kBlock) {
+        const ChanStatic ch = a.chans[c];
+        sc[c] = ch;
+        const uint64_t g0 = P + seg0, g1 = P + seg1;
+        const uint64_t nlo = ceil_div_u64(g0 * ch.up, ch.down);
+        const uint64_t nhi = ceil_div_u64(g1 * ch.up, ch.down);
+        const uint64_t n0 = ceil_div_u64(P * ch.up, ch.down);
+        const uint64_t num = nlo * ch.down;
+        const uint64_t qb = num / ch.up;
+        s_qb[c] = (uint32_t)(qb - g0);
+        s_rb[c] = (uint32_t)(num - qb * ch.up);
+        s_jb[c] = (uint32_t)(nlo - n0);
+        s_ks[c] = (uint32_t)mod_u64(g0, ch.r, ch.inv_r);
+        s_pref[c + 1] = (uint32_t)(nhi - nlo);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // inclusive scan of the counts in s_pref[1..C], 64 channels at a time
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < C; base += 64) {
+            const uint32_t c = base + threadIdx.x;
+            uint32_t v = c < C ? s_pref[c + 1] : 0;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t u = __shfl_up(v, off, 64);
+                if ((int)threadIdx.x >= off)
+                    v += u;
+            }
+            if (c < C)
+                s_pref[c + 1] = v + carry;
+            carry += __shfl(v, 63, 64);
+        }
+        if (threadIdx.x == 0)
+            s_pref[0] = 0;
+    }
+    __syncthreads();
+
+    const uint32_t total = s_pref[C];
+    uint32_t c = 0;
+    for (uint32_t q = threadIdx.x; q < total; q += kBlock) {
+        while (q >= s_pref[c + 1])
+            ++c;
+        const uint32_t qq = q - s_pref[c];
+        const uint32_t up = sc[c].up, cpp = sc[c].cpp;
+        const uint32_t num = s_rb[c] + qq * sc[c].down;
+        const uint32_t adv = (up == 1) ? num : num / up;
+        const uint32_t ph = num - adv * up;
+        const uint32_t al = s_qb[c] + adv;  // anchor, relative to seg0
+        const float2 *xw = xs + (H + al - (cpp - 1));
+        const float2 *g = tp + sc[c].taps_off + ph * cpp;
+        uint64_t k = (uint64_t)s_ks[c] + al;  // NCO index of the anchor (R < 2^32, al < seg_len < R)
+        const uint32_t r = sc[c].r;
+        if (k >= r)
+            k -= r;
+        float2 acc = make_float2(0.f, 0.f);
+        if (k >= cpp - 1) {
+            for (uint32_t t = 0; t < cpp; ++t)
+                cmac(acc, g[t], xw[t]);
+        } else {
+            // a rollover falls inside the window: samples t < split precede it
+            const uint32_t split = cpp - 1 - (uint32_t)k;
+            float2 acc2 = make_float2(0.f, 0.f);
+            for (uint32_t t = 0; t < split; ++t)
+                cmac(acc2, g[t], xw[t]);
+            for (uint32_t t = split; t < cpp; ++t)
+                cmac(acc, g[t], xw[t]);
+            const float2 w = cmul(sc[c].wrap, acc2);
+            acc.x += w.x;
+            acc.y += w.y;
+        }
+        const float2 y = cmul(nco_phasor((int64_t)k, sc[c].nu), acc);
+        reinterpret_cast<float2 *>(sc[c].out)[s_jb[c] + qq] = y;
+    }
+}
+
+__global__ void chan_advance_kernel(ChanPos *pos, uint32_t n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        pos->p += n;
+        pos->parity ^= 1u;
+    }
+}
+
+size_t chan_fused_lds_bytes(uint32_t n_chan, uint32_t taps_total, uint32_t seg_len, uint32_t hist_len)
+{
+    return (size_t)n_chan * sizeof(ChanStatic) + (size_t)taps_total * sizeof(float2) +
+           (size_t)(seg_len + hist_len) * sizeof(float2) + (size_t)(5 * n_chan + 1) * sizeof(uint32_t);
+}
+
+hipError_t launch_chan_fused(const FusedArgs &f, hipStream_t s)
+{
+    FusedDev d;
+    d.in = f.in;
+    d.n = f.n;
+    d.hist_len = f.hist_len;
+    d.hist0 = f.hist[0];
+    d.hist1 = f.hist[1];
+    d.histw0 = f.hist_w[0];
+    d.histw1 = f.hist_w[1];
+    d.chans = f.chans;
+    d.n_chan = f.n_chan;
+    d.taps = f.taps;
+    d.taps_total = f.taps_total;
+    d.seg_len = f.seg_len;
+    d.pos = f.pos;
+    const uint32_t blocks = f.n ? (f.n + f.seg_len - 1) / f.seg_len : 1;
+    const size_t lds = chan_fused_lds_bytes(f.n_chan, f.taps_total, f.seg_len, f.hist_len);
+    if (lds > 64 * 1024)
+        return hipErrorInvalidValue;
+    switch (f.block) {
+    case 128:
+        hipLaunchKernelGGL(chan_fused_kernel<128>, dim3(blocks), dim3(128), lds, s, d);
+        break;
+    case 512:
+        hipLaunchKernelGGL(chan_fused_kernel<512>, dim3(blocks), dim3(512), lds, s, d);
+        break;
+    case 1024:
+        hipLaunchKernelGGL(chan_fused_kernel<1024>, dim3(blocks), dim3(1024), lds, s, d);
+        break;
+    default:
+        hipLaunchKernelGGL(chan_fused_kernel<256>, dim3(blocks), dim3(256), lds, s, d);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_chan_advance(ChanPos *pos, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(chan_advance_kernel, dim3(1), dim3(64), 0, s, pos, n);
+    return hipGetLastError();
+}
+
+}  // namespace fdsp

@@ -1,0 +1,495 @@
+"""Python mirror of the reference's operator classes, over the C ABI.
+
+Each class keeps the reference's name, argument meaning and error behaviour
+(methods return what the reference returns: bools, counts, arrays) while the
+work runs in the gfx950 kernels of libfdsp_hip.so:
+
+=============================  ======================================================
+this module                    reference (hdr/transform/*.h)
+=============================  ======================================================
+FreqShift                      falcon_dsp_freq_shift{,_cuda}      (freq_shift.h:98-121)
+PolyphaseResampler             falcon_dsp_polyphase_resampler{,_cuda} (.h:168-195)
+FirFilter                      falcon_dsp_fir_filter{,_cuda}      (fir_filter.h:119-140)
+FirBank                        multi-channel falcon_dsp_fir_filter with real taps
+PolarDiscriminator             falcon_dsp_polar_discriminator     (.h:97-114)
+MultiRateChannelizer           falcon_dsp_multi_rate_channelizer{,_cuda} (.h:97-134)
+freq_shift / resample /        free functions (freq_shift.h:85, resample.h:85-95,
+upfirdn / fir_filter /         fir_filter.h:87-91, polar_discriminator.h:81-85)
+polar_discriminator
+=============================  ======================================================
+
+Host arrays are numpy complex64 (interleaved float pairs, the
+std::complex<float> layout). ``DeviceBuffer`` holds device memory for the
+asynchronous ``*_device`` forms.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from ._native import check, lib
+
+_vp = C.c_void_p
+
+
+def _cf32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.complex64))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+# ---------------------------------------------------------------------------
+# device plumbing
+# ---------------------------------------------------------------------------
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().fdsp_device_count(C.byref(n)))
+    return n.value
+
+
+class Stream:
+    def __init__(self):
+        self.h = _vp()
+        check(lib().fdsp_stream_create(C.byref(self.h)))
+
+    def sync(self):
+        check(lib().fdsp_stream_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().fdsp_stream_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = _vp()
+        check(lib().fdsp_event_create(C.byref(self.h)))
+
+    def record(self, stream: Optional[Stream] = None):
+        check(lib().fdsp_event_record(self.h, stream.h if stream else None))
+
+    def sync(self):
+        check(lib().fdsp_event_sync(self.h))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().fdsp_event_elapsed_ms(self.h, end.h, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fdsp_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device allocation of ``count`` elements of ``dtype``."""
+
+    def __init__(self, count: int, dtype=np.complex64):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        self.ptr = _vp()
+        check(lib().fdsp_malloc(C.byref(self.ptr), max(self.nbytes, 1)))
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, stream: Optional[Stream] = None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.size, a.dtype)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a: np.ndarray, stream: Optional[Stream] = None, offset: int = 0):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if offset + a.size > self.count:
+            raise ValueError("upload past the end of the buffer")
+        dst = _vp(self.ptr.value + offset * self.dtype.itemsize)
+        check(lib().fdsp_memcpy_h2d(dst, _ptr(a), a.nbytes, stream.h if stream else None))
+
+    def download(self, count: Optional[int] = None, offset: int = 0, stream: Optional[Stream] = None) -> np.ndarray:
+        count = self.count - offset if count is None else int(count)
+        out = np.empty(count, dtype=self.dtype)
+        if count:
+            src = _vp(self.ptr.value + offset * self.dtype.itemsize)
+            check(lib().fdsp_memcpy_d2h(_ptr(out), src, out.nbytes, stream.h if stream else None))
+            if stream:
+                stream.sync()
+        return out
+
+    def at(self, offset: int) -> _vp:
+        return _vp(self.ptr.value + offset * self.dtype.itemsize)
+
+    def zero(self, stream: Optional[Stream] = None):
+        check(lib().fdsp_memset(self.ptr, 0, self.nbytes, stream.h if stream else None))
+
+    def free(self):
+        if self.ptr:
+            lib().fdsp_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    check(lib().fdsp_device_sync())
+
+
+# ---------------------------------------------------------------------------
+# rate helpers
+# ---------------------------------------------------------------------------
+def calculate_gcd(a: int, b: int) -> int:
+    """calculate_gcd (reference src/utilities/falcon_dsp_utils.cc:123-126)."""
+    return int(lib().fdsp_gcd(a, b))
+
+
+def calculate_lcm(a: int, b: int) -> int:
+    """calculate_lcm (reference src/utilities/falcon_dsp_utils.cc:133-137)."""
+    return int(lib().fdsp_lcm(a, b))
+
+
+def rat_approx(f: float, max_denom: int):
+    """rat_approx (reference src/utilities/falcon_dsp_utils.cc:231-294) -> (num, denom)."""
+    n, d = C.c_int64(), C.c_int64()
+    lib().fdsp_rat_approx(float(f), int(max_denom), C.byref(n), C.byref(d))
+    return n.value, d.value
+
+
+def get_freq_shift_params(fs: int, f_hz: int):
+    """falcon_dsp_freq_shift::get_freq_shift_params (reference freq_shift.cc:194-235)
+    -> (rollover index, angular frequency in rad/sample)."""
+    r, w = C.c_uint32(), C.c_double()
+    check(lib().fdsp_freq_shift_params(fs, f_hz, C.byref(r), C.byref(w)))
+    return r.value, w.value
+
+
+def reduce_rates(in_rate: int, out_rate: int):
+    """Exact up/down for a rational resample: (out/g, in/g) with g = gcd."""
+    g = calculate_gcd(in_rate, out_rate)
+    return out_rate // g, in_rate // g
+
+
+# ---------------------------------------------------------------------------
+# operators
+# ---------------------------------------------------------------------------
+class _Handle:
+    _destroy = ""
+
+    def __init__(self):
+        self.h = _vp()
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(lib(), self._destroy)(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FreqShift(_Handle):
+    """NCO frequency shift with one or more output shifts.
+
+    ``FreqShift(fs, f)`` mirrors falcon_dsp_freq_shift::initialize(fs, f);
+    ``FreqShift(fs, [f0, f1, ...])`` mirrors falcon_dsp_freq_shift_cuda's
+    multi-channel initialize. ``apply`` returns one array per shift."""
+    _destroy = "fdsp_freq_shift_destroy"
+
+    def __init__(self, fs: int, shifts):
+        super().__init__()
+        self.single = np.isscalar(shifts)
+        sh = [int(shifts)] if self.single else [int(s) for s in shifts]
+        self.n_shifts = len(sh)
+        arr = (C.c_int32 * len(sh))(*sh)
+        check(lib().fdsp_freq_shift_create(fs, arr, len(sh), C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_freq_shift_reset(self.h))
+
+    def samples_handled(self, chan: int = 0) -> int:
+        v = C.c_uint64()
+        check(lib().fdsp_freq_shift_get_state(self.h, chan, C.byref(v)))
+        return v.value
+
+    def apply(self, x):
+        x = _cf32(x)
+        outs = [np.empty(x.size, dtype=np.complex64) for _ in range(self.n_shifts)]
+        ptrs = (_vp * self.n_shifts)(*[o.ctypes.data for o in outs])
+        check(lib().fdsp_freq_shift_apply(self.h, _ptr(x), x.size, ptrs))
+        return outs[0] if self.single else outs
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, out_stride: int,
+                     stream: Optional[Stream] = None, in_offset: int = 0):
+        check(lib().fdsp_freq_shift_apply_device(self.h, d_in.at(in_offset), n, d_out.ptr, out_stride,
+                                                 stream.h if stream else None))
+
+
+class PolyphaseResampler(_Handle):
+    """falcon_dsp_polyphase_resampler: streaming upfirdn with real taps."""
+    _destroy = "fdsp_resampler_destroy"
+
+    def __init__(self, up: int, down: int, taps):
+        super().__init__()
+        t = _cf32(taps)
+        self._taps = t
+        self.up, self.down = int(up), int(down)
+        check(lib().fdsp_resampler_create(up, down, _ptr(t), t.size, C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_resampler_reset(self.h))
+
+    def needed_out_count(self, in_count: int) -> int:
+        v = C.c_uint32()
+        check(lib().fdsp_resampler_needed_out_count(self.h, in_count, C.byref(v)))
+        return v.value
+
+    def coeffs_per_phase(self) -> int:
+        v = C.c_uint32()
+        check(lib().fdsp_resampler_coeffs_per_phase(self.h, C.byref(v)))
+        return v.value
+
+    def state(self):
+        """(coeff_phase, xOffset, inputs consumed, outputs produced)."""
+        ph, xo, it, ot = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().fdsp_resampler_get_state(self.h, C.byref(ph), C.byref(xo), C.byref(it), C.byref(ot)))
+        return ph.value, xo.value, it.value, ot.value
+
+    def apply(self, x) -> np.ndarray:
+        x = _cf32(x)
+        cap = self.needed_out_count(x.size)
+        out = np.empty(max(cap, 1), dtype=np.complex64)
+        n = C.c_size_t()
+        check(lib().fdsp_resampler_apply(self.h, _ptr(x), x.size, _ptr(out), cap, C.byref(n)))
+        return out[: n.value]
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None,
+                     in_offset: int = 0) -> int:
+        got = C.c_size_t()
+        check(lib().fdsp_resampler_apply_device(self.h, d_in.at(in_offset), n, d_out.ptr, d_out.count,
+                                                C.byref(got), stream.h if stream else None))
+        return got.value
+
+
+class FirFilter(_Handle):
+    """falcon_dsp_fir_filter: y[n] = sum_k h[k] x[n-k] with carried history."""
+    _destroy = "fdsp_fir_destroy"
+
+    def __init__(self, coeffs):
+        super().__init__()
+        c = _cf32(coeffs)
+        self._c = c
+        check(lib().fdsp_fir_create(_ptr(c), c.size, C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_fir_reset(self.h))
+
+    def apply(self, x) -> np.ndarray:
+        x = _cf32(x)
+        out = np.empty(x.size, dtype=np.complex64)
+        check(lib().fdsp_fir_apply(self.h, _ptr(x), x.size, _ptr(out)))
+        return out
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None):
+        check(lib().fdsp_fir_apply_device(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
+
+
+class FirBank(_Handle):
+    """Many independent FIR channels with real taps, one launch for all."""
+    _destroy = "fdsp_fir_bank_destroy"
+
+    def __init__(self, taps: np.ndarray):
+        super().__init__()
+        t = np.ascontiguousarray(np.asarray(taps, dtype=np.float32))
+        if t.ndim != 2:
+            raise ValueError("taps must be (n_chan, n_taps)")
+        self.n_chan, self.n_taps = t.shape
+        self._t = t
+        check(lib().fdsp_fir_bank_create(self.n_chan, self.n_taps, _ptr(t), C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_fir_bank_reset(self.h))
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, in_stride: int, d_out: DeviceBuffer, out_stride: int,
+                     stream: Optional[Stream] = None):
+        check(lib().fdsp_fir_bank_apply_device(self.h, d_in.ptr, n, in_stride, d_out.ptr, out_stride,
+                                               stream.h if stream else None))
+
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        """x: (n_chan, n) complex64 -> (n_chan, n)."""
+        x = _cf32(x).reshape(self.n_chan, -1)
+        n = x.shape[1]
+        d_in = DeviceBuffer.from_array(x.ravel())
+        d_out = DeviceBuffer(x.size)
+        self.apply_device(d_in, n, n, d_out, n)
+        device_sync()
+        return d_out.download().reshape(self.n_chan, n)
+
+
+class PolarDiscriminator(_Handle):
+    """falcon_dsp_polar_discriminator: atan2 of x[n]*conj(x[n-1])."""
+    _destroy = "fdsp_discriminator_destroy"
+
+    def __init__(self):
+        super().__init__()
+        check(lib().fdsp_discriminator_create(C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_discriminator_reset(self.h))
+
+    def apply(self, x) -> np.ndarray:
+        x = _cf32(x)
+        out = np.empty(max(x.size, 1), dtype=np.float32)
+        n = C.c_size_t()
+        check(lib().fdsp_discriminator_apply(self.h, _ptr(x), x.size, _ptr(out), C.byref(n)))
+        return out[: n.value]
+
+
+class MultiRateChannelizer(_Handle):
+    """falcon_dsp_multi_rate_channelizer{,_cuda}: per channel, NCO shift then
+    polyphase resample.
+
+    ``channels``: sequence of dicts with the fields of
+    multi_rate_channelizer_channel_s (output_sample_rate_in_sps,
+    freq_shift_in_hz, up_rate, down_rate, taps) and optionally
+    ``tail="discriminator"``. ``batched=True`` selects the fused one-launch
+    kernel; ``batched=False`` the reference-structured per-channel form."""
+    _destroy = "fdsp_channelizer_destroy"
+
+    def __init__(self, fs: int, channels: Sequence[dict], batched: bool = True):
+        super().__init__()
+        self.fs = int(fs)
+        self.n = len(channels)
+        self._taps = [_cf32(c["taps"]) for c in channels]
+        self.tails = [1 if c.get("tail") in ("discriminator", N.FDSP_TAIL_DISCRIMINATOR) else 0 for c in channels]
+        descs = (N.ChannelDesc * self.n)()
+        for i, c in enumerate(channels):
+            d = descs[i]
+            d.output_sample_rate_in_sps = int(c.get("output_sample_rate_in_sps", 0))
+            d.freq_shift_in_hz = int(c["freq_shift_in_hz"])
+            d.up_rate = int(c["up_rate"])
+            d.down_rate = int(c["down_rate"])
+            d.taps = self._taps[i].ctypes.data
+            d.n_taps = self._taps[i].size
+            d.tail = self.tails[i]
+        self.mode = N.FDSP_CHAN_BATCHED if batched else N.FDSP_CHAN_PLAIN
+        check(lib().fdsp_channelizer_create(self.fs, descs, self.n, self.mode, C.byref(self.h)))
+        self.ups = [int(c["up_rate"]) for c in channels]
+        self.downs = [int(c["down_rate"]) for c in channels]
+
+    def reset_state(self):
+        check(lib().fdsp_channelizer_reset(self.h))
+
+    def needed_out_count(self, chan: int, in_count: int) -> int:
+        v = C.c_uint32()
+        check(lib().fdsp_channelizer_needed_out_count(self.h, chan, in_count, C.byref(v)))
+        return v.value
+
+    def state(self, chan: int):
+        """(nco samples handled, coeff_phase, xOffset, inputs consumed, outputs produced)."""
+        k, ph, xo, it, ot = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().fdsp_channelizer_get_state(self.h, chan, C.byref(k), C.byref(ph), C.byref(xo), C.byref(it),
+                                               C.byref(ot)))
+        return k.value, ph.value, xo.value, it.value, ot.value
+
+    def set_tiling(self, segment_len: int, block_threads: int):
+        check(lib().fdsp_channelizer_set_tiling(self.h, segment_len, block_threads))
+
+    def get_tiling(self):
+        s, b = C.c_uint32(), C.c_uint32()
+        check(lib().fdsp_channelizer_get_tiling(self.h, C.byref(s), C.byref(b)))
+        return s.value, b.value
+
+    def apply(self, x) -> List[np.ndarray]:
+        x = _cf32(x)
+        caps = [self.needed_out_count(i, x.size) for i in range(self.n)]
+        outs = [np.empty(max(c, 1), dtype=np.float32 if self.tails[i] else np.complex64)
+                for i, c in enumerate(caps)]
+        op = (_vp * self.n)(*[o.ctypes.data for o in outs])
+        cp = (C.c_size_t * self.n)(*caps)
+        counts = (C.c_size_t * self.n)()
+        check(lib().fdsp_channelizer_apply(self.h, _ptr(x), x.size, op, cp, counts))
+        return [outs[i][: counts[i]] for i in range(self.n)]
+
+    def max_out(self, chunk: int) -> List[int]:
+        """Upper bound of outputs per channel for any chunk of ``chunk`` inputs."""
+        return [-(-chunk * u // d) for u, d in zip(self.ups, self.downs)]
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, d_outs: Sequence[DeviceBuffer],
+                     stream: Optional[Stream] = None, in_offset: int = 0) -> List[int]:
+        op = (_vp * self.n)(*[b.ptr.value for b in d_outs])
+        cp = (C.c_size_t * self.n)(*[b.count for b in d_outs])
+        counts = (C.c_size_t * self.n)()
+        check(lib().fdsp_channelizer_apply_device(self.h, d_in.at(in_offset), n, op, cp, counts,
+                                                  stream.h if stream else None))
+        return list(counts)
+
+    # graph mode
+    def graph_create(self, chunk_len: int, d_in: DeviceBuffer, d_outs: Sequence[DeviceBuffer]):
+        op = (_vp * self.n)(*[b.ptr.value for b in d_outs])
+        cp = (C.c_size_t * self.n)(*[b.count for b in d_outs])
+        check(lib().fdsp_channelizer_graph_create(self.h, chunk_len, d_in.ptr, op, cp))
+
+    def graph_set_timing_events(self, start: Optional[Event], stop: Optional[Event]):
+        check(lib().fdsp_channelizer_graph_set_timing_events(self.h, start.h if start else None,
+                                                             stop.h if stop else None))
+
+    def graph_launch(self, stream: Optional[Stream] = None):
+        check(lib().fdsp_channelizer_graph_launch(self.h, stream.h if stream else None))
+
+    def graph_counts(self) -> List[int]:
+        counts = (C.c_size_t * self.n)()
+        check(lib().fdsp_channelizer_graph_counts(self.h, counts))
+        return list(counts)
+
+    def graph_destroy(self):
+        check(lib().fdsp_channelizer_graph_destroy(self.h))
+
+
+# ---------------------------------------------------------------------------
+# free functions (one-shot wrappers, as in the reference)
+# ---------------------------------------------------------------------------
+def freq_shift(in_sample_rate_in_sps: int, x, freq_shift_in_hz):
+    """freq_shift / freq_shift_cuda (reference freq_shift.h:85, freq_shift_cuda.h:140-156)."""
+    return FreqShift(in_sample_rate_in_sps, freq_shift_in_hz).apply(x)
+
+
+def resample(in_sample_rate_in_sps: int, x, filter_coeffs, out_sample_rate_in_sps: int):
+    """resample / resample_cuda (reference src/transform/falcon_dsp_resample.cc:126-137):
+    up/down from rat_approx(out/in, 1024), then one apply."""
+    up, down = rat_approx(out_sample_rate_in_sps / in_sample_rate_in_sps, 1024)
+    if up <= 0 or down <= 0:
+        return None
+    return PolyphaseResampler(up, down, filter_coeffs).apply(x)
+
+
+upfirdn = resample
+
+
+def fir_filter(coeffs, x):
+    """fir_filter / fir_filter_cuda (reference fir_filter.cc:92-97)."""
+    return FirFilter(coeffs).apply(x)
+
+
+def polar_discriminator(x):
+    """polar_discriminator (reference polar_discriminator.cc:91-96)."""
+    return PolarDiscriminator().apply(x)
