@@ -219,11 +219,6 @@ int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_len, const 
 int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t stream);
 int fdsp_channelizer_graph_counts(fdsp_channelizer_t h, size_t *counts);
 int fdsp_channelizer_graph_destroy(fdsp_channelizer_t h);
-/* Optional kernel-level timing: events recorded right before and right after
- * the fused kernel inside the graph, so fdsp_event_elapsed_ms(start, stop)
- * after a launch is that launch's kernel time. Set before graph_create;
- * NULLs remove them. */
-int fdsp_channelizer_graph_set_timing_events(fdsp_channelizer_t h, fdsp_event_t start, fdsp_event_t stop);
 
 /* tile parameters of the batched kernel (set by the autotuner) */
 int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads);

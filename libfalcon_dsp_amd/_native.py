@@ -126,7 +126,6 @@ _SIGS = [
     ("fdsp_channelizer_graph_launch", _i, [_vp, _vp]),
     ("fdsp_channelizer_graph_counts", _i, [_vp, _P(_sz)]),
     ("fdsp_channelizer_graph_destroy", _i, [_vp]),
-    ("fdsp_channelizer_graph_set_timing_events", _i, [_vp, _vp, _vp]),
     ("fdsp_channelizer_set_tiling", _i, [_vp, _u32, _u32]),
     ("fdsp_channelizer_get_tiling", _i, [_vp, _P(_u32), _P(_u32)]),
 ]

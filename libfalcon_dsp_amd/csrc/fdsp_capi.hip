@@ -985,13 +985,20 @@ struct fdsp_channelizer_s {
     DevBuf d_nco, d_full;       // NCO params, full-rate shifted copies (C x n)
     // batched mode
     std::vector<ChanStatic> stat;
-    DevBuf d_stat, d_taps, d_hist[2], d_pos;
+    DevBuf d_stat, d_stat_graph, d_taps, d_hist[2], d_pos;
     ChanStatic *pinned_stat = nullptr;
     hipEvent_t stat_ev = nullptr;   // last table upload
     hipEvent_t use_ev = nullptr;    // last fused launch that read the table
     bool stat_valid = false;
+    bool pos_dirty = false;         // device position behind the host's (eager calls since the last graph launch)
     uint32_t taps_total = 0, hist_len = 0;
     uint32_t seg_len = 4096, block = 256;
+    // work plan: channel groups (kGroupWords u32 each) and their MFMA B tables
+    std::vector<uint32_t> groups;
+    std::vector<float> btab;
+    DevBuf d_groups, d_btab;
+    uint32_t n_groups = 0, lds_bytes = 0;
+    std::vector<uint32_t> prefix;
     // host-pointer path
     DevBuf d_in;
     std::vector<DevBuf> d_outs;
@@ -1000,7 +1007,6 @@ struct fdsp_channelizer_s {
     hipGraphExec_t graph_exec = nullptr;
     size_t graph_chunk = 0;
     std::vector<size_t> graph_counts;
-    hipEvent_t time_start = nullptr, time_stop = nullptr;
 };
 
 namespace {
@@ -1030,12 +1036,190 @@ int chan_upload_outs(fdsp_channelizer_s *h, void *const *outs)
     return FDSP_OK;
 }
 
-uint32_t fit_seg_len(const fdsp_channelizer_s *h, uint32_t want)
+constexpr size_t kLdsCap = 64 * 1024;
+
+// window length of a channel on the MFMA path (its taps padded in front with
+// zeros to 32 or 64), or 0 when the channel does not qualify
+uint32_t mfma_window(const ChanHost &k)
 {
-    uint32_t s = want;
-    while (s > 256 && chan_fused_lds_bytes((uint32_t)h->ch.size(), h->taps_total, s, h->hist_len) > 64 * 1024)
-        s >>= 1;
-    return s;
+    if (k.up != 1)
+        return 0;
+    return k.cpp <= 32 ? 32 : k.cpp <= 64 ? 64 : 0;
+}
+
+// LDS bytes of one tile of `rows` outputs of a group, per kind
+size_t group_lds(uint32_t kind, uint32_t up, uint32_t down, uint32_t w, uint32_t rows, uint32_t block)
+{
+    switch (kind) {
+    case kGroupMfma: {
+        const size_t head = kMfmaMaxChan * sizeof(ChanStatic);
+        if (down < w) {
+            const size_t mc = (((size_t)(rows - 1) * down + w + down - 1) / down) | 1u;
+            return head + (down * mc + 16) * sizeof(float2);
+        }
+        return head + (size_t)w * (rows + 1) * sizeof(float2);
+    }
+    case kGroupVDense: {
+        const size_t mc = (((size_t)(rows - 1) * down + w + down - 1) / down) | 1u;
+        return (down * mc + 64 * kVDenseOpl) * sizeof(float2);
+    }
+    case kGroupVSparse:
+        return (size_t)kVSparsePass * (block + 1) * sizeof(float2);
+    default:
+        return (((size_t)(rows - 1) * down + up - 1) / up + w + 1) * sizeof(float2);
+    }
+}
+
+// Rows per tile of each group for a segment of `seg` inputs per tile and
+// `block` threads, written into word 5 of the group records; the largest
+// LDS need goes to *lds. Fails when a group's smallest tile does not fit.
+int size_groups(std::vector<uint32_t> &groups, uint32_t seg, uint32_t block, uint32_t *lds)
+{
+    size_t most = 0;
+    for (size_t g = 0; g < groups.size() / kGroupWords; ++g) {
+        uint32_t *gw = groups.data() + g * kGroupWords;
+        const uint32_t kind = gw[0], up = gw[1], down = gw[2], w = gw[3];
+        uint32_t rows, step, lo;
+        switch (kind) {
+        case kGroupMfma:
+            step = lo = 16;
+            rows = std::max(lo, seg / down / step * step);
+            if (down >= w)
+                rows = std::min(rows, 256u);
+            break;
+        case kGroupVDense:
+            step = lo = 64;
+            rows = std::min(std::max(lo, seg / down / step * step), block * kVDenseOpl);
+            break;
+        case kGroupVSparse:
+            step = lo = rows = block;
+            break;
+        default:
+            step = lo = 1;
+            rows = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, (uint64_t)seg * up / down), 4 * block);
+            break;
+        }
+        while (rows > lo && group_lds(kind, up, down, w, rows, block) > kLdsCap)
+            rows = kind == kGroupGeneric ? rows / 2 : rows - step;
+        const size_t need = group_lds(kind, up, down, w, rows, block);
+        if (need > kLdsCap || (uint64_t)rows * down + up >= (1ull << 31))
+            return set_err(FDSP_ERANGE, "batched channelizer: a tile of channel %u (up %u, down %u, %u taps per "
+                                        "phase) does not fit one block; use FDSP_CHAN_PLAIN",
+                           gw[8], up, down, w);
+        gw[5] = rows;
+        most = std::max(most, need);
+    }
+    *lds = (uint32_t)most;
+    return FDSP_OK;
+}
+
+// Group the channels. Channels with up 1, the same down and the same padded
+// window (32 or 64) form MFMA groups of up to kMfmaMaxChan when there are at
+// least kMfmaMinChan of them; every other channel is a one-channel VALU group.
+// `taps` holds every channel's modulated taps (ChanStatic::taps_off).
+void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
+{
+    const uint32_t C = (uint32_t)h->ch.size();
+    std::vector<bool> done(C, false);
+    h->groups.clear();
+    h->btab.clear();
+    auto record = [&](uint32_t kind, const ChanHost &k, uint32_t w, const std::vector<uint32_t> &members) {
+        uint32_t gw[kGroupWords] = {kind, k.up, k.down, w, (uint32_t)members.size(), 0, 0, 0};
+        for (size_t i = 0; i < members.size(); ++i)
+            gw[8 + i] = members[i];
+        if (kind == kGroupMfma) {
+            // B operand of step s, lane l: B[k = 4(s mod S) + l/16][j = l mod 16],
+            // S = w/4 steps over the real parts then S over the imaginary parts.
+            // Column 2c gives channel c's real output, 2c+1 its imaginary output.
+            gw[6] = (uint32_t)h->btab.size();
+            const uint32_t S = w / 4;
+            for (uint32_t s = 0; s < 2 * S; ++s)
+                for (uint32_t l = 0; l < 64; ++l) {
+                    const uint32_t t = 4 * (s % S) + l / 16, j = l % 16, c = j / 2;
+                    float v = 0.f;
+                    if (c < members.size()) {
+                        const ChanStatic &st = h->stat[members[c]];
+                        const uint32_t pad = w - st.cpp;
+                        if (t >= pad) {
+                            const float2 g = taps[st.taps_off + t - pad];
+                            if (s < S)
+                                v = (j & 1) ? g.y : g.x;
+                            else
+                                v = (j & 1) ? g.x : -g.y;
+                        }
+                    }
+                    h->btab.push_back(v);
+                }
+        }
+        h->groups.insert(h->groups.end(), gw, gw + kGroupWords);
+    };
+    for (uint32_t c = 0; c < C; ++c) {
+        if (done[c])
+            continue;
+        const ChanHost &k = h->ch[c];
+        const uint32_t w = mfma_window(k);
+        std::vector<uint32_t> same;
+        if (w)
+            for (uint32_t d = c; d < C; ++d)
+                if (!done[d] && mfma_window(h->ch[d]) == w && h->ch[d].down == k.down)
+                    same.push_back(d);
+        if (same.size() >= kMfmaMinChan) {
+            // near-equal split into ceil(n / 8) groups
+            const size_t ng = (same.size() + kMfmaMaxChan - 1) / kMfmaMaxChan;
+            for (size_t i = 0, at = 0; i < ng; ++i) {
+                const size_t take = (same.size() - at) / (ng - i);
+                std::vector<uint32_t> members(same.begin() + at, same.begin() + at + take);
+                record(kGroupMfma, k, w, members);
+                at += take;
+            }
+            for (uint32_t d : same)
+                done[d] = true;
+            continue;
+        }
+        done[c] = true;
+        const uint32_t kind = k.up > 1 ? kGroupGeneric : k.down < k.cpp ? kGroupVDense : kGroupVSparse;
+        record(kind, k, k.cpp, {c});
+    }
+    h->n_groups = (uint32_t)(h->groups.size() / kGroupWords);
+}
+
+// first block of each group for a chunk of n inputs (a chunk emits at most
+// floor(n*up/down) + 1 outputs per channel)
+void group_prefix(fdsp_channelizer_s *h, uint32_t n)
+{
+    h->prefix.assign(1, 0);
+    uint64_t b = 0;
+    for (uint32_t g = 0; g < h->n_groups; ++g) {
+        const uint32_t *gw = h->groups.data() + g * kGroupWords;
+        const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
+        b += (most + gw[5] - 1) / gw[5];
+        h->prefix.push_back((uint32_t)b);
+    }
+}
+
+GroupsArgs groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool device_pos)
+{
+    GroupsArgs f{};
+    f.in = reinterpret_cast<const float2 *>(d_in);
+    f.n = n;
+    f.hist_len = h->hist_len;
+    f.hist[0] = h->d_hist[0].as<float2>();
+    f.hist[1] = h->d_hist[1].as<float2>();
+    f.hist_w[0] = h->d_hist[0].as<float2>();
+    f.hist_w[1] = h->d_hist[1].as<float2>();
+    f.chans = h->d_stat.as<ChanStatic>();
+    f.taps = h->d_taps.as<float2>();
+    f.btab = h->d_btab.as<float>();
+    f.groups = h->d_groups.as<uint32_t>();
+    f.n_groups = h->n_groups;
+    f.block = h->block;
+    f.lds_bytes = h->lds_bytes;
+    f.pos = device_pos ? h->d_pos.as<ChanPos>() : nullptr;
+    f.p = h->p;
+    f.parity = h->parity;
+    group_prefix(h, n);
+    f.prefix = h->prefix.data();
+    return f;
 }
 
 }  // namespace
@@ -1110,6 +1294,8 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             s.nu = k.nco.nu;
             s.r = k.nco.r;
             s.inv_r = k.nco.inv_r;
+            s.inv_up = 1.0 / (double)k.up;
+            s.inv_down = 1.0 / (double)k.down;
             s.up = k.up;
             s.down = k.down;
             s.cpp = k.cpp;
@@ -1127,12 +1313,27 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
                 }
         }
         h->taps_total = (uint32_t)taps.size();
-        h->hist_len = (max_cpp - 1 + 1) & ~1u;  // even, so staged segments stay 16-byte aligned
-        h->seg_len = fit_seg_len(h, 4096);
-        if (chan_fused_lds_bytes(n_chan, h->taps_total, h->seg_len, h->hist_len) > 64 * 1024)
-            rc = set_err(FDSP_ERANGE, "batched channelizer: %u channels with %u taps in total do not fit one "
-                                      "block's LDS; use FDSP_CHAN_PLAIN",
-                         n_chan, h->taps_total);
+        // history: every window's reach back (MFMA windows are padded to 32/64),
+        // even so staged spans stay 16-byte aligned
+        uint32_t max_w = max_cpp;
+        for (auto &k : h->ch)
+            max_w = std::max(max_w, mfma_window(k));
+        h->hist_len = max_w & ~1u;
+        build_groups(h, taps);
+        if (h->n_groups > kGroupsInline)
+            rc = set_err(FDSP_ERANGE, "batched channelizer: %u channel groups exceed %u; use FDSP_CHAN_PLAIN",
+                         h->n_groups, kGroupsInline);
+        if (!rc)
+            rc = size_groups(h->groups, h->seg_len, h->block, &h->lds_bytes);
+        if (!rc)
+            rc = h->d_groups.ensure(h->groups.size() * sizeof(uint32_t));
+        if (!rc)
+            rc = h->d_btab.ensure(std::max<size_t>(h->btab.size(), 1) * sizeof(float));
+        if (!rc && (hipMemcpy(h->d_groups.p, h->groups.data(), h->groups.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice) != hipSuccess ||
+                    (!h->btab.empty() && hipMemcpy(h->d_btab.p, h->btab.data(), h->btab.size() * sizeof(float),
+                                                   hipMemcpyHostToDevice) != hipSuccess)))
+            rc = set_err(FDSP_EHIP, "work plan upload failed");
         const size_t hb = (size_t)(h->hist_len ? h->hist_len : 1) * sizeof(float2);
         if (!rc)
             rc = h->d_taps.ensure(taps.size() * sizeof(float2));
@@ -1196,10 +1397,13 @@ extern "C" int fdsp_channelizer_destroy(fdsp_channelizer_t h)
     h->d_nco.release();
     h->d_full.release();
     h->d_stat.release();
+    h->d_stat_graph.release();
     h->d_taps.release();
     h->d_hist[0].release();
     h->d_hist[1].release();
     h->d_pos.release();
+    h->d_groups.release();
+    h->d_btab.release();
     h->d_in.release();
     for (auto &b : h->d_outs)
         b.release();
@@ -1219,8 +1423,11 @@ extern "C" int fdsp_channelizer_reset(fdsp_channelizer_t h)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
+    // launches on caller streams may still read the history being cleared
+    HIP_TRY(hipDeviceSynchronize());
     h->p = 0;
     h->parity = 0;
+    h->pos_dirty = false;
     for (auto &k : h->ch) {
         k.has_prev = false;
         k.prev_parity = 0;
@@ -1349,24 +1556,11 @@ extern "C" int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf
         } else if (int rc = chan_upload_outs(h, d_outs)) {
             return rc;
         }
-        FusedArgs f{};
-        f.in = reinterpret_cast<const float2 *>(d_in);
-        f.n = (uint32_t)n;
-        f.hist_len = h->hist_len;
-        f.hist[0] = h->d_hist[0].as<float2>();
-        f.hist[1] = h->d_hist[1].as<float2>();
-        f.hist_w[0] = h->d_hist[0].as<float2>();
-        f.hist_w[1] = h->d_hist[1].as<float2>();
-        f.chans = h->d_stat.as<ChanStatic>();
-        f.n_chan = (uint32_t)C;
-        f.taps = h->d_taps.as<float2>();
-        f.taps_total = h->taps_total;
-        f.seg_len = h->seg_len;
-        f.block = h->block;
-        f.pos = h->d_pos.as<ChanPos>();
-        HIP_TRY(launch_chan_fused(f, s));
-        HIP_TRY(launch_chan_advance(h->d_pos.as<ChanPos>(), (uint32_t)n, s));
+        // eager form: position and parity travel as kernel arguments, one launch
+        HIP_TRY(launch_chan_groups(groups_args(h, d_in, (uint32_t)n, false), s));
         HIP_TRY(hipEventRecord(h->use_ev, s));
+        h->parity ^= 1;
+        h->pos_dirty = true;
     }
     // discriminator tails
     for (size_t c = 0; c < C; ++c) {
@@ -1462,34 +1656,22 @@ extern "C" int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_
     if (int rc = check_chunk(chunk_len, max_up))
         return rc;
     fdsp_channelizer_graph_destroy(h);
-    if (int rc = chan_upload_outs(h, d_outs))
+    // the graph keeps its own channel table (its output buffers are fixed)
+    std::vector<ChanStatic> gstat = h->stat;
+    for (size_t c = 0; c < C; ++c)
+        gstat[c].out = d_outs[c];
+    if (int rc = h->d_stat_graph.ensure(sizeof(ChanStatic) * C))
         return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    FusedArgs f{};
-    f.in = reinterpret_cast<const float2 *>(d_in);
-    f.n = (uint32_t)chunk_len;
-    f.hist_len = h->hist_len;
-    f.hist[0] = h->d_hist[0].as<float2>();
-    f.hist[1] = h->d_hist[1].as<float2>();
-    f.hist_w[0] = h->d_hist[0].as<float2>();
-    f.hist_w[1] = h->d_hist[1].as<float2>();
-    f.chans = h->d_stat.as<ChanStatic>();
-    f.n_chan = (uint32_t)C;
-    f.taps = h->d_taps.as<float2>();
-    f.taps_total = h->taps_total;
-    f.seg_len = h->seg_len;
-    f.block = h->block;
-    f.pos = h->d_pos.as<ChanPos>();
+    HIP_TRY(hipMemcpy(h->d_stat_graph.p, gstat.data(), sizeof(ChanStatic) * C, hipMemcpyHostToDevice));
+    GroupsArgs f = groups_args(h, d_in, (uint32_t)chunk_len, true);
+    f.chans = h->d_stat_graph.as<ChanStatic>();
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e0 = h->time_start ? hipEventRecord(h->time_start, h->stream) : hipSuccess;
-    hipError_t e1 = launch_chan_fused(f, h->stream);
-    hipError_t e4 = h->time_stop ? hipEventRecord(h->time_stop, h->stream) : hipSuccess;
+    hipError_t e1 = launch_chan_groups(f, h->stream);
     hipError_t e2 = launch_chan_advance(h->d_pos.as<ChanPos>(), (uint32_t)chunk_len, h->stream);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(h->stream, &g);
-    HIP_TRY(e0);
     HIP_TRY(e1);
-    HIP_TRY(e4);
     HIP_TRY(e2);
     HIP_TRY(e3);
     h->graph = g;
@@ -1505,10 +1687,19 @@ extern "C" int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t
         return set_err(FDSP_ENOTINIT, "null handle");
     if (!h->graph_exec)
         return set_err(FDSP_ENOTINIT, "no graph: call fdsp_channelizer_graph_create first");
+    hipStream_t s = pick(h->stream, stream);
+    if (h->pos_dirty) {
+        // eager calls advanced the host position only: bring the device copy along
+        const ChanPos pos{h->p, h->parity, 0};
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy(h->d_pos.p, &pos, sizeof(pos), hipMemcpyHostToDevice));
+        h->pos_dirty = false;
+    }
     for (size_t c = 0; c < h->ch.size(); ++c)
         h->graph_counts[c] = chunk_count(h->p, h->graph_chunk, h->ch[c].up, h->ch[c].down);
-    HIP_TRY(hipGraphLaunch(h->graph_exec, pick(h->stream, stream)));
+    HIP_TRY(hipGraphLaunch(h->graph_exec, s));
     h->p += h->graph_chunk;
+    h->parity ^= 1;
     return FDSP_OK;
 }
 
@@ -1523,18 +1714,6 @@ extern "C" int fdsp_channelizer_graph_counts(fdsp_channelizer_t h, size_t *count
     return FDSP_OK;
 }
 
-extern "C" int fdsp_channelizer_graph_set_timing_events(fdsp_channelizer_t h, fdsp_event_t start,
-                                                        fdsp_event_t stop)
-{
-    if (!h)
-        return set_err(FDSP_ENOTINIT, "null handle");
-    if ((start == nullptr) != (stop == nullptr))
-        return set_err(FDSP_EINVAL, "pass both events or neither");
-    h->time_start = static_cast<hipEvent_t>(start);
-    h->time_stop = static_cast<hipEvent_t>(stop);
-    return FDSP_OK;
-}
-
 extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads)
 {
     if (!h)
@@ -1543,12 +1722,19 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
         return set_err(FDSP_EINVAL, "tiling applies to the batched mode");
     if (segment_len < 256 || segment_len > (1u << 16) || (segment_len & (segment_len - 1)))
         return set_err(FDSP_EINVAL, "segment_len must be a power of two in [256, 65536]");
-    if (block_threads != 128 && block_threads != 256 && block_threads != 512 && block_threads != 1024)
-        return set_err(FDSP_EINVAL, "block_threads must be 128, 256, 512 or 1024");
-    if (chan_fused_lds_bytes((uint32_t)h->ch.size(), h->taps_total, segment_len, h->hist_len) > 64 * 1024)
-        return set_err(FDSP_ERANGE, "segment_len %u does not fit one block's LDS", segment_len);
+    if (block_threads != 128 && block_threads != 256 && block_threads != 512)
+        return set_err(FDSP_EINVAL, "block_threads must be 128, 256 or 512");
     if (h->graph_exec)
         return set_err(FDSP_EBUSY, "destroy the graph before retiling");
+    std::vector<uint32_t> groups = h->groups;
+    uint32_t lds = 0;
+    if (int rc = size_groups(groups, segment_len, block_threads, &lds))
+        return rc;
+    // launches already queued read the device plan: let them finish first
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h->d_groups.p, groups.data(), groups.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    h->groups.swap(groups);
+    h->lds_bytes = lds;
     h->seg_len = segment_len;
     h->block = block_threads;
     return FDSP_OK;

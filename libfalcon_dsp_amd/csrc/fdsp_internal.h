@@ -26,14 +26,15 @@ __device__ __forceinline__ void cmac(float2 &acc, float2 a, float2 b)
 
 // e^{j 2 pi nu k}: the phase is formed and reduced to [-1/2, 1/2] revolutions
 // in FP64 (k < 2^32 and |nu * k| < 2^21, so the reduced phase carries
-// < 1e-9 revolutions of error), then FP32 sincospi evaluates it.
+// < 1e-9 revolutions of error), then the hardware v_sin_f32 / v_cos_f32,
+// which take their argument in revolutions, evaluate it (max abs error
+// 1.3e-7 over [-1/2, 1/2], measured against FP64 on gfx950).
 __device__ __forceinline__ float2 nco_phasor(int64_t k, double nu)
 {
     double t = (double)k * nu;
     t -= rint(t);
-    float s, c;
-    sincospif(2.0f * (float)t, &s, &c);
-    return make_float2(c, s);
+    const float tf = (float)t;
+    return make_float2(__builtin_amdgcn_cosf(tf), __builtin_amdgcn_sinf(tf));
 }
 
 // a mod m for a < 2^52, m >= 1, using an FP64 quotient estimate plus one
@@ -66,6 +67,7 @@ struct NcoChan {
 struct ChanStatic {
     double nu;        // f/fs (0 for the identity shift)
     double inv_r;     // 1/R
+    double inv_up, inv_down;
     uint32_t r;       // rollover index
     uint32_t up, down, cpp;
     uint32_t taps_off;  // offset of this channel's modulated taps (up*cpp complex)
@@ -108,25 +110,48 @@ hipError_t launch_fir_bank(const float2 *in, uint32_t n, uint64_t in_stride, con
 hipError_t launch_discriminator(const float2 *in, uint32_t n, const float2 *prev, int has_prev,
                                 float2 *prev_out, float *out, hipStream_t s);
 
-// fused multi-channel shift + resample (batched channelizer)
-struct FusedArgs {
+// Work plan of the fused channelizer, built on the host at create time: the
+// channels fall in groups, each computed by its own row tiles (one block per
+// tile of `rows` consecutive outputs). Group record, kGroupWords u32 each:
+//   [0] kind  [1] up  [2] down  [3] taps per window (padded to 32/64 for MFMA)
+//   [4] channels m  [5] rows per tile  [6] offset of the MFMA B table (floats)
+//   [7] unused  [8 .. 8+m) channel indices
+enum : uint32_t {
+    kGroupMfma = 0,    // 3..8 channels, up 1, equal down and padded taps: FP32 MFMA
+    kGroupVDense = 1,  // one channel, up 1, down < cpp: VALU over a transposed span
+    kGroupVSparse = 2, // one channel, up 1, down >= cpp: VALU over gathered windows
+    kGroupGeneric = 3, // one channel, up > 1: VALU with per-output filter phase
+};
+constexpr uint32_t kGroupWords = 16;
+constexpr uint32_t kMfmaMaxChan = 8;
+constexpr uint32_t kMfmaMinChan = 3;
+// groups whose first-block prefix travels in the kernel arguments
+constexpr uint32_t kGroupsInline = 256;
+// taps per LDS pass of the one-channel sparse path
+constexpr uint32_t kVSparsePass = 8;
+// outputs per lane of the one-channel dense path
+constexpr uint32_t kVDenseOpl = 2;
+
+struct GroupsArgs {
     const float2 *in;
     uint32_t n;
-    uint32_t hist_len;          // H: history samples kept (max cpp - 1, rounded up)
+    uint32_t hist_len;          // H: history samples kept (>= every window - 1, even)
     const float2 *hist[2];      // ping-pong history buffers
     float2 *hist_w[2];
     const ChanStatic *chans;
-    uint32_t n_chan;
     const float2 *taps;         // all channels' modulated taps
-    uint32_t taps_total;        // complex entries
-    uint32_t seg_len;           // input samples per block
-    uint32_t block;             // threads per block
-    ChanPos *pos;               // device position (read by the kernel)
+    const float *btab;          // MFMA B tables in lane order
+    const uint32_t *groups;     // n_groups records of kGroupWords
+    uint32_t n_groups;
+    uint32_t block;             // threads per block: 128, 256 or 512
+    uint32_t lds_bytes;         // dynamic LDS per block
+    const ChanPos *pos;         // device position (graph mode), or null:
+    uint64_t p;                 //   then the position and parity come from here
+    uint32_t parity;
+    const uint32_t *prefix;     // host: first block of each group, n_groups + 1 entries
 };
-hipError_t launch_chan_fused(const FusedArgs &a, hipStream_t s);
+hipError_t launch_chan_groups(const GroupsArgs &a, hipStream_t s);
 // advance the device position by n inputs and flip the history parity
 hipError_t launch_chan_advance(ChanPos *pos, uint32_t n, hipStream_t s);
-// LDS bytes one block of the fused kernel needs
-size_t chan_fused_lds_bytes(uint32_t n_chan, uint32_t taps_total, uint32_t seg_len, uint32_t hist_len);
 
 }  // namespace fdsp

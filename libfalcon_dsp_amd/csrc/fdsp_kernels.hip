@@ -432,205 +432,4 @@ hipError_t launch_discriminator(const float2 *in, uint32_t n, const float2 *prev
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Fused multi-rate channelizer (all channels, one launch, input read once)
-// (replaces falcon_dsp_multi_rate_channelizer{,_cuda}::apply,
-//  reference src/transform/falcon_dsp_multi_rate_channelizer.cc:146-164,
-//  _cuda.cu:375-514)
-//
-// Shift-then-filter equals filter-with-modulated-taps then one phasor per
-// output: with k(a) the NCO index of the output's anchor sample a and
-// d = a - i the distance of window sample i,
-//   y = e^{j2pi nu k(a)} * sum_t [h_t e^{-j2pi nu d_t}] x[i_t]
-// as long as no rollover (k restarting at 0) falls inside the window; when one
-// does (k(a) < cpp-1, rare: R >= 1e5), the samples before it carry an extra
-// e^{j2pi nu R}. Block b stages input segment b (plus history) in LDS once and
-// computes every channel's outputs whose anchors lie in the segment; the
-// (channel, output) pairs are flattened so all threads stay busy.
-// ---------------------------------------------------------------------------
-struct FusedDev {
-    const float2 *in;
-    uint32_t n;
-    uint32_t hist_len;
-    const float2 *hist0, *hist1;
-    float2 *histw0, *histw1;
-    const ChanStatic *chans;
-    uint32_t n_chan;
-    const float2 *taps;
-    uint32_t taps_total;
-    uint32_t seg_len;
-    const ChanPos *pos;
-};
-
-template <int kBlock>
-__global__ __launch_bounds__(kBlock) void chan_fused_kernel(FusedDev a)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t C = a.n_chan;
-    const uint32_t H = a.hist_len;
-    const uint32_t S = a.seg_len;
-    ChanStatic *sc = reinterpret_cast<ChanStatic *>(smem);
-    float2 *tp = reinterpret_cast<float2 *>(sc + C);
-    float2 *xs = tp + a.taps_total;
-    uint32_t *s_pref = reinterpret_cast<uint32_t *>(xs + S + H);  // C+1
-    uint32_t *s_qb = s_pref + C + 1;
-    uint32_t *s_rb = s_qb + C;
-    uint32_t *s_jb = s_rb + C;
-    uint32_t *s_ks = s_jb + C;
-
-    const uint64_t P = a.pos->p;
-    const uint32_t par = a.pos->parity;
-    const float2 *hist = par ? a.hist1 : a.hist0;
-    const uint32_t seg0 = blockIdx.x * S;
-    const uint32_t seg1 = min(seg0 + S, a.n);
-
-    if (blockIdx.x == 0) {
-        float2 *hw = par ? a.histw0 : a.histw1;
-        for (uint32_t t = threadIdx.x; t < H; t += kBlock) {
-            const int64_t i = (int64_t)a.n - H + t;
-            hw[t] = i >= 0 ? a.in[i] : hist[H + i];
-        }
-    }
-
-    // stage input segment with H samples of history before it
-    const uint32_t span = seg1 - seg0 + H;
-    for (uint32_t t = threadIdx.x; t < span; t += kBlock) {
-        const int64_t i = (int64_t)seg0 - H + t;
-        xs[t] = i >= 0 ? a.in[i] : hist[H + i];
-    }
-    for (uint32_t t = threadIdx.x; t < a.taps_total; t += kBlock)
-        tp[t] = a.taps[t];
-    // per-channel output range of this segment
-    for (uint32_t c = threadIdx.x; c < C; c += kBlock) {
-        const ChanStatic ch = a.chans[c];
-        sc[c] = ch;
-        const uint64_t g0 = P + seg0, g1 = P + seg1;
-        const uint64_t nlo = ceil_div_u64(g0 * ch.up, ch.down);
-        const uint64_t nhi = ceil_div_u64(g1 * ch.up, ch.down);
-        const uint64_t n0 = ceil_div_u64(P * ch.up, ch.down);
-        const uint64_t num = nlo * ch.down;
-        const uint64_t qb = num / ch.up;
-        s_qb[c] = (uint32_t)(qb - g0);
-        s_rb[c] = (uint32_t)(num - qb * ch.up);
-        s_jb[c] = (uint32_t)(nlo - n0);
-        s_ks[c] = (uint32_t)mod_u64(g0, ch.r, ch.inv_r);
-        s_pref[c + 1] = (uint32_t)(nhi - nlo);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        // inclusive scan of the counts in s_pref[1..C], 64 channels at a time
-        uint32_t carry = 0;
-        for (uint32_t base = 0; base < C; base += 64) {
-            const uint32_t c = base + threadIdx.x;
-            uint32_t v = c < C ? s_pref[c + 1] : 0;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint32_t u = __shfl_up(v, off, 64);
-                if ((int)threadIdx.x >= off)
-                    v += u;
-            }
-            if (c < C)
-                s_pref[c + 1] = v + carry;
-            carry += __shfl(v, 63, 64);
-        }
-        if (threadIdx.x == 0)
-            s_pref[0] = 0;
-    }
-    __syncthreads();
-
-    const uint32_t total = s_pref[C];
-    uint32_t c = 0;
-    for (uint32_t q = threadIdx.x; q < total; q += kBlock) {
-        while (q >= s_pref[c + 1])
-            ++c;
-        const uint32_t qq = q - s_pref[c];
-        const uint32_t up = sc[c].up, cpp = sc[c].cpp;
-        const uint32_t num = s_rb[c] + qq * sc[c].down;
-        const uint32_t adv = (up == 1) ? num : num / up;
-        const uint32_t ph = num - adv * up;
-        const uint32_t al = s_qb[c] + adv;  // anchor, relative to seg0
-        const float2 *xw = xs + (H + al - (cpp - 1));
-        const float2 *g = tp + sc[c].taps_off + ph * cpp;
-        uint64_t k = (uint64_t)s_ks[c] + al;  // NCO index of the anchor (R < 2^32, al < seg_len < R)
-        const uint32_t r = sc[c].r;
-        if (k >= r)
-            k -= r;
-        float2 acc = make_float2(0.f, 0.f);
-        if (k >= cpp - 1) {
-            for (uint32_t t = 0; t < cpp; ++t)
-                cmac(acc, g[t], xw[t]);
-        } else {
-            // a rollover falls inside the window: samples t < split precede it
-            const uint32_t split = cpp - 1 - (uint32_t)k;
-            float2 acc2 = make_float2(0.f, 0.f);
-            for (uint32_t t = 0; t < split; ++t)
-                cmac(acc2, g[t], xw[t]);
-            for (uint32_t t = split; t < cpp; ++t)
-                cmac(acc, g[t], xw[t]);
-            const float2 w = cmul(sc[c].wrap, acc2);
-            acc.x += w.x;
-            acc.y += w.y;
-        }
-        const float2 y = cmul(nco_phasor((int64_t)k, sc[c].nu), acc);
-        reinterpret_cast<float2 *>(sc[c].out)[s_jb[c] + qq] = y;
-    }
-}
-
-__global__ void chan_advance_kernel(ChanPos *pos, uint32_t n)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        pos->p += n;
-        pos->parity ^= 1u;
-    }
-}
-
-size_t chan_fused_lds_bytes(uint32_t n_chan, uint32_t taps_total, uint32_t seg_len, uint32_t hist_len)
-{
-    return (size_t)n_chan * sizeof(ChanStatic) + (size_t)taps_total * sizeof(float2) +
-           (size_t)(seg_len + hist_len) * sizeof(float2) + (size_t)(5 * n_chan + 1) * sizeof(uint32_t);
-}
-
-hipError_t launch_chan_fused(const FusedArgs &f, hipStream_t s)
-{
-    FusedDev d;
-    d.in = f.in;
-    d.n = f.n;
-    d.hist_len = f.hist_len;
-    d.hist0 = f.hist[0];
-    d.hist1 = f.hist[1];
-    d.histw0 = f.hist_w[0];
-    d.histw1 = f.hist_w[1];
-    d.chans = f.chans;
-    d.n_chan = f.n_chan;
-    d.taps = f.taps;
-    d.taps_total = f.taps_total;
-    d.seg_len = f.seg_len;
-    d.pos = f.pos;
-    const uint32_t blocks = f.n ? (f.n + f.seg_len - 1) / f.seg_len : 1;
-    const size_t lds = chan_fused_lds_bytes(f.n_chan, f.taps_total, f.seg_len, f.hist_len);
-    if (lds > 64 * 1024)
-        return hipErrorInvalidValue;
-    switch (f.block) {
-    case 128:
-        hipLaunchKernelGGL(chan_fused_kernel<128>, dim3(blocks), dim3(128), lds, s, d);
-        break;
-    case 512:
-        hipLaunchKernelGGL(chan_fused_kernel<512>, dim3(blocks), dim3(512), lds, s, d);
-        break;
-    case 1024:
-        hipLaunchKernelGGL(chan_fused_kernel<1024>, dim3(blocks), dim3(1024), lds, s, d);
-        break;
-    default:
-        hipLaunchKernelGGL(chan_fused_kernel<256>, dim3(blocks), dim3(256), lds, s, d);
-        break;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_chan_advance(ChanPos *pos, uint32_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(chan_advance_kernel, dim3(1), dim3(64), 0, s, pos, n);
-    return hipGetLastError();
-}
-
 }  // namespace fdsp

@@ -449,10 +449,6 @@ class MultiRateChannelizer(_Handle):
         cp = (C.c_size_t * self.n)(*[b.count for b in d_outs])
         check(lib().fdsp_channelizer_graph_create(self.h, chunk_len, d_in.ptr, op, cp))
 
-    def graph_set_timing_events(self, start: Optional[Event], stop: Optional[Event]):
-        check(lib().fdsp_channelizer_graph_set_timing_events(self.h, start.h if start else None,
-                                                             stop.h if stop else None))
-
     def graph_launch(self, stream: Optional[Stream] = None):
         check(lib().fdsp_channelizer_graph_launch(self.h, stream.h if stream else None))
 

@@ -80,15 +80,49 @@ def test_batched_equals_plain_and_state():
             assert a.state(c) == b.state(c)
 
 
-def test_alone_equals_in_the_mix():
-    # each channel's output does not depend on which other channels run
-    fs = 20_000_000
-    chans = _chans_small(fs)
+def _chans_grouped(fs=1_000_000):
+    # channels sharing a rate form matrix-core groups: dense windows (down 20,
+    # 25 and 4 < taps), sparse windows (down 80 >= 64 taps), windows padded
+    # from 5 and 48 taps, an identity shift, and nine channels of one rate
+    # (split into groups of 5 and 4)
+    return ([channel(fs, f, 50_000) for f in (23_000, -7, 333_333, 0)]
+            + [channel(fs, f, 12_500, taps_per_up=48) for f in (100_001, -250_000, 7)]
+            + [channel(fs, f, 250_000, taps_per_up=5) for f in (1, -499_999, 125_000)]
+            + [channel(fs, 3_000 * k + 11, 40_000) for k in range(9)])
+
+
+def test_grouped_channels_chunked():
+    # R = 1000 for the 23 kHz shift: windows straddle NCO rollovers often
+    fs = 1_000_000
+    x = cwgn(21, 700_000)
+    _compare(fs, _chans_grouped(fs), x, [434_780, 5, 1, 131_071, 3], True)
+
+
+@pytest.mark.parametrize("seg,block", [(256, 128), (2048, 512), (16384, 256)])
+def test_grouped_tilings_agree(seg, block):
+    fs = 1_000_000
+    chans = _chans_grouped(fs)
+    x = cwgn(22, 200_000)
+    base = fd.MultiRateChannelizer(fs, chans).apply(x)
+    g = fd.MultiRateChannelizer(fs, chans)
+    g.set_tiling(seg, block)
+    for a, b in zip(base, g.apply(x)):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("chans_of", [_chans_small, _chans_grouped])
+def test_alone_equals_in_the_mix(chans_of):
+    # each channel's output does not depend on which other channels run (a
+    # channel alone takes the one-channel path, in the mix it may share a
+    # matrix-core group, so the two agree within the tolerance)
+    fs = 20_000_000 if chans_of is _chans_small else 1_000_000
+    chans = chans_of(fs)
     x = cwgn(4, 200_000)
     mix = fd.MultiRateChannelizer(fs, chans).apply(x)
     for c, ch in enumerate(chans):
         alone = fd.MultiRateChannelizer(fs, [ch]).apply(x)[0]
-        np.testing.assert_array_equal(alone, mix[c])
+        assert alone.size == mix[c].size
+        assert_close_rms(alone, mix[c], rms(x), what=f"chan {c}")
 
 
 def test_reset_replays_identically():
@@ -118,12 +152,13 @@ def test_discriminator_tail():
         want = dref.apply(ref.apply(x[lo:hi])[0])
         assert got.dtype == np.float32 and got.size == want.size
         d = np.angle(np.exp(1j * (got.astype(np.float64) - want)))
-        assert np.max(np.abs(d)) <= 1e-3
+        assert d.size == 0 or np.max(np.abs(d)) <= 1e-3
 
 
-def test_graph_replays_consecutive_chunks():
-    fs = 20_000_000
-    chans = _chans_small(fs)
+@pytest.mark.parametrize("chans_of", [_chans_small, _chans_grouped])
+def test_graph_replays_consecutive_chunks(chans_of):
+    fs = 20_000_000 if chans_of is _chans_small else 1_000_000
+    chans = chans_of(fs)
     chunk = 65_536
     x = cwgn(8, chunk * 5)
     g = fd.MultiRateChannelizer(fs, chans)
@@ -157,7 +192,7 @@ def test_workload_chan_32_slice():
             assert_close_rms(ys[c], rs[c], xr, what=f"chan {c}")
 
 
-@pytest.mark.parametrize("seg,block", [(256, 128), (1024, 256), (2048, 512), (8192, 1024)])
+@pytest.mark.parametrize("seg,block", [(256, 128), (1024, 256), (2048, 512), (512, 512), (4096, 512)])
 def test_tilings_agree(seg, block):
     fs = 20_000_000
     chans = _chans_small(fs)
