@@ -220,7 +220,11 @@ int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t stream);
 int fdsp_channelizer_graph_counts(fdsp_channelizer_t h, size_t *counts);
 int fdsp_channelizer_graph_destroy(fdsp_channelizer_t h);
 
-/* tile parameters of the batched kernel (set by the autotuner) */
+/* Work split of the batched kernel (picked by the autotuner): one block runs
+ * consecutive output tiles of one channel group until it has covered about
+ * segment_len input samples (a power of two in [256, 2^20]); block_threads is
+ * 128, 256 or 512. Outputs agree across settings to float32 rounding of the
+ * NCO phasor (a few ulp). */
 int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads);
 int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads);
 

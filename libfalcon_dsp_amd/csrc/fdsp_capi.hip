@@ -992,13 +992,13 @@ struct fdsp_channelizer_s {
     bool stat_valid = false;
     bool pos_dirty = false;         // device position behind the host's (eager calls since the last graph launch)
     uint32_t taps_total = 0, hist_len = 0;
-    uint32_t seg_len = 4096, block = 256;
+    uint32_t seg_len = 8192, block = 256;
     // work plan: channel groups (kGroupWords u32 each) and their MFMA B tables
     std::vector<uint32_t> groups;
     std::vector<float> btab;
     DevBuf d_groups, d_btab;
     uint32_t n_groups = 0, lds_bytes = 0;
-    std::vector<uint32_t> prefix;
+    std::vector<uint32_t> prefix, tps;
     // host-pointer path
     DevBuf d_in;
     std::vector<DevBuf> d_outs;
@@ -1051,14 +1051,15 @@ uint32_t mfma_window(const ChanHost &k)
 size_t group_lds(uint32_t kind, uint32_t up, uint32_t down, uint32_t w, uint32_t rows, uint32_t block)
 {
     switch (kind) {
-    case kGroupMfma: {
-        const size_t head = kMfmaMaxChan * sizeof(ChanStatic);
+    case kGroupMfma:
+        // the group's taps, then the whole staging capacity: dense spans
+        // transposed by down (mc columns, rounded up to odd), sparse windows
+        // at a pitch of w + 1
         if (down < w) {
-            const size_t mc = (((size_t)(rows - 1) * down + w + down - 1) / down) | 1u;
-            return head + (down * mc + 16) * sizeof(float2);
+            const size_t cap = (size_t)mfma_stage(block) * block;
+            return (kMfmaMaxChan * w + down * (((cap + down - 1) / down) | 1u)) * sizeof(float2);
         }
-        return head + (size_t)w * (rows + 1) * sizeof(float2);
-    }
+        return (kMfmaMaxChan * w + (size_t)mfma_stage(block) * (block / w) * (w + 1)) * sizeof(float2);
     case kGroupVDense: {
         const size_t mc = (((size_t)(rows - 1) * down + w + down - 1) / down) | 1u;
         return (down * mc + 64 * kVDenseOpl) * sizeof(float2);
@@ -1070,46 +1071,55 @@ size_t group_lds(uint32_t kind, uint32_t up, uint32_t down, uint32_t w, uint32_t
     }
 }
 
-// Rows per tile of each group for a segment of `seg` inputs per tile and
-// `block` threads, written into word 5 of the group records; the largest
-// LDS need goes to *lds. Fails when a group's smallest tile does not fit.
-int size_groups(std::vector<uint32_t> &groups, uint32_t seg, uint32_t block, uint32_t *lds)
+// Rows per tile of each group for `block` threads, written into word 5 of
+// the group records; the launch's LDS size (the largest need) goes to *lds.
+// MFMA tiles hold what one staging pass of mfma_stage(block) samples per
+// thread brings in; the one-channel tiles are sized to the same LDS budget where
+// they can be (all blocks of the launch get the same LDS allocation).
+int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t *lds)
 {
-    size_t most = 0;
-    for (size_t g = 0; g < groups.size() / kGroupWords; ++g) {
-        uint32_t *gw = groups.data() + g * kGroupWords;
-        const uint32_t kind = gw[0], up = gw[1], down = gw[2], w = gw[3];
-        uint32_t rows, step, lo;
-        switch (kind) {
-        case kGroupMfma:
-            step = lo = 16;
-            rows = std::max(lo, seg / down / step * step);
-            if (down >= w)
-                rows = std::min(rows, 256u);
-            break;
-        case kGroupVDense:
-            step = lo = 64;
-            rows = std::min(std::max(lo, seg / down / step * step), block * kVDenseOpl);
-            break;
-        case kGroupVSparse:
-            step = lo = rows = block;
-            break;
-        default:
-            step = lo = 1;
-            rows = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, (uint64_t)seg * up / down), 4 * block);
-            break;
+    const size_t G = groups.size() / kGroupWords;
+    size_t budget = 32 * 1024;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (size_t g = 0; g < G; ++g) {
+            uint32_t *gw = groups.data() + g * kGroupWords;
+            const uint32_t kind = gw[0], up = gw[1], down = gw[2], w = gw[3];
+            if ((kind == kGroupMfma) != (pass == 0))
+                continue;
+            const size_t cap = pass == 0 ? kLdsCap : budget;
+            uint32_t rows, step, lo;
+            switch (kind) {
+            case kGroupMfma: {
+                const uint32_t stage = mfma_stage(block) * block;
+                rows = down < w ? ((stage - w) / down + 1) : stage / w;
+                step = lo = 16;
+                rows = std::max(lo, rows / step * step);
+                break;
+            }
+            case kGroupVDense:
+                step = lo = 64;
+                rows = block * kVDenseOpl;
+                break;
+            case kGroupVSparse:
+                step = lo = rows = block;
+                break;
+            default:
+                step = lo = 1;
+                rows = 2 * block;
+                break;
+            }
+            while (rows > lo && group_lds(kind, up, down, w, rows, block) > cap)
+                rows = kind == kGroupGeneric ? rows / 2 : rows - step;
+            const size_t need = group_lds(kind, up, down, w, rows, block);
+            if (need > kLdsCap || (uint64_t)rows * down + up >= (1ull << 31))
+                return set_err(FDSP_ERANGE, "batched channelizer: a tile of channel %u (up %u, down %u, %u taps per "
+                                            "phase) does not fit one block; use FDSP_CHAN_PLAIN",
+                               gw[8], up, down, w);
+            gw[5] = rows;
+            budget = std::max(budget, need);
         }
-        while (rows > lo && group_lds(kind, up, down, w, rows, block) > kLdsCap)
-            rows = kind == kGroupGeneric ? rows / 2 : rows - step;
-        const size_t need = group_lds(kind, up, down, w, rows, block);
-        if (need > kLdsCap || (uint64_t)rows * down + up >= (1ull << 31))
-            return set_err(FDSP_ERANGE, "batched channelizer: a tile of channel %u (up %u, down %u, %u taps per "
-                                        "phase) does not fit one block; use FDSP_CHAN_PLAIN",
-                           gw[8], up, down, w);
-        gw[5] = rows;
-        most = std::max(most, need);
     }
-    *lds = (uint32_t)most;
+    *lds = (uint32_t)budget;
     return FDSP_OK;
 }
 
@@ -1163,7 +1173,7 @@ void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
             for (uint32_t d = c; d < C; ++d)
                 if (!done[d] && mfma_window(h->ch[d]) == w && h->ch[d].down == k.down)
                     same.push_back(d);
-        if (same.size() >= kMfmaMinChan) {
+        if (same.size() >= kMfmaMinChan && k.down <= kMfmaMaxDown) {
             // near-equal split into ceil(n / 8) groups
             const size_t ng = (same.size() + kMfmaMaxChan - 1) / kMfmaMaxChan;
             for (size_t i = 0, at = 0; i < ng; ++i) {
@@ -1183,16 +1193,22 @@ void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
     h->n_groups = (uint32_t)(h->groups.size() / kGroupWords);
 }
 
-// first block of each group for a chunk of n inputs (a chunk emits at most
-// floor(n*up/down) + 1 outputs per channel)
+// Blocks of a chunk of n inputs: each block runs a stripe of consecutive
+// tiles of one group covering about seg_len inputs; a chunk emits at most
+// floor(n*up/down) + 1 outputs per channel.
 void group_prefix(fdsp_channelizer_s *h, uint32_t n)
 {
     h->prefix.assign(1, 0);
+    h->tps.assign(h->n_groups, 1);
     uint64_t b = 0;
     for (uint32_t g = 0; g < h->n_groups; ++g) {
         const uint32_t *gw = h->groups.data() + g * kGroupWords;
         const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
-        b += (most + gw[5] - 1) / gw[5];
+        const uint64_t tiles = (most + gw[5] - 1) / gw[5];
+        const uint64_t span = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
+        const uint64_t tps = std::min<uint64_t>(std::max<uint64_t>(1, h->seg_len / span), tiles);
+        h->tps[g] = (uint32_t)tps;
+        b += (tiles + tps - 1) / tps;
         h->prefix.push_back((uint32_t)b);
     }
 }
@@ -1214,11 +1230,15 @@ GroupsArgs groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n,
     f.n_groups = h->n_groups;
     f.block = h->block;
     f.lds_bytes = h->lds_bytes;
+    f.wide = false;
+    for (uint32_t g = 0; g < h->n_groups; ++g)
+        f.wide |= h->groups[g * kGroupWords] == kGroupMfma && h->groups[g * kGroupWords + 3] == 64;
     f.pos = device_pos ? h->d_pos.as<ChanPos>() : nullptr;
     f.p = h->p;
     f.parity = h->parity;
     group_prefix(h, n);
     f.prefix = h->prefix.data();
+    f.tps = h->tps.data();
     return f;
 }
 
@@ -1324,7 +1344,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             rc = set_err(FDSP_ERANGE, "batched channelizer: %u channel groups exceed %u; use FDSP_CHAN_PLAIN",
                          h->n_groups, kGroupsInline);
         if (!rc)
-            rc = size_groups(h->groups, h->seg_len, h->block, &h->lds_bytes);
+            rc = size_groups(h->groups, h->block, &h->lds_bytes);
         if (!rc)
             rc = h->d_groups.ensure(h->groups.size() * sizeof(uint32_t));
         if (!rc)
@@ -1720,15 +1740,15 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
         return set_err(FDSP_ENOTINIT, "null handle");
     if (h->mode != FDSP_CHAN_BATCHED)
         return set_err(FDSP_EINVAL, "tiling applies to the batched mode");
-    if (segment_len < 256 || segment_len > (1u << 16) || (segment_len & (segment_len - 1)))
-        return set_err(FDSP_EINVAL, "segment_len must be a power of two in [256, 65536]");
+    if (segment_len < 256 || segment_len > (1u << 20) || (segment_len & (segment_len - 1)))
+        return set_err(FDSP_EINVAL, "segment_len must be a power of two in [256, 2^20]");
     if (block_threads != 128 && block_threads != 256 && block_threads != 512)
         return set_err(FDSP_EINVAL, "block_threads must be 128, 256 or 512");
     if (h->graph_exec)
         return set_err(FDSP_EBUSY, "destroy the graph before retiling");
     std::vector<uint32_t> groups = h->groups;
     uint32_t lds = 0;
-    if (int rc = size_groups(groups, segment_len, block_threads, &lds))
+    if (int rc = size_groups(groups, block_threads, &lds))
         return rc;
     // launches already queued read the device plan: let them finish first
     HIP_TRY(hipDeviceSynchronize());

@@ -42,6 +42,15 @@ namespace fdsp {
 namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+// Output pointers come from the channel table in memory, so the compiler
+// cannot infer their address space; flat stores would also count against
+// the LDS wait counter and stall every following LDS read until they land.
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) fvec2 *gvec2_ptr;
+__device__ __forceinline__ void store_out(void *base, uint32_t j, float2 v)
+{
+    ((gvec2_ptr)base)[j] = fvec2{v.x, v.y};
+}
 
 struct GroupsDev {
     const float2 *in;
@@ -57,8 +66,9 @@ struct GroupsDev {
     uint32_t parity;
     const uint32_t *groups;
     uint32_t n_groups;
-    // first block of each group (n_groups + 1 entries) when n_groups <= kGroupsInline
+    // first block of each group (n_groups + 1 entries) and tiles per block
     uint32_t prefix[kGroupsInline + 1];
+    uint32_t tps[kGroupsInline];
 };
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -125,7 +135,7 @@ __device__ __forceinline__ void finish(const GroupsDev &a, const float2 *__restr
         const float2 *__restrict__ g = a.taps + ch.taps_off;
         acc = window_rollover(a, hist, [&](uint32_t t) { return g[t]; }, anchor, ch.cpp, (uint32_t)k, ch.wrap);
     }
-    reinterpret_cast<float2 *>(ch.out)[j] = cmul(nco_phasor((int64_t)k, ch.nu), acc);
+    store_out(ch.out, j, cmul(nco_phasor((int64_t)k, ch.nu), acc));
 }
 
 // Stage the span x[lo .. lo+len) into LDS transposed by d: sample lo+i goes
@@ -214,32 +224,119 @@ __device__ __forceinline__ void gather_windows(const GroupsDev &a, const float2 
 // ---------------------------------------------------------------------------
 // MFMA groups
 // ---------------------------------------------------------------------------
-template <int kBlock, uint32_t kS>
-__device__ __forceinline__ void mfma_group(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P,
-                                           const uint32_t *gw, uint32_t j0, uint32_t nt, int64_t a0,
-                                           unsigned char *smem)
+// A tile's input passes through registers (kStage samples per thread) on
+// its way to LDS, so the loads of tile i+1 are in flight while tile i runs on
+// the matrix cores. Dense tiles stage the span transposed by down; sparse
+// tiles gather the windows tap-major (thread tid always carries tap
+// tid mod w, rows tid/w + s*(kBlock/w)).
+// Raw buffer loads: 32-bit offsets from a base held in SGPRs, and an offset
+// at or past `bytes` reads as zero instead of touching memory.
+constexpr int kBufferFlags = 0x00020000;  // gfx9 descriptor word 3: 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const float2 *p, int64_t count)
 {
-    constexpr uint32_t kCpp = 4 * kS;  // padded taps per window (zero taps in front)
-    const uint32_t down = gw[2], m = gw[4], rows = gw[5];
-    ChanStatic *sc = reinterpret_cast<ChanStatic *>(smem);  // the group's channels
-    float2 *xs = reinterpret_cast<float2 *>(smem + kMfmaMaxChan * sizeof(ChanStatic));
-    if (threadIdx.x < m)
-        sc[threadIdx.x] = a.chans[gw[8 + threadIdx.x]];
+    const int64_t bytes = count * (int64_t)sizeof(float2);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(p), 0, (int)min(bytes, (int64_t)0x7ffffff8),
+                                             kBufferFlags);
+}
+__device__ __forceinline__ float2 buffer_sample(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return make_float2(__int_as_float(v[0]), __int_as_float(v[1]));
+}
 
-    const bool dense = down < kCpp;
-    uint32_t mc = 0, pitch = 0;
+// Loads of one MFMA tile (anchor a0 of row 0). Element s of a thread is span
+// sample s*kBlock + tid (dense) or window tap tid mod w of row
+// s*(kBlock/w) + tid/w (sparse). Elements past the tile read what lies there
+// (zero past the chunk) and are never used. Only element 0 can precede the
+// chunk (kBlock > w - 1, and sparse rows are down >= w apart), so a tile that
+// reaches into the history takes element 0 from there.
+template <int kBlock, uint32_t kW, uint32_t kStage = mfma_stage(kBlock)>
+__device__ __forceinline__ void mfma_load(const GroupsDev &a, const float2 *__restrict__ hist, bool dense,
+                                          int64_t a0, uint32_t down, float2 (&v)[kStage])
+{
+    constexpr uint32_t kRowsPer = kBlock / kW;
+    const int64_t lo = a0 - (int64_t)(kW - 1);
+    // base may lie before the chunk: offsets below -lo are never read through it
+    const __amdgpu_buffer_rsrc_t r = buffer_of(a.in + lo, (int64_t)a.n - lo);
+    uint32_t e0 = dense ? threadIdx.x : threadIdx.x % kW + (threadIdx.x / kW) * down;
+    // Opaque per call: keeps the per-element offsets from being hoisted out
+    // of the tile loop into registers that would stay live across it.
+    asm volatile("" : "+v"(e0));
+    const uint32_t stride = dense ? kBlock * 8u : kRowsPer * down * 8u;
+    if (lo >= 0)
+        v[0] = buffer_sample(r, e0 * 8u, 0);
+    else
+        v[0] = sample(a, hist, lo + (int64_t)e0);
+    // the whole offset goes in the VGPR operand, which the range check covers
+#pragma unroll
+    for (uint32_t s = 1; s < kStage; ++s)
+        v[s] = buffer_sample(r, e0 * 8u + s * stride, 0);
+}
+
+// LDS layout of a staged MFMA tile. Dense: span sample i at
+// (i mod down)*mc + i/down with mc = ceil(capacity/down) rounded up to odd,
+// so the 16 rows of a sub-tile (same tap, consecutive rows) read 16
+// consecutive slots and the residue rows start on different banks. Sparse:
+// window r at r*(w+1), tap t at +t (rows w+1 apart fall on different banks).
+// The whole staging capacity is written.
+__host__ __device__ constexpr uint32_t mfma_dense_mc(uint32_t capacity, uint32_t down)
+{
+    return ((capacity + down - 1) / down) | 1u;
+}
+
+template <int kBlock, uint32_t kW, uint32_t kStage = mfma_stage(kBlock)>
+__device__ __forceinline__ void mfma_commit(bool dense, uint32_t down, uint32_t mc, const float2 (&v)[kStage],
+                                            float2 *xs)
+{
+    constexpr uint32_t kRowsPer = kBlock / kW;
     if (dense) {
-        const uint32_t len = (nt - 1) * down + kCpp;
-        mc = ((len + down - 1) / down) | 1u;
-        stage_transposed<kBlock>(a, hist, a0 - (int64_t)(kCpp - 1), len, down, mc, xs);
+        uint32_t i0 = threadIdx.x;
+        asm volatile("" : "+v"(i0));  // per tile: keeps the slots out of long-lived registers
+        const uint32_t bq = kBlock / down, br = kBlock - bq * down;
+        uint32_t q = i0 / down, rr = i0 - q * down;
+#pragma unroll
+        for (uint32_t s = 0; s < kStage; ++s) {
+            xs[rr * mc + q] = v[s];
+            q += bq;
+            rr += br;
+            if (rr >= down) {
+                rr -= down;
+                ++q;
+            }
+        }
     } else {
-        pitch = rows + 1;
-        gather_windows<kBlock, 8>(a, hist, a0, down, kCpp, nt, 0, kCpp, pitch, xs);
+        float2 *p = xs + (threadIdx.x / kW) * (kW + 1) + threadIdx.x % kW;
+#pragma unroll
+        for (uint32_t s = 0; s < kStage; ++s)
+            p[s * kRowsPer * (kW + 1)] = v[s];
+    }
+}
+
+// Tiles tb .. tb_end-1 of an MFMA group (tile i: rows i*rows .. of the chunk's
+// `count` outputs, first output n0 of the stream).
+template <int kBlock, uint32_t kS, bool kDense>
+__device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P,
+                                            const uint32_t *gw, uint64_t n0, uint32_t count, uint32_t tb,
+                                            uint32_t tb_end, unsigned char *smem)
+{
+    constexpr uint32_t kW = 4 * kS;  // padded taps per window (zero taps in front)
+    constexpr uint32_t kWaves = kBlock / 64;
+    const uint32_t down = gw[2], m = gw[4], rows = gw[5];
+    // LDS: the group's modulated taps [channel][w] (zero-padded in front, as
+    // the windows are), then the staged tile
+    float2 *taps_l = reinterpret_cast<float2 *>(smem);
+    float2 *xs = taps_l + kMfmaMaxChan * kW;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = uniform(threadIdx.x >> 6);
+    const uint32_t mc = kDense ? mfma_dense_mc(mfma_stage(kBlock) * kBlock, down) : 0;
+    for (uint32_t e = threadIdx.x; e < m * kW; e += kBlock) {
+        const ChanStatic &c = a.chans[gw[8 + e / kW]];
+        const uint32_t t = e % kW, pad = kW - c.cpp;
+        taps_l[e] = t < pad ? make_float2(0.f, 0.f) : a.taps[c.taps_off + t - pad];
     }
 
     // B operands: step s holds B[k = 4s + lane/16][j = lane%16]; steps 0..kS-1
     // cover the real parts of the window, kS..2kS-1 the imaginary parts
-    const uint32_t lane = threadIdx.x & 63;
     float bv[2 * kS];
     {
         const float *__restrict__ bt = a.btab + gw[6];
@@ -247,16 +344,17 @@ __device__ __forceinline__ void mfma_group(const GroupsDev &a, const float2 *__r
         for (uint32_t s = 0; s < 2 * kS; ++s)
             bv[s] = bt[s * 64 + lane];
     }
-    // LDS offset of window tap t = 4s + lane/16 of row 0
-    uint32_t off[kS];
+    // A operand of step s: row r = st*16 + lane%16, window tap 4s + lane/16,
+    // read at xa[s] + 16*st
+    const float2 *xa[kS];
     {
-        const uint32_t t0 = lane >> 4;
-        if (dense) {
+        const uint32_t t0 = lane >> 4, r0 = lane & 15;
+        if constexpr (kDense) {
             uint32_t q = t0 / down, rr = t0 - q * down;
             const uint32_t sq = 4 / down, sr = 4 - sq * down;
 #pragma unroll
             for (uint32_t s = 0; s < kS; ++s) {
-                off[s] = rr * mc + q;
+                xa[s] = xs + rr * mc + q + r0;
                 q += sq;
                 rr += sr;
                 if (rr >= down) {
@@ -267,45 +365,114 @@ __device__ __forceinline__ void mfma_group(const GroupsDev &a, const float2 *__r
         } else {
 #pragma unroll
             for (uint32_t s = 0; s < kS; ++s)
-                off[s] = (4 * s + t0) * pitch;
+                xa[s] = xs + r0 * (kW + 1) + t0 + 4 * s;
         }
     }
-    __syncthreads();
 
-    constexpr uint32_t kWaves = kBlock / 64;
-    const uint32_t wave = uniform(threadIdx.x >> 6);
-    const uint32_t subtiles = (nt + 15) >> 4;
-    const uint32_t col = lane & 15, cprime = col >> 1;
-    const bool even = (col & 1) == 0;
-    for (uint32_t st = wave; st < subtiles; st += kWaves) {
-        const uint32_t rbase = st * 16 + (lane & 15);
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (uint32_t s = 0; s < kS; ++s) {
-            const float2 x = xs[off[s] + rbase];
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, bv[s], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, bv[kS + s], acc, 0, 0, 0);
+    // D[i][j] sits in lane j + 16*(i/4), element i%4: lanes 2c and 2c+1 hold
+    // channel c's real and imaginary parts of rows 4*(lane/16) .. +3. The even
+    // lane finishes the first two of those rows (e = 0, 1), the odd lane the
+    // last two.
+    const uint32_t col = lane & 15;
+    const bool odd = col & 1, live = (col >> 1) < m;
+    const uint32_t cl = min(col >> 1, m - 1);
+    const ChanStatic *ch = a.chans + gw[8 + cl];
+    const uint64_t R = ch->r;
+    const uint32_t cpp1 = ch->cpp - 1;
+    const double nu = ch->nu, inv_r = ch->inv_r;
+    const float2 wrap = ch->wrap;
+    const float2 unwrap = make_float2(wrap.x, -wrap.y);
+    gvec2_ptr out = (gvec2_ptr)ch->out;
+    const float2 *my_taps = taps_l + cl * kW;
+    const uint32_t ro = 4 * (lane >> 4) + (odd ? 2 : 0);
+    // NCO bookkeeping per output row: u = k(row 16*wave of the tile) + (row -
+    // 16*wave)*down, unwrapped (a tile spans fewer than R inputs, so at most
+    // one rollover: u < 2R); the phasor runs on as e^{j2pi nu u} and picks up
+    // e^{-j2pi nu R} past the rollover.
+    const uint32_t du = 16 * kWaves * down;  // from one of the wave's sub-tiles to the next
+    const float2 step = nco_phasor((int64_t)du, nu);
+
+    float2 v[mfma_stage(kBlock)];
+    if (tb < tb_end && tb * rows < count)
+        mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + tb * rows) * down - P), down, v);
+    for (; tb < tb_end; ++tb) {
+        const uint32_t j0 = tb * rows;
+        if (j0 >= count)
+            break;  // uniform
+        const uint32_t nt = min(rows, count - j0);
+        const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // chunk-local anchor of row 0
+        __syncthreads();  // the previous tile's reads of LDS are done
+        mfma_commit<kBlock, kW>(kDense, down, mc, v, xs);
+        __syncthreads();
+        {
+            const uint32_t jn = j0 + rows;
+            if (tb + 1 < tb_end && jn < count)
+                mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + jn) * down - P), down, v);
         }
-        // D[i][j] sits in lane j + 16*(i/4), element i%4. Lanes 2c and 2c+1
-        // hold channel c's real and imaginary parts of the same rows: swap
-        // through DPP (quad_perm 1,0,3,2); the even lane finishes rows 0, 1
-        // of its quad of rows, the odd lane rows 2, 3.
-        float oth[4];
+        uint64_t u[2];
+        float2 ph[2];
+        {
+            const uint64_t k0 = mod_u64(P + (uint64_t)a0 + (uint64_t)(wave * 16) * down, (uint32_t)R, inv_r);
 #pragma unroll
-        for (uint32_t v = 0; v < 4; ++v)
-            oth[v] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[v]), 0xB1, 0xF, 0xF, false));
-        if (cprime < m) {
-            const ChanStatic &ch = sc[cprime];
+            for (uint32_t e = 0; e < 2; ++e) {
+                u[e] = k0 + (uint64_t)(ro + e) * down;
+                ph[e] = nco_phasor((int64_t)u[e], nu);
+            }
+        }
+        const uint32_t subtiles = (nt + 15) >> 4;
+        for (uint32_t st = wave; st < subtiles; st += kWaves) {
+            float2 x[kS];
 #pragma unroll
-            for (uint32_t vv = 0; vv < 2; ++vv) {
-                const uint32_t v = even ? vv : 2 + vv;
-                const float mine = even ? (vv ? acc[1] : acc[0]) : (vv ? acc[3] : acc[2]);
-                const float other = even ? (vv ? oth[1] : oth[0]) : (vv ? oth[3] : oth[2]);
-                const uint32_t row = st * 16 + 4 * (lane >> 4) + v;
-                if (row < nt) {
-                    const float2 y = even ? make_float2(mine, other) : make_float2(other, mine);
-                    finish(a, hist, P, ch, a0 + (int64_t)row * down, j0 + row, y);
+            for (uint32_t s = 0; s < kS; ++s)
+                x[s] = xa[s][st * 16 * (kDense ? 1 : kW + 1)];
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (uint32_t s = 0; s < kS; ++s) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].x, bv[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].y, bv[kS + s], acc, 0, 0, 0);
+            }
+            // all of the sub-tile's LDS reads first, then its MFMA chain: one
+            // LDS latency per sub-tile instead of one per step
+            __builtin_amdgcn_sched_group_barrier(0x100, kS, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 2 * kS, 0);
+            // partner lane's values (quad_perm 1,0,3,2 swaps lanes 2c and 2c+1)
+            float oth[4];
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e)
+                oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
+#pragma unroll
+            for (uint32_t e = 0; e < 2; ++e) {
+                const uint32_t row = st * 16 + ro + e;
+                if (live && row < nt) {
+                    float2 y = odd ? make_float2(oth[2 + e], acc[2 + e]) : make_float2(acc[e], oth[e]);
+                    float2 p = ph[e];
+                    uint64_t k = u[e];
+                    if (k >= R) {
+                        k -= R;
+                        p = cmul(p, unwrap);
+                    }
+                    if (k < cpp1) {
+                        // a rollover inside the window: window taps before it
+                        // (t < w-1-k) carry e^{j2pi nu R}; add (e^{j2pi nu R} - 1)
+                        // times their partial sum, from the staged tile in LDS
+                        const uint32_t split = kW - 1 - (uint32_t)k;
+                        float2 part = make_float2(0.f, 0.f);
+                        uint32_t q = 0, rr = 0;
+                        for (uint32_t t = 0; t < split; ++t) {
+                            const float2 xv = kDense ? xs[rr * mc + q + row] : xs[row * (kW + 1) + t];
+                            cmac(part, my_taps[t], xv);
+                            if (++rr == down) {
+                                rr = 0;
+                                ++q;
+                            }
+                        }
+                        cmac(y, make_float2(wrap.x - 1.f, wrap.y), part);
+                    }
+                    const float2 o = cmul(p, y);
+                    out[j0 + row] = fvec2{o.x, o.y};
                 }
+                u[e] += du;
+                ph[e] = cmul(ph[e], step);
             }
         }
     }
@@ -436,16 +603,28 @@ __device__ __forceinline__ void generic_group(const GroupsDev &a, const float2 *
         const uint64_t k = mod_u64(P + (uint64_t)anchor, ch.r, ch.inv_r);
         if (k < cpp - 1)
             acc = window_rollover(a, hist, [&](uint32_t t) { return gg[t]; }, anchor, cpp, (uint32_t)k, ch.wrap);
-        reinterpret_cast<float2 *>(ch.out)[j0 + q] = cmul(nco_phasor((int64_t)k, ch.nu), acc);
+        store_out(ch.out, j0 + q, cmul(nco_phasor((int64_t)k, ch.nu), acc));
     }
 }
 
-template <int kBlock>
-__global__ __launch_bounds__(kBlock) void chan_groups_kernel(GroupsDev a)
+// kWide: the plan holds 64-tap MFMA groups (their B operands and window
+// reads take 32 more registers, so plans without them get a leaner build
+// that fits 128 registers: four waves per SIMD)
+template <int kBlock, bool kWide>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWide ? 1 : 4))) void chan_groups_kernel(
+    GroupsDev a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint64_t P = a.pos ? a.pos->p : a.p;
-    const uint32_t par = a.pos ? a.pos->parity : a.parity;
+    // stream position and history parity: from the arguments, or (graph
+    // replays) from device memory; wavefront-uniform either way
+    uint64_t P = a.p;
+    uint32_t par = a.parity;
+    if (a.pos) {
+        P = a.pos->p;
+        par = a.pos->parity;
+    }
+    P = ((uint64_t)uniform((uint32_t)(P >> 32)) << 32) | uniform((uint32_t)P);
+    par = uniform(par);
     const float2 *hist = par ? a.hist1 : a.hist0;
     const uint32_t H = a.hist_len;
     if (blockIdx.x == 0) {
@@ -465,33 +644,50 @@ __global__ __launch_bounds__(kBlock) void chan_groups_kernel(GroupsDev a)
         else
             hi = mid;
     }
-    const uint32_t g = lo, tb = blockIdx.x - a.prefix[lo];
+    // this block's stripe: tiles tb .. tb + tps - 1 of group g
+    const uint32_t g = lo;
+    uint32_t tb = (blockIdx.x - a.prefix[lo]) * a.tps[g];
+    const uint32_t tb_end = tb + a.tps[g];
     const uint32_t *gw = a.groups + g * kGroupWords;
     const uint32_t kind = gw[0], up = gw[1], down = gw[2], rows = gw[5];
-    const ChanStatic &c0 = a.chans[gw[8]];
-    const uint64_t n0 = cdiv(P * up, down, c0.inv_down);
-    const uint32_t count = (uint32_t)(cdiv((P + a.n) * up, down, c0.inv_down) - n0);
-    const uint32_t j0 = tb * rows;
-    if (j0 >= count)
-        return;  // uniform across the block
-    const uint32_t nt = min(rows, count - j0);
-    const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // up == 1: anchor of row 0
-    switch (kind) {
-    case kGroupMfma:
-        if (gw[3] == 32)
-            mfma_group<kBlock, 8>(a, hist, P, gw, j0, nt, a0, smem);
+    const double inv_down = a.chans[gw[8]].inv_down;
+    const uint64_t n0 = cdiv(P * up, down, inv_down);
+    const uint32_t count = (uint32_t)(cdiv((P + a.n) * up, down, inv_down) - n0);
+    if (kind == kGroupMfma) {
+        if constexpr (kWide) {
+            if (gw[3] != 32) {
+                if (down < 64)
+                    mfma_stripe<kBlock, 16, true>(a, hist, P, gw, n0, count, tb, tb_end, smem);
+                else
+                    mfma_stripe<kBlock, 16, false>(a, hist, P, gw, n0, count, tb, tb_end, smem);
+                return;
+            }
+        }
+        if (down < 32)
+            mfma_stripe<kBlock, 8, true>(a, hist, P, gw, n0, count, tb, tb_end, smem);
         else
-            mfma_group<kBlock, 16>(a, hist, P, gw, j0, nt, a0, smem);
-        break;
-    case kGroupVDense:
-        vdense_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
-        break;
-    case kGroupVSparse:
-        vsparse_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
-        break;
-    default:
-        generic_group<kBlock>(a, hist, P, gw, j0, nt, n0, smem);
-        break;
+            mfma_stripe<kBlock, 8, false>(a, hist, P, gw, n0, count, tb, tb_end, smem);
+        return;
+    }
+    for (const uint32_t first = tb; tb < tb_end; ++tb) {
+        const uint32_t j0 = tb * rows;
+        if (j0 >= count)
+            return;  // uniform across the block
+        const uint32_t nt = min(rows, count - j0);
+        const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // up == 1: anchor of row 0
+        if (tb != first)
+            __syncthreads();  // the previous tile's reads of LDS are done
+        switch (kind) {
+        case kGroupVDense:
+            vdense_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
+            break;
+        case kGroupVSparse:
+            vsparse_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
+            break;
+        default:
+            generic_group<kBlock>(a, hist, P, gw, j0, nt, n0, smem);
+            break;
+        }
     }
 }
 
@@ -529,16 +725,30 @@ hipError_t launch_chan_groups(const GroupsArgs &f, hipStream_t s)
     d.n_groups = f.n_groups;
     for (uint32_t k = 0; k <= f.n_groups; ++k)
         d.prefix[k] = f.prefix[k];
+    for (uint32_t k = 0; k < f.n_groups; ++k)
+        d.tps[k] = f.tps[k];
     const uint32_t blocks = f.prefix[f.n_groups];
-    switch (f.block) {
-    case 128:
-        hipLaunchKernelGGL(chan_groups_kernel<128>, dim3(blocks), dim3(128), f.lds_bytes, s, d);
+    auto go = [&](auto kernel, uint32_t threads) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), f.lds_bytes, s, d);
+    };
+    switch (f.block * 2 + (f.wide ? 1 : 0)) {
+    case 256:
+        go(chan_groups_kernel<128, false>, 128);
         break;
-    case 512:
-        hipLaunchKernelGGL(chan_groups_kernel<512>, dim3(blocks), dim3(512), f.lds_bytes, s, d);
+    case 257:
+        go(chan_groups_kernel<128, true>, 128);
+        break;
+    case 1024:
+        go(chan_groups_kernel<512, false>, 512);
+        break;
+    case 1025:
+        go(chan_groups_kernel<512, true>, 512);
+        break;
+    case 513:
+        go(chan_groups_kernel<256, true>, 256);
         break;
     default:
-        hipLaunchKernelGGL(chan_groups_kernel<256>, dim3(blocks), dim3(256), f.lds_bytes, s, d);
+        go(chan_groups_kernel<256, false>, 256);
         break;
     }
     return hipGetLastError();

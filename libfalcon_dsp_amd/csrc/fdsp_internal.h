@@ -125,12 +125,16 @@ enum : uint32_t {
 constexpr uint32_t kGroupWords = 16;
 constexpr uint32_t kMfmaMaxChan = 8;
 constexpr uint32_t kMfmaMinChan = 3;
+// largest down of an MFMA group (keeps a tile's buffer offsets below 2^30)
+constexpr uint32_t kMfmaMaxDown = 1u << 20;
 // groups whose first-block prefix travels in the kernel arguments
 constexpr uint32_t kGroupsInline = 256;
 // taps per LDS pass of the one-channel sparse path
 constexpr uint32_t kVSparsePass = 8;
 // outputs per lane of the one-channel dense path
 constexpr uint32_t kVDenseOpl = 2;
+// input samples per thread staged for one MFMA tile (4096 per block at most)
+constexpr uint32_t mfma_stage(uint32_t block) { return block >= 512 ? 8 : 16; }
 
 struct GroupsArgs {
     const float2 *in;
@@ -145,10 +149,12 @@ struct GroupsArgs {
     uint32_t n_groups;
     uint32_t block;             // threads per block: 128, 256 or 512
     uint32_t lds_bytes;         // dynamic LDS per block
+    bool wide;                  // the plan holds 64-tap MFMA groups
     const ChanPos *pos;         // device position (graph mode), or null:
     uint64_t p;                 //   then the position and parity come from here
     uint32_t parity;
     const uint32_t *prefix;     // host: first block of each group, n_groups + 1 entries
+    const uint32_t *tps;        // host: tiles per block of each group
 };
 hipError_t launch_chan_groups(const GroupsArgs &a, hipStream_t s);
 // advance the device position by n inputs and flip the history parity

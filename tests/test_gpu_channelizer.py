@@ -16,6 +16,9 @@ from tests.helpers import assert_close_rms, chunks, rms
 
 pytestmark = pytest.mark.gpu
 
+# agreement between work splits of the batched kernel, relative to input RMS
+TILING_TOL = 2e-6
+
 
 def _chans_small(fs=20_000_000):
     return [
@@ -106,8 +109,11 @@ def test_grouped_tilings_agree(seg, block):
     base = fd.MultiRateChannelizer(fs, chans).apply(x)
     g = fd.MultiRateChannelizer(fs, chans)
     g.set_tiling(seg, block)
-    for a, b in zip(base, g.apply(x)):
-        np.testing.assert_array_equal(a, b)
+    # the work split only changes where the NCO phasor recurrence restarts
+    # (a few float32 rounding steps), so tilings agree to ~1e-7
+    for c, (a, b) in enumerate(zip(base, g.apply(x))):
+        assert a.size == b.size
+        assert_close_rms(b, a, rms(x), tol=TILING_TOL, what=f"chan {c}")
 
 
 @pytest.mark.parametrize("chans_of", [_chans_small, _chans_grouped])
@@ -200,5 +206,8 @@ def test_tilings_agree(seg, block):
     base = fd.MultiRateChannelizer(fs, chans).apply(x)
     g = fd.MultiRateChannelizer(fs, chans)
     g.set_tiling(seg, block)
-    for a, b in zip(base, g.apply(x)):
-        np.testing.assert_array_equal(a, b)
+    # the work split only changes where the NCO phasor recurrence restarts
+    # (a few float32 rounding steps), so tilings agree to ~1e-7
+    for c, (a, b) in enumerate(zip(base, g.apply(x))):
+        assert a.size == b.size
+        assert_close_rms(b, a, rms(x), tol=TILING_TOL, what=f"chan {c}")
