@@ -92,6 +92,22 @@ def cpu_baseline(fs, chans, x, n_single, n_par):
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "chan_groups_kernel_chan32_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes, each scaled by the
+    same counter's reading for a device copy of known size; (None, None)
+    when no such measurement is committed."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return int(t["traffic_bytes_per_dispatch"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,9 +117,24 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-channel plain GPU form")
     ap.add_argument("--latency-chunk", type=int, default=1 << 18)
     ap.add_argument("--latency-chunks", type=int, default=200)
-    ap.add_argument("--seg", type=int, default=0, help="fused-kernel segment length (0 = default/autotuned)")
+    ap.add_argument("--seg", type=int, default=0, help="input samples per block of the fused kernel (0 = default)")
     ap.add_argument("--block", type=int, default=0, help="fused-kernel block threads (0 = default)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device-to-device copies of 2^24 complex samples (128 MiB), the known-size "
+                         "copy that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import libfalcon_dsp_amd as fd
+        n = 1 << 24
+        a, b = fd.DeviceBuffer(n), fd.DeviceBuffer(n)
+        a.zero()
+        st = fd.Stream()
+        for _ in range(3):
+            fd.lib().fdsp_memcpy_d2d(b.ptr, a.ptr, n * 8, st.h)
+        st.sync()
+        print(json.dumps({"copy_bytes_per_dispatch": n * 8, "dispatches": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the channelizer shards by "
@@ -155,21 +186,23 @@ def main():
 
     # kernel time of the fused kernel: the eager form launches exactly that one
     # kernel per call, so events on its stream bracket it
-    kms = []
+    # (back-to-back launches between one pair of events on the kernel's
+    # stream: the host queues the next launch while the current one runs, so
+    # the average is kernel time, not host latency)
     out_per = []
     k0, k1 = fd.Event(), fd.Event()
     g.apply_device(d_in, n, outs, stream)  # first call uploads the output table
     stream.sync()
+    k0.record(stream)
     for _ in range(args.steps):
-        k0.record(stream)
         out_per.append(sum(g.apply_device(d_in, n, outs, stream)))
-        k1.record(stream)
-        k1.sync()
-        kms.append(k0.elapsed_ms(k1))
-    k_avg = statistics.mean(kms)
+    k1.record(stream)
+    k1.sync()
+    k_avg = k0.elapsed_ms(k1) / args.steps
     alg_bytes = 8 * n + 8 * statistics.mean(out_per)
     achieved = alg_bytes / (k_avg / 1e3) / 1e9
     log(f"fused kernel: avg {k_avg * 1e3:.1f} us, algorithmic {alg_bytes / 1e6:.1f} MB -> {achieved:.0f} GB/s")
+    traffic, traffic_src = measured_traffic()
 
     # p50/p99 chunk latency: streaming chunks of --latency-chunk through a graph
     lc = args.latency_chunk
@@ -219,7 +252,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference channelizer on the host cores)")
-        cpu = cpu_baseline(w.fs, chans, x, 1 << 21, 1 << 22)
+        cpu = cpu_baseline(w.fs, chans, x, 1 << 22, n)
         log(f"CPU baseline: {cpu['value']} Msps on {cpu['cores']} threads, "
             f"{cpu['single_thread_value']} Msps single thread ({cpu['kind']})")
 
@@ -245,8 +278,9 @@ def main():
         "p99_chunk_latency_ms": round(p99, 5),
         "latency_chunk_samples": lc,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "chan_fused_kernel", "kernel_avg_us": round(k_avg * 1e3, 2),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": "chan_groups_kernel", "kernel_avg_us": round(k_avg * 1e3, 2),
                      "algorithmic_bytes": int(alg_bytes)},
         "plain_gpu": plain,
         "batched_vs_plain": round(value / plain["value"], 1) if plain else None,

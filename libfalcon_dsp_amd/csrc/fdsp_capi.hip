@@ -998,7 +998,10 @@ struct fdsp_channelizer_s {
     std::vector<float> btab;
     DevBuf d_groups, d_btab;
     uint32_t n_groups = 0, lds_bytes = 0;
-    std::vector<uint32_t> prefix, tps;
+    std::vector<uint32_t> tps;
+    // block tables by chunk length (eager calls), and the graph's own
+    std::vector<std::pair<uint32_t, DevBuf>> tables;
+    DevBuf d_table_graph;
     // host-pointer path
     DevBuf d_in;
     std::vector<DevBuf> d_outs;
@@ -1193,14 +1196,21 @@ void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
     h->n_groups = (uint32_t)(h->groups.size() / kGroupWords);
 }
 
-// Blocks of a chunk of n inputs: each block runs a stripe of consecutive
-// tiles of one group covering about seg_len inputs; a chunk emits at most
-// floor(n*up/down) + 1 outputs per channel.
-void group_prefix(fdsp_channelizer_s *h, uint32_t n)
+// Work split of a chunk of n inputs (a chunk emits at most floor(n*up/down)+1
+// outputs per channel). Each block runs a stripe of tps consecutive tiles of
+// one group covering about seg_len inputs. Stripes are dealt to the kXcds
+// queues by the input segment (seg_len inputs) they start in, so all groups'
+// stripes over the same input go to one XCD and share its L2; the queues are
+// then interleaved to follow the round-robin dispatch of workgroups to XCDs.
+// table: (group, stripe) per block, kNoGroup where a queue is short.
+void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table)
 {
-    h->prefix.assign(1, 0);
     h->tps.assign(h->n_groups, 1);
-    uint64_t b = 0;
+    struct Item {
+        uint64_t start;
+        uint32_t g, s;
+    };
+    std::vector<Item> items;
     for (uint32_t g = 0; g < h->n_groups; ++g) {
         const uint32_t *gw = h->groups.data() + g * kGroupWords;
         const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
@@ -1208,12 +1218,65 @@ void group_prefix(fdsp_channelizer_s *h, uint32_t n)
         const uint64_t span = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
         const uint64_t tps = std::min<uint64_t>(std::max<uint64_t>(1, h->seg_len / span), tiles);
         h->tps[g] = (uint32_t)tps;
-        b += (tiles + tps - 1) / tps;
-        h->prefix.push_back((uint32_t)b);
+        for (uint64_t st = 0; st * tps < tiles; ++st)
+            items.push_back({st * tps * span, g, (uint32_t)st});
     }
+    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.start < y.start; });
+    std::vector<std::vector<uint32_t>> q(kXcds);
+    for (const Item &it : items) {
+        auto &qq = q[(it.start / h->seg_len) % kXcds];
+        qq.push_back(it.g);
+        qq.push_back(it.s);
+    }
+    size_t most = 0;
+    for (auto &qq : q)
+        most = std::max(most, qq.size() / 2);
+    table->assign(2 * most * kXcds, 0);
+    for (size_t j = 0; j < most; ++j)
+        for (uint32_t x = 0; x < kXcds; ++x) {
+            uint32_t *e = table->data() + 2 * (j * kXcds + x);
+            if (j < q[x].size() / 2) {
+                e[0] = q[x][2 * j];
+                e[1] = q[x][2 * j + 1];
+            } else {
+                e[0] = kNoGroup;
+            }
+        }
 }
 
-GroupsArgs groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool device_pos)
+// device copy of the block table for chunk length n (cached per length)
+int table_for(fdsp_channelizer_s *h, uint32_t n, bool for_graph, const uint32_t **dev, uint32_t *blocks)
+{
+    std::vector<uint32_t> t;
+    block_table(h, n, &t);
+    *blocks = (uint32_t)(t.size() / 2);
+    DevBuf *buf = nullptr;
+    if (for_graph) {
+        buf = &h->d_table_graph;
+    } else {
+        for (auto &e : h->tables)
+            if (e.first == n) {
+                *dev = e.second.as<uint32_t>();
+                return FDSP_OK;
+            }
+        if (h->tables.size() >= 8) {
+            // launches in flight may still read the cached tables
+            HIP_TRY(hipDeviceSynchronize());
+            for (auto &e : h->tables)
+                e.second.release();
+            h->tables.clear();
+        }
+        h->tables.emplace_back(n, DevBuf{});
+        buf = &h->tables.back().second;
+    }
+    if (int rc = buf->ensure(t.size() * sizeof(uint32_t)))
+        return rc;
+    HIP_TRY(hipMemcpy(buf->p, t.data(), t.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    *dev = buf->as<uint32_t>();
+    return FDSP_OK;
+}
+
+int groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool device_pos, GroupsArgs *out)
 {
     GroupsArgs f{};
     f.in = reinterpret_cast<const float2 *>(d_in);
@@ -1236,10 +1299,11 @@ GroupsArgs groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n,
     f.pos = device_pos ? h->d_pos.as<ChanPos>() : nullptr;
     f.p = h->p;
     f.parity = h->parity;
-    group_prefix(h, n);
-    f.prefix = h->prefix.data();
+    if (int rc = table_for(h, n, device_pos, &f.table, &f.blocks))
+        return rc;
     f.tps = h->tps.data();
-    return f;
+    *out = f;
+    return FDSP_OK;
 }
 
 }  // namespace
@@ -1424,6 +1488,9 @@ extern "C" int fdsp_channelizer_destroy(fdsp_channelizer_t h)
     h->d_pos.release();
     h->d_groups.release();
     h->d_btab.release();
+    h->d_table_graph.release();
+    for (auto &e : h->tables)
+        e.second.release();
     h->d_in.release();
     for (auto &b : h->d_outs)
         b.release();
@@ -1577,7 +1644,10 @@ extern "C" int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf
             return rc;
         }
         // eager form: position and parity travel as kernel arguments, one launch
-        HIP_TRY(launch_chan_groups(groups_args(h, d_in, (uint32_t)n, false), s));
+        GroupsArgs f;
+        if (int rc = groups_args(h, d_in, (uint32_t)n, false, &f))
+            return rc;
+        HIP_TRY(launch_chan_groups(f, s));
         HIP_TRY(hipEventRecord(h->use_ev, s));
         h->parity ^= 1;
         h->pos_dirty = true;
@@ -1684,7 +1754,9 @@ extern "C" int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_
         return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(h->d_stat_graph.p, gstat.data(), sizeof(ChanStatic) * C, hipMemcpyHostToDevice));
-    GroupsArgs f = groups_args(h, d_in, (uint32_t)chunk_len, true);
+    GroupsArgs f;
+    if (int rc = groups_args(h, d_in, (uint32_t)chunk_len, true, &f))
+        return rc;
     f.chans = h->d_stat_graph.as<ChanStatic>();
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = launch_chan_groups(f, h->stream);
@@ -1755,6 +1827,9 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
     HIP_TRY(hipMemcpy(h->d_groups.p, groups.data(), groups.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     h->groups.swap(groups);
     h->lds_bytes = lds;
+    for (auto &e : h->tables)
+        e.second.release();
+    h->tables.clear();
     h->seg_len = segment_len;
     h->block = block_threads;
     return FDSP_OK;

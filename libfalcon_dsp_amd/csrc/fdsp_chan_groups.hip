@@ -66,9 +66,8 @@ struct GroupsDev {
     uint32_t parity;
     const uint32_t *groups;
     uint32_t n_groups;
-    // first block of each group (n_groups + 1 entries) and tiles per block
-    uint32_t prefix[kGroupsInline + 1];
-    uint32_t tps[kGroupsInline];
+    const uint32_t *table;  // (group, stripe) of each block
+    uint32_t tps[kGroupsInline];  // tiles per stripe of each group
 };
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -635,18 +634,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWide ? 
             hw[t] = i >= 0 ? a.in[i] : hist[H + i];
         }
     }
-    // which group, which row tile
-    uint32_t lo = 0, hi = a.n_groups;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.prefix[mid] <= blockIdx.x)
-            lo = mid;
-        else
-            hi = mid;
-    }
     // this block's stripe: tiles tb .. tb + tps - 1 of group g
-    const uint32_t g = lo;
-    uint32_t tb = (blockIdx.x - a.prefix[lo]) * a.tps[g];
+    const uint32_t g = uniform(a.table[2 * blockIdx.x]);
+    if (g == kNoGroup)
+        return;  // padding of an XCD's queue
+    uint32_t tb = uniform(a.table[2 * blockIdx.x + 1]) * a.tps[g];
     const uint32_t tb_end = tb + a.tps[g];
     const uint32_t *gw = a.groups + g * kGroupWords;
     const uint32_t kind = gw[0], up = gw[1], down = gw[2], rows = gw[5];
@@ -723,11 +715,10 @@ hipError_t launch_chan_groups(const GroupsArgs &f, hipStream_t s)
     d.parity = f.parity;
     d.groups = f.groups;
     d.n_groups = f.n_groups;
-    for (uint32_t k = 0; k <= f.n_groups; ++k)
-        d.prefix[k] = f.prefix[k];
+    d.table = f.table;
     for (uint32_t k = 0; k < f.n_groups; ++k)
         d.tps[k] = f.tps[k];
-    const uint32_t blocks = f.prefix[f.n_groups];
+    const uint32_t blocks = f.blocks;
     auto go = [&](auto kernel, uint32_t threads) {
         hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), f.lds_bytes, s, d);
     };

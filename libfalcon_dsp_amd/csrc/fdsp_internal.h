@@ -127,8 +127,13 @@ constexpr uint32_t kMfmaMaxChan = 8;
 constexpr uint32_t kMfmaMinChan = 3;
 // largest down of an MFMA group (keeps a tile's buffer offsets below 2^30)
 constexpr uint32_t kMfmaMaxDown = 1u << 20;
-// groups whose first-block prefix travels in the kernel arguments
+// groups whose tiles-per-block count travels in the kernel arguments
 constexpr uint32_t kGroupsInline = 256;
+// MI355X: 256 CUs in 8 XCDs, each with its own L2; consecutive workgroup ids
+// are dealt round-robin to the XCDs
+constexpr uint32_t kXcds = 8;
+// block-table entry of a block with no work (pads the per-XCD queues)
+constexpr uint32_t kNoGroup = 0xffffffffu;
 // taps per LDS pass of the one-channel sparse path
 constexpr uint32_t kVSparsePass = 8;
 // outputs per lane of the one-channel dense path
@@ -153,8 +158,9 @@ struct GroupsArgs {
     const ChanPos *pos;         // device position (graph mode), or null:
     uint64_t p;                 //   then the position and parity come from here
     uint32_t parity;
-    const uint32_t *prefix;     // host: first block of each group, n_groups + 1 entries
-    const uint32_t *tps;        // host: tiles per block of each group
+    const uint32_t *table;      // device: (group, stripe) of each block, 2 x blocks u32
+    uint32_t blocks;
+    const uint32_t *tps;        // host: tiles per block (one stripe) of each group
 };
 hipError_t launch_chan_groups(const GroupsArgs &a, hipStream_t s);
 // advance the device position by n inputs and flip the history parity
