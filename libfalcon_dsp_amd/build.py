@@ -51,7 +51,20 @@ def build_dropin():
         return
     _run(["g++", "-O2", "-std=c++14", "-fPIC", "-shared", "-Wall", f"-I{os.path.join(HERE, 'hdr')}",
           f"-I{os.path.join(ROOT, 'include')}", *srcs, "-o", DROPIN_LIB, f"-L{HERE}", "-lfdsp_hip",
-          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+          "-Wl,-rpath,$ORIGIN"])
+
+
+DROPIN_TEST_SRC = os.path.join(ROOT, "tests", "dropin", "test_dropin.cc")
+DROPIN_TEST_BIN = os.path.join(ROOT, "tests", "dropin", "test_dropin")
+
+
+def build_dropin_test():
+    """The C++ program that drives the drop-in classes like the reference's unit tests."""
+    if not os.path.exists(DROPIN_TEST_SRC) or not os.path.exists(DROPIN_LIB):
+        return
+    _run(["g++", "-O2", "-std=c++14", "-Wall", f"-I{os.path.join(HERE, 'hdr')}", f"-I{os.path.join(ROOT, 'include')}",
+          DROPIN_TEST_SRC, "-o", DROPIN_TEST_BIN, f"-L{HERE}", "-lfalcon_dsp", "-lfdsp_hip",
+          "-Wl,-rpath,$ORIGIN/../../libfalcon_dsp_amd"])
 
 
 def build_oracle():
@@ -62,6 +75,7 @@ def build_oracle():
 def build_all():
     build_hip()
     build_dropin()
+    build_dropin_test()
     build_oracle()
 
 

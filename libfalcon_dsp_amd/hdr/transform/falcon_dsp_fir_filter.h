@@ -1,0 +1,34 @@
+// falcon_dsp_fir_filter.h -- streaming complex FIR, drop-in API.
+// (Same class and signatures as reference hdr/transform/falcon_dsp_fir_filter.h:87-140.)
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+struct fdsp_fir_s;
+
+namespace falcon_dsp
+{
+    bool fir_filter(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<float>>& in,
+                    std::vector<std::complex<float>>& out);
+
+    class falcon_dsp_fir_filter
+    {
+    public:
+        explicit falcon_dsp_fir_filter(std::vector<std::complex<float>>& coeffs);
+        virtual ~falcon_dsp_fir_filter(void);
+        falcon_dsp_fir_filter(void) = delete;
+        falcon_dsp_fir_filter(const falcon_dsp_fir_filter&) = delete;
+
+        void reset_state(void);
+        // y[n] = sum_k h[k] x[n-k] with the history carried between calls;
+        // false for an empty input or taps
+        virtual bool apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out);
+
+    protected:
+        std::mutex  m_mutex;
+        fdsp_fir_s *m_handle;
+    };
+}

@@ -1,0 +1,439 @@
+// falcon_dsp_dropin.cc -- the reference's class API (hdr/transform,
+// hdr/utilities) as thin host code over the C ABI in include/fdsp.h. Every
+// call is synchronous, as the reference's are: inputs are copied to the GPU,
+// the kernels run, outputs come back before the call returns. Errors surface
+// as the reference's bool / count returns; the C ABI's fdsp_last_error()
+// holds the reason.
+#include <cmath>
+#include <cstring>
+
+#include "fdsp.h"
+#include "transform/falcon_dsp_fir_filter_cuda.h"
+#include "transform/falcon_dsp_freq_shift_cuda.h"
+#include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
+#include "transform/falcon_dsp_polar_discriminator.h"
+#include "transform/falcon_dsp_polyphase_resampler_cuda.h"
+#include "transform/falcon_dsp_resample_cuda.h"
+#include "utilities/falcon_dsp_utils.h"
+
+namespace
+{
+// std::complex<float> is layout-compatible with float[2], as is fdsp_cf32
+inline const fdsp_cf32 *cf(const std::complex<float> *p) { return reinterpret_cast<const fdsp_cf32 *>(p); }
+inline fdsp_cf32 *cf(std::complex<float> *p) { return reinterpret_cast<fdsp_cf32 *>(p); }
+}  // namespace
+
+namespace falcon_dsp
+{
+// ---------------------------------------------------------------------------
+// utilities (reference src/utilities/falcon_dsp_utils.cc:86-137,231-294)
+// ---------------------------------------------------------------------------
+uint64_t calculate_gcd(uint64_t a, uint64_t b) { return fdsp_gcd(a, b); }
+uint64_t calculate_lcm(uint64_t a, uint64_t b) { return fdsp_lcm(a, b); }
+void rat_approx(double f, int64_t md, int64_t &num, int64_t &denom) { fdsp_rat_approx(f, md, &num, &denom); }
+
+uint32_t calculate_filter_delay_from_up_down_rates(uint32_t num_coeffs, uint32_t up_rate, uint32_t down_rate)
+{
+    const uint32_t divisor = up_rate > down_rate ? down_rate : up_rate;
+    return divisor ? num_coeffs / divisor / 2 : 0;
+}
+
+uint32_t calculate_filter_delay_from_sample_rates(uint32_t num_coeffs, uint32_t in_sample_rate_in_sps,
+                                                  uint32_t out_sample_rate_in_sps)
+{
+    int64_t up = 0, down = 0;
+    fdsp_rat_approx((double)out_sample_rate_in_sps / (double)in_sample_rate_in_sps, 1024, &up, &down);
+    return calculate_filter_delay_from_up_down_rates(num_coeffs, (uint32_t)up, (uint32_t)down);
+}
+
+// ---------------------------------------------------------------------------
+// frequency shift (reference src/transform/falcon_dsp_freq_shift.cc:107-235,
+// falcon_dsp_freq_shift_cuda.cu:275-488)
+// ---------------------------------------------------------------------------
+std::pair<uint32_t, double> falcon_dsp_freq_shift::get_freq_shift_params(uint32_t input_sample_rate_in_sps,
+                                                                         int32_t freq_shift_in_hz)
+{
+    uint32_t r = 0;
+    double w = 0.0;
+    fdsp_freq_shift_params(input_sample_rate_in_sps, freq_shift_in_hz, &r, &w);
+    return {r, w};
+}
+
+falcon_dsp_freq_shift::falcon_dsp_freq_shift(void) : m_handle(nullptr), m_num_shifts(0) {}
+
+falcon_dsp_freq_shift::~falcon_dsp_freq_shift(void) { fdsp_freq_shift_destroy(m_handle); }
+
+bool falcon_dsp_freq_shift::_create(uint32_t fs, const int32_t *shifts, uint32_t n)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle || n == 0)
+        return false;
+    if (fdsp_freq_shift_create(fs, shifts, n, &m_handle) != FDSP_OK) {
+        m_handle = nullptr;
+        return false;
+    }
+    m_num_shifts = n;
+    return true;
+}
+
+bool falcon_dsp_freq_shift::initialize(uint32_t input_sample_rate_in_sps, int32_t freq_shift_in_hz)
+{
+    return _create(input_sample_rate_in_sps, &freq_shift_in_hz, 1);
+}
+
+void falcon_dsp_freq_shift::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_freq_shift_reset(m_handle);
+}
+
+bool falcon_dsp_freq_shift::apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || m_num_shifts != 1 || in.empty())
+        return false;
+    out.resize(in.size());
+    fdsp_cf32 *o = cf(out.data());
+    if (fdsp_freq_shift_apply(m_handle, cf(in.data()), in.size(), &o) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+bool falcon_dsp_freq_shift_cuda::initialize(uint32_t input_sample_rate_in_sps, int32_t freq_shift_in_hz)
+{
+    return _create(input_sample_rate_in_sps, &freq_shift_in_hz, 1);
+}
+
+bool falcon_dsp_freq_shift_cuda::initialize(uint32_t input_sample_rate_in_sps, std::vector<int32_t> freq_shift_in_hz)
+{
+    return _create(input_sample_rate_in_sps, freq_shift_in_hz.data(), (uint32_t)freq_shift_in_hz.size());
+}
+
+bool falcon_dsp_freq_shift_cuda::apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out)
+{
+    return falcon_dsp_freq_shift::apply(in, out);
+}
+
+bool falcon_dsp_freq_shift_cuda::apply(std::vector<std::complex<float>>& in,
+                                       std::vector<std::vector<std::complex<float>>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.assign(m_num_shifts, std::vector<std::complex<float>>(in.size()));
+    std::vector<fdsp_cf32 *> outs(m_num_shifts);
+    for (uint32_t c = 0; c < m_num_shifts; ++c)
+        outs[c] = cf(out[c].data());
+    if (fdsp_freq_shift_apply(m_handle, cf(in.data()), in.size(), outs.data()) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+bool freq_shift(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in, int32_t freq_shift_in_hz,
+                std::vector<std::complex<float>>& out)
+{
+    falcon_dsp_freq_shift s;
+    return s.initialize(in_sample_rate_in_sps, freq_shift_in_hz) && s.apply(in, out);
+}
+
+bool freq_shift_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in, int32_t freq_shift_in_hz,
+                     std::vector<std::complex<float>>& out)
+{
+    return freq_shift(in_sample_rate_in_sps, in, freq_shift_in_hz, out);
+}
+
+bool freq_shift_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
+                     std::vector<int32_t>& freq_shift_in_hz, std::vector<std::vector<std::complex<float>>>& out)
+{
+    falcon_dsp_freq_shift_cuda s;
+    return s.initialize(in_sample_rate_in_sps, freq_shift_in_hz) && s.apply(in, out);
+}
+
+// ---------------------------------------------------------------------------
+// polyphase resampler (reference src/transform/falcon_dsp_polyphase_resampler.cc:114-346)
+// ---------------------------------------------------------------------------
+falcon_dsp_polyphase_resampler::falcon_dsp_polyphase_resampler(void) : m_handle(nullptr) {}
+
+falcon_dsp_polyphase_resampler::~falcon_dsp_polyphase_resampler(void) { fdsp_resampler_destroy(m_handle); }
+
+bool falcon_dsp_polyphase_resampler::initialize(uint32_t up_rate, uint32_t down_rate,
+                                                const std::vector<std::complex<float>>& filter_coeffs)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle || filter_coeffs.empty())
+        return false;
+    if (fdsp_resampler_create(up_rate, down_rate, cf(filter_coeffs.data()), (uint32_t)filter_coeffs.size(),
+                              &m_handle) != FDSP_OK) {
+        m_handle = nullptr;
+        return false;
+    }
+    return true;
+}
+
+void falcon_dsp_polyphase_resampler::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_resampler_reset(m_handle);
+}
+
+uint32_t falcon_dsp_polyphase_resampler::needed_out_count(uint32_t in_count)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    uint32_t n = 0;
+    if (m_handle)
+        fdsp_resampler_needed_out_count(m_handle, in_count, &n);
+    return n;
+}
+
+uint32_t falcon_dsp_polyphase_resampler::coeffs_per_phase(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    uint32_t cpp = 0;
+    if (m_handle)
+        fdsp_resampler_coeffs_per_phase(m_handle, &cpp);
+    return cpp;
+}
+
+int32_t falcon_dsp_polyphase_resampler::apply(std::vector<std::complex<float>>& in,
+                                              std::vector<std::complex<float>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return 0;
+    uint32_t need = 0;
+    fdsp_resampler_needed_out_count(m_handle, (uint32_t)in.size(), &need);
+    out.resize(need);
+    size_t got = 0;
+    if (fdsp_resampler_apply(m_handle, cf(in.data()), in.size(), cf(out.data()), out.size(), &got) != FDSP_OK) {
+        out.clear();
+        return 0;
+    }
+    out.resize(got);
+    return (int32_t)got;
+}
+
+int32_t falcon_dsp_polyphase_resampler::apply(std::vector<float>& in, std::vector<float>& out)
+{
+    // real input: the imaginary parts stay zero, the real parts come back
+    std::vector<std::complex<float>> cin(in.begin(), in.end()), cout;
+    const int32_t n = apply(cin, cout);
+    out.clear();
+    out.reserve(cout.size());
+    for (const auto &v : cout)
+        out.push_back(v.real());
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// FIR filter (reference src/transform/falcon_dsp_fir_filter.cc:136-284)
+// ---------------------------------------------------------------------------
+falcon_dsp_fir_filter::falcon_dsp_fir_filter(std::vector<std::complex<float>>& coeffs) : m_handle(nullptr)
+{
+    if (!coeffs.empty() && fdsp_fir_create(cf(coeffs.data()), (uint32_t)coeffs.size(), &m_handle) != FDSP_OK)
+        m_handle = nullptr;
+}
+
+falcon_dsp_fir_filter::~falcon_dsp_fir_filter(void) { fdsp_fir_destroy(m_handle); }
+
+void falcon_dsp_fir_filter::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_fir_reset(m_handle);
+}
+
+bool falcon_dsp_fir_filter::apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    if (fdsp_fir_apply(m_handle, cf(in.data()), in.size(), cf(out.data())) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+bool fir_filter(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<float>>& in,
+                std::vector<std::complex<float>>& out)
+{
+    falcon_dsp_fir_filter f(coeffs);
+    return f.apply(in, out);
+}
+
+bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<float>>& in,
+                     std::vector<std::complex<float>>& out)
+{
+    return fir_filter(coeffs, in, out);
+}
+
+// ---------------------------------------------------------------------------
+// polar discriminator (reference src/transform/falcon_dsp_polar_discriminator.cc:103-171)
+// ---------------------------------------------------------------------------
+falcon_dsp_polar_discriminator::falcon_dsp_polar_discriminator(void) : m_handle(nullptr)
+{
+    if (fdsp_discriminator_create(&m_handle) != FDSP_OK)
+        m_handle = nullptr;
+}
+
+falcon_dsp_polar_discriminator::~falcon_dsp_polar_discriminator(void) { fdsp_discriminator_destroy(m_handle); }
+
+void falcon_dsp_polar_discriminator::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_discriminator_reset(m_handle);
+}
+
+bool falcon_dsp_polar_discriminator::apply(const std::vector<std::complex<float>>& in, std::vector<float>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    size_t got = 0;
+    if (fdsp_discriminator_apply(m_handle, cf(in.data()), in.size(), out.data(), &got) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    out.resize(got);
+    return true;
+}
+
+bool polar_discriminator(const std::vector<std::complex<float>>& in, std::vector<float>& out)
+{
+    falcon_dsp_polar_discriminator d;
+    return d.apply(in, out);
+}
+
+// ---------------------------------------------------------------------------
+// multi-rate channelizer (reference src/transform/falcon_dsp_multi_rate_channelizer.cc:81-173,
+// falcon_dsp_multi_rate_channelizer_cuda.cu:375-577)
+// ---------------------------------------------------------------------------
+falcon_dsp_multi_rate_channelizer::falcon_dsp_multi_rate_channelizer(void) : m_handle(nullptr), m_num_channels(0) {}
+
+falcon_dsp_multi_rate_channelizer::~falcon_dsp_multi_rate_channelizer(void) { fdsp_channelizer_destroy(m_handle); }
+
+bool falcon_dsp_multi_rate_channelizer::_initialize(uint32_t input_sample_rate,
+                                                    const std::vector<multi_rate_channelizer_channel_s>& channels,
+                                                    int mode)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle || channels.empty())
+        return false;
+    std::vector<fdsp_channel_desc> d(channels.size());
+    for (size_t c = 0; c < channels.size(); ++c) {
+        const auto &ch = channels[c];
+        d[c] = fdsp_channel_desc{ch.output_sample_rate_in_sps, ch.freq_shift_in_hz, ch.up_rate, ch.down_rate,
+                                 cf(ch.resample_filter_coeffs.data()), (uint32_t)ch.resample_filter_coeffs.size(),
+                                 FDSP_TAIL_NONE};
+    }
+    if (fdsp_channelizer_create(input_sample_rate, d.data(), (uint32_t)d.size(), mode, &m_handle) != FDSP_OK) {
+        m_handle = nullptr;
+        return false;
+    }
+    m_num_channels = (uint32_t)channels.size();
+    return true;
+}
+
+bool falcon_dsp_multi_rate_channelizer::initialize(uint32_t input_sample_rate,
+                                                   std::vector<multi_rate_channelizer_channel_s> channels)
+{
+    return _initialize(input_sample_rate, channels, FDSP_CHAN_BATCHED);
+}
+
+bool falcon_dsp_multi_rate_channelizer_cuda::initialize(uint32_t input_sample_rate,
+                                                        std::vector<multi_rate_channelizer_channel_s> channels)
+{
+    return _initialize(input_sample_rate, channels, FDSP_CHAN_PLAIN);
+}
+
+bool falcon_dsp_multi_rate_channelizer::apply(std::vector<std::complex<float>>& in,
+                                              std::vector<std::vector<std::complex<float>>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle)
+        return false;
+    out.resize(m_num_channels);
+    if (in.empty())
+        return m_num_channels > 0;
+    std::vector<void *> ptrs(m_num_channels);
+    std::vector<size_t> caps(m_num_channels), counts(m_num_channels);
+    for (uint32_t c = 0; c < m_num_channels; ++c) {
+        uint32_t need = 0;
+        fdsp_channelizer_needed_out_count(m_handle, c, (uint32_t)in.size(), &need);
+        out[c].resize(need);
+        ptrs[c] = out[c].data();
+        caps[c] = need;
+    }
+    if (fdsp_channelizer_apply(m_handle, cf(in.data()), in.size(), ptrs.data(), caps.data(), counts.data()) !=
+        FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    for (uint32_t c = 0; c < m_num_channels; ++c)
+        out[c].resize(counts[c]);
+    return true;
+}
+
+void falcon_dsp_multi_rate_channelizer::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_channelizer_reset(m_handle);
+}
+
+// ---------------------------------------------------------------------------
+// one-shot resampling (reference src/transform/falcon_dsp_resample.cc:126-148)
+// ---------------------------------------------------------------------------
+bool resample(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
+              std::vector<std::complex<float>>& filter_coeffs, uint32_t out_sample_rate_in_sps,
+              std::vector<std::complex<float>>& out)
+{
+    int64_t up = 0, down = 0;
+    fdsp_rat_approx((double)out_sample_rate_in_sps / (double)in_sample_rate_in_sps, 1024, &up, &down);
+    falcon_dsp_polyphase_resampler r;
+    return r.initialize((uint32_t)up, (uint32_t)down, filter_coeffs) && r.apply(in, out) > 0;
+}
+
+bool upfirdn(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
+             std::vector<std::complex<float>>& filter_coeffs, uint32_t out_sample_rate_in_sps,
+             std::vector<std::complex<float>>& out)
+{
+    return resample(in_sample_rate_in_sps, in, filter_coeffs, out_sample_rate_in_sps, out);
+}
+
+bool upfirdn(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in, std::vector<float>& filter_coeffs,
+             uint32_t out_sample_rate_in_sps, std::vector<std::complex<float>>& out)
+{
+    std::vector<std::complex<float>> c(filter_coeffs.begin(), filter_coeffs.end());
+    return resample(in_sample_rate_in_sps, in, c, out_sample_rate_in_sps, out);
+}
+
+bool resample_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
+                   std::vector<std::complex<float>>& filter_coeffs, uint32_t out_sample_rate_in_sps,
+                   std::vector<std::complex<float>>& out)
+{
+    return resample(in_sample_rate_in_sps, in, filter_coeffs, out_sample_rate_in_sps, out);
+}
+
+bool upfirdn_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
+                  std::vector<float>& filter_coeffs, uint32_t out_sample_rate_in_sps,
+                  std::vector<std::complex<float>>& out)
+{
+    return upfirdn(in_sample_rate_in_sps, in, filter_coeffs, out_sample_rate_in_sps, out);
+}
+}  // namespace falcon_dsp

@@ -1,0 +1,273 @@
+// test_dropin.cc -- the C++ drop-in classes (libfalcon_dsp_amd/hdr, built into
+// libfalcon_dsp.so) driven the way the reference's own unit tests drive its
+// classes, with their known answers (reference test/tests/transform/*_unit_tests,
+// test/tests/utilities/falcon_dsp_utils_unit_tests.cc), plus closed forms.
+// Needs a GPU. Prints one line per check; exit status 0 when all pass.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdint>
+#include <vector>
+
+#include "transform/falcon_dsp_fir_filter_cuda.h"
+#include "transform/falcon_dsp_freq_shift_cuda.h"
+#include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
+#include "transform/falcon_dsp_polar_discriminator.h"
+#include "transform/falcon_dsp_polyphase_resampler_cuda.h"
+#include "transform/falcon_dsp_resample_cuda.h"
+#include "utilities/falcon_dsp_utils.h"
+
+using cf = std::complex<float>;
+using namespace falcon_dsp;
+
+static int g_fail = 0;
+
+static void check(bool ok, const char *what)
+{
+    std::printf("%s %s\n", ok ? "PASS" : "FAIL", what);
+    std::fflush(stdout);
+    if (!ok)
+        ++g_fail;
+}
+
+// deterministic complex noise (splitmix64 -> Box-Muller), unit variance
+static std::vector<cf> noise(uint64_t seed, size_t n)
+{
+    auto next = [&seed]() {
+        uint64_t z = (seed += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    std::vector<cf> x(n);
+    for (auto &v : x) {
+        const double u1 = next() + 1e-300, u2 = next();
+        const double r = std::sqrt(-std::log(u1)), t = 2 * M_PI * u2;
+        v = cf((float)(r * std::cos(t)), (float)(r * std::sin(t)));
+    }
+    return x;
+}
+
+static double rms(const std::vector<cf> &x)
+{
+    double s = 0;
+    for (auto v : x)
+        s += std::norm(std::complex<double>(v));
+    return x.empty() ? 0 : std::sqrt(s / x.size());
+}
+
+static void test_utils()
+{
+    check(calculate_gcd(840, 210) == 210 && calculate_gcd(2857, 97) == 1 && calculate_gcd(200, 600) == 200,
+          "calculate_gcd KATs");
+    check(calculate_lcm(382, 164) == 31324 && calculate_lcm(600, 200) == 600, "calculate_lcm KATs");
+    int64_t n = 0, d = 0;
+    bool ok = true;
+    const double f[] = {1.0, 0.5, 0.5, 0.33333, 0.28571, 0.625, 0.768, 1.30208};
+    const int64_t md[] = {1, 1, 10, 100, 100, 100, 100, 100};
+    const int64_t want[][2] = {{1, 1}, {0, 1}, {1, 2}, {1, 3}, {2, 7}, {5, 8}, {53, 69}, {125, 96}};
+    for (int i = 0; i < 8; ++i) {
+        rat_approx(f[i], md[i], n, d);
+        ok = ok && n == want[i][0] && d == want[i][1];
+    }
+    check(ok, "rat_approx KATs");
+}
+
+static void test_resampler_kats()
+{
+    // upfirdn([.5, 1, 2], ones(7), 1, 2) = [.5, 3.5, 3.5, 3.5, 2]
+    {
+        std::vector<cf> h = {{0.5f, 0}, {1.0f, 0}, {2.0f, 0}}, x(7, cf(1, 0)), y;
+        falcon_dsp_polyphase_resampler_cuda r;
+        const float want[] = {0.5f, 3.5f, 3.5f, 3.5f};
+        bool ok = r.initialize(1, 2, h) && r.apply(x, y) == 4;
+        for (int i = 0; ok && i < 4; ++i)
+            ok = std::abs(y[i] - cf(want[i], 0)) < 1e-5f;
+        check(ok, "resampler basic_001 (1/2, 3 taps, ones(7))");
+    }
+    {
+        std::vector<cf> h = {{0.5f, 0}, {0.5f, 0}, {1.0f, 0}, {0.3f, 0}, {0.07f, 0}, {0.1f, 0}, {0.4f, 0}};
+        std::vector<cf> x(1000, cf(1, 0)), y;
+        falcon_dsp_polyphase_resampler_cuda r;
+        bool ok = r.initialize(1, 2, h) && r.apply(x, y) == 500;
+        ok = ok && std::abs(y[0] - cf(0.5f, 0)) < 1e-5f && std::abs(y[1] - cf(2.0f, 0)) < 1e-5f &&
+             std::abs(y[2] - cf(2.37f, 0)) < 1e-5f;
+        for (int i = 3; ok && i < 500; ++i)
+            ok = std::abs(y[i] - cf(2.87f, 0)) < 1e-5f;
+        check(ok, "resampler basic_003 (1/2, 7 taps, ones(1000))");
+    }
+    {
+        std::vector<cf> h = {{0.5f, 0}, {1.0f, 0}, {2.0f, 0}}, x(10, cf(1, 0)), y;
+        falcon_dsp_polyphase_resampler_cuda r;
+        const float want[] = {0.5f, 1.0f, 2.0f, 0.0f, 0.5f, 1.0f, 2.0f, 0.0f};
+        bool ok = r.initialize(4, 5, h) && r.apply(x, y) == 8;
+        for (int i = 0; ok && i < 8; ++i)
+            ok = std::abs(y[i] - cf(want[i], 0)) < 1e-5f;
+        check(ok, "resampler basic_004 (4/5, 3 taps, ones(10))");
+    }
+    {
+        // streaming: any split gives the same outputs; counts match needed_out_count
+        auto x = noise(3, 20000);
+        std::vector<cf> h(96);
+        for (int i = 0; i < 96; ++i)
+            h[i] = cf((float)(std::sin(0.3 * (i - 47.5)) / (i - 47.5)), 0.f);
+        falcon_dsp_polyphase_resampler a, b;
+        std::vector<cf> ya, yb, part, in;
+        bool ok = a.initialize(4, 5, h) && b.initialize(4, 5, h) && a.coeffs_per_phase() == 24;
+        a.apply(x, ya);
+        const size_t sizes[] = {1, 2, 4999, 3, 7000, 7995};
+        size_t at = 0;
+        for (size_t s : sizes) {
+            in.assign(x.begin() + at, x.begin() + at + s);
+            const uint32_t need = b.needed_out_count((uint32_t)s);
+            ok = ok && b.apply(in, part) == (int32_t)need;
+            yb.insert(yb.end(), part.begin(), part.end());
+            at += s;
+        }
+        ok = ok && ya.size() == yb.size() && ya.size() == 16000;
+        double e = 0;
+        for (size_t i = 0; ok && i < ya.size(); ++i)
+            e = std::max(e, (double)std::abs(ya[i] - yb[i]));
+        check(ok && e < 1e-5 * rms(x) * 4, "resampler chunk invariance and needed_out_count");
+        std::vector<cf> yr;
+        ok = resample(1000000, x, h, 800000, yr) && yr.size() == ya.size();
+        check(ok, "resample() free function (rat_approx 4/5)");
+    }
+}
+
+static void test_fir_and_discriminator()
+{
+    std::vector<cf> h = {{0.5f, 0}, {2.0f, 0}, {4.0f, 0}};
+    {
+        falcon_dsp_fir_filter_cuda f(h);
+        std::vector<cf> x = {{1, 0}, {1, 0}, {1, 0}, {2, 0}, {2, 0}, {2, 0}, {5, 0}, {5, 0}, {5, 0}, {1, 0}, {1, 0}};
+        std::vector<cf> y;
+        const float want[] = {0.5f, 2.5f, 6.5f, 7.0f, 9.0f, 13.0f, 14.5f, 20.5f, 32.5f, 30.5f, 22.5f};
+        bool ok = f.apply(x, y) && y.size() == 11;
+        for (int i = 0; ok && i < 11; ++i)
+            ok = y[i] == cf(want[i], 0);
+        check(ok, "fir basic_filter_float_003");
+    }
+    {
+        // one sample per call carries the history (basic_filter_float_002)
+        falcon_dsp_fir_filter f(h);
+        std::vector<cf> one(1, cf(1, 0)), y;
+        const float want[] = {0.5f, 2.5f, 6.5f, 6.5f, 6.5f};
+        bool ok = true;
+        for (int i = 0; i < 5; ++i)
+            ok = ok && f.apply(one, y) && y.size() == 1 && y[0] == cf(want[i], 0);
+        std::vector<cf> empty;
+        check(ok && !f.apply(empty, y), "fir one sample per call; empty input is an error");
+    }
+    {
+        std::vector<cf> c = {{88, -85}, {22, 97}, {54, 91}, {6, -13}, {-29, 58}};
+        std::vector<float> y;
+        const float want[] = {2.11582422f, -0.31252633f, -2.17362758f, -3.11035282f};
+        falcon_dsp_polar_discriminator d;
+        bool ok = d.apply(c, y) && y.size() == 4;
+        for (int i = 0; ok && i < 4; ++i)
+            ok = std::fabs(y[i] - want[i]) < 1e-4f;
+        ok = ok && d.apply(c, y) && y.size() == 5;  // later calls pair with the carried sample
+        check(ok, "polar discriminator KAT and carried sample");
+    }
+}
+
+static void test_freq_shift()
+{
+    const uint32_t fs = 1000000;
+    const int32_t f = 23000;
+    auto p = falcon_dsp_freq_shift::get_freq_shift_params(fs, f);
+    check(p.first == 434783 && std::fabs(p.second - 2 * M_PI * f / fs) < 1e-15, "get_freq_shift_params");
+    auto x = noise(5, 1000000);
+    falcon_dsp_freq_shift s;
+    std::vector<cf> y, part, in;
+    bool ok = s.initialize(fs, f) && !s.initialize(fs, f);
+    size_t at = 0;
+    for (size_t n : {400000, 1, 599999}) {
+        in.assign(x.begin() + at, x.begin() + at + n);
+        ok = ok && s.apply(in, part);
+        y.insert(y.end(), part.begin(), part.end());
+        at += n;
+    }
+    double e = 0;
+    for (size_t i = 0; ok && i < x.size(); ++i) {
+        const double th = p.second * (double)(i % p.first);
+        const std::complex<double> want = std::complex<double>(x[i]) * std::polar(1.0, th);
+        e = std::max(e, std::abs(std::complex<double>(y[i]) - want));
+    }
+    check(ok && y.size() == x.size() && e < 2e-6 * rms(x), "freq shift vs FP64 closed form across rollover");
+    // multi-shift: one pass, one output per shift, equal to single shifts
+    std::vector<int32_t> shifts = {-9000000, 5, 1234567};
+    std::vector<std::vector<cf>> ys;
+    auto x2 = noise(6, 100000);
+    ok = freq_shift_cuda(25000000, x2, shifts, ys) && ys.size() == 3;
+    for (size_t c = 0; ok && c < 3; ++c) {
+        std::vector<cf> one;
+        ok = freq_shift(25000000, x2, shifts[c], one) && one.size() == x2.size();
+        for (size_t i = 0; ok && i < one.size(); ++i)
+            ok = std::abs(one[i] - ys[c][i]) < 1e-6f * 4;
+    }
+    check(ok, "multi-shift freq_shift_cuda equals single shifts");
+}
+
+static void test_channelizer()
+{
+    const uint32_t fs = 20000000;
+    std::vector<multi_rate_channelizer_channel_s> chans;
+    const uint32_t rates[] = {250000, 250000, 250000, 2500000, 100000, 3000000};
+    const int64_t shifts[] = {1234567, -8000000, 5, -3000001, 7777777, -1};
+    for (int c = 0; c < 6; ++c) {
+        multi_rate_channelizer_channel_s d;
+        const uint64_t g = calculate_gcd(fs, rates[c]);
+        d.output_sample_rate_in_sps = rates[c];
+        d.freq_shift_in_hz = shifts[c];
+        d.up_rate = (uint32_t)(rates[c] / g);
+        d.down_rate = (uint32_t)(fs / g);
+        const uint32_t m = 32 * d.up_rate;
+        const double fc = 0.5 / std::max(d.up_rate, d.down_rate);
+        double sum = 0;
+        std::vector<double> t(m);
+        for (uint32_t i = 0; i < m; ++i) {
+            const double k = i - (m - 1) / 2.0;
+            t[i] = k == 0 ? 2 * fc : std::sin(2 * M_PI * fc * k) / (M_PI * k);
+            sum += t[i];
+        }
+        for (uint32_t i = 0; i < m; ++i)
+            d.resample_filter_coeffs.push_back(cf((float)(t[i] / sum * d.up_rate), 0.f));
+        chans.push_back(d);
+    }
+    falcon_dsp_multi_rate_channelizer batched;
+    falcon_dsp_multi_rate_channelizer_cuda plain;
+    bool ok = batched.initialize(fs, chans) && plain.initialize(fs, chans) && !batched.initialize(fs, chans);
+    auto x = noise(7, 600000);
+    const double xr = rms(x);
+    std::vector<std::vector<cf>> ya, yb;
+    size_t at = 0;
+    double e = 0;
+    for (size_t n : {262144, 3, 337853}) {
+        std::vector<cf> in(x.begin() + at, x.begin() + at + n);
+        ok = ok && batched.apply(in, ya) && plain.apply(in, yb) && ya.size() == 6 && yb.size() == 6;
+        for (size_t c = 0; ok && c < 6; ++c) {
+            ok = ya[c].size() == yb[c].size();
+            for (size_t i = 0; ok && i < ya[c].size(); ++i)
+                e = std::max(e, (double)std::abs(ya[c][i] - yb[c][i]));
+        }
+        at += n;
+    }
+    check(ok && e <= 1e-4 * xr, "channelizer: batched class equals the per-channel class (1e-4 RMS)");
+    batched.reset_state();
+    std::vector<cf> first(x.begin(), x.begin() + 1000);
+    ok = batched.apply(first, ya) && ya[0].size() == 13;  // ceil(1000/80)
+    check(ok, "channelizer reset_state restarts the stream");
+}
+
+int main()
+{
+    test_utils();
+    test_resampler_kats();
+    test_fir_and_discriminator();
+    test_freq_shift();
+    test_channelizer();
+    std::printf("%s: %d failed\n", g_fail ? "FAILED" : "OK", g_fail);
+    return g_fail ? 1 : 0;
+}

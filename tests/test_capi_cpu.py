@@ -75,3 +75,20 @@ def test_python_rate_helpers():
     # reduce_rates gives the channelizer's up/down from a rate pair
     assert fd.reduce_rates(20_000_000, 250_000) == (1, 80)
     assert fd.reduce_rates(1_000_000, 800_000) == (4, 5)
+
+
+def test_dropin_library_exports_reference_classes():
+    import subprocess
+    lib = os.path.join(ROOT, "libfalcon_dsp_amd", "libfalcon_dsp.so")
+    assert os.path.exists(lib), "build first: python -m libfalcon_dsp_amd.build"
+    C.CDLL(lib)
+    syms = subprocess.run(["nm", "-D", "-C", "--defined-only", lib], capture_output=True, text=True).stdout
+    for name in ["falcon_dsp::falcon_dsp_freq_shift::apply", "falcon_dsp::falcon_dsp_freq_shift::get_freq_shift_params",
+                 "falcon_dsp::falcon_dsp_freq_shift_cuda::initialize", "falcon_dsp::falcon_dsp_polyphase_resampler::apply",
+                 "falcon_dsp::falcon_dsp_polyphase_resampler::needed_out_count",
+                 "falcon_dsp::falcon_dsp_fir_filter::apply", "falcon_dsp::falcon_dsp_polar_discriminator::apply",
+                 "falcon_dsp::falcon_dsp_multi_rate_channelizer::apply",
+                 "falcon_dsp::falcon_dsp_multi_rate_channelizer_cuda::initialize", "falcon_dsp::resample(",
+                 "falcon_dsp::upfirdn_cuda(", "falcon_dsp::fir_filter_cuda(", "falcon_dsp::polar_discriminator(",
+                 "falcon_dsp::freq_shift_cuda(", "falcon_dsp::calculate_gcd", "falcon_dsp::rat_approx"]:
+        assert name in syms, name
