@@ -220,13 +220,28 @@ int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t stream);
 int fdsp_channelizer_graph_counts(fdsp_channelizer_t h, size_t *counts);
 int fdsp_channelizer_graph_destroy(fdsp_channelizer_t h);
 
-/* Work split of the batched kernel (picked by the autotuner): one block runs
- * consecutive output tiles of one channel group until it has covered about
- * segment_len input samples (a power of two in [256, 2^20]); block_threads is
- * 128, 256 or 512. Outputs agree across settings to float32 rounding of the
- * NCO phasor (a few ulp). */
+/* Work split of the batched kernel (picked by the autotuner). The first two
+ * forms launch as many blocks as the GPU holds at once, each starting in a
+ * channel group picked in proportion to the group's estimated cost.
+ *  - FDSP_TILING_STRIDED (1, the default): block s of a group runs its tiles
+ *    s, s + b, ... (b = the group's block count).
+ *  - FDSP_TILING_CLAIM (0): blocks claim output tiles one at a time from
+ *    per-channelizer device counters and move to other groups when theirs
+ *    is done. Launches of one channelizer must not overlap in time (they
+ *    share the counters); the last block of a launch resets them.
+ *  - a power of two in [256, 2^20]: one block runs consecutive output tiles
+ *    of one channel group until it has covered about segment_len inputs.
+ * block_threads is 128, 256 or 512. Outputs agree across settings to float32
+ * rounding of the NCO phasor (a few ulp). */
+#define FDSP_TILING_CLAIM 0u
+#define FDSP_TILING_STRIDED 1u
 int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads);
 int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads);
+/* The batched kernel's work split for a chunk of chunk_len samples: blocks
+ * launched, blocks the GPU holds at once, channel groups, LDS bytes per block
+ * (diagnostics for the autotuner; any output may be null). */
+int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t *blocks, uint32_t *capacity,
+                               uint32_t *groups, uint32_t *lds_bytes);
 
 #ifdef __cplusplus
 }

@@ -128,6 +128,7 @@ _SIGS = [
     ("fdsp_channelizer_graph_destroy", _i, [_vp]),
     ("fdsp_channelizer_set_tiling", _i, [_vp, _u32, _u32]),
     ("fdsp_channelizer_get_tiling", _i, [_vp, _P(_u32), _P(_u32)]),
+    ("fdsp_channelizer_plan_info", _i, [_vp, _sz, _P(_u32), _P(_u32), _P(_u32), _P(_u32)]),
 ]
 
 SYMBOLS = [s[0] for s in _SIGS]

@@ -992,11 +992,13 @@ struct fdsp_channelizer_s {
     bool stat_valid = false;
     bool pos_dirty = false;         // device position behind the host's (eager calls since the last graph launch)
     uint32_t taps_total = 0, hist_len = 0;
-    uint32_t seg_len = 8192, block = 256;
+    uint32_t seg_len = FDSP_TILING_STRIDED, block = 256;
+    uint32_t capacity = 1024;  // fused-kernel blocks resident at once (occupancy x CUs)
     // work plan: channel groups (kGroupWords u32 each) and their MFMA B tables
     std::vector<uint32_t> groups;
     std::vector<float> btab;
     DevBuf d_groups, d_btab;
+    DevBuf d_claims;  // per-group tile claim counters + a finished-block counter (zero between launches)
     uint32_t n_groups = 0, lds_bytes = 0;
     std::vector<uint32_t> tps;
     // block tables by chunk length (eager calls), and the graph's own
@@ -1126,6 +1128,20 @@ int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t *lds)
     return FDSP_OK;
 }
 
+// blocks of the fused kernel the GPU holds at once for the current plan
+int update_capacity(fdsp_channelizer_s *h)
+{
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    bool wide = false;
+    for (uint32_t g = 0; g < h->n_groups; ++g)
+        wide |= h->groups[g * kGroupWords] == kGroupMfma && h->groups[g * kGroupWords + 3] == 64;
+    const int per_cu = chan_groups_occupancy(h->block, wide, h->lds_bytes);
+    h->capacity = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+    return FDSP_OK;
+}
+
 // Group the channels. Channels with up 1, the same down and the same padded
 // window (32 or 64) form MFMA groups of up to kMfmaMaxChan when there are at
 // least kMfmaMinChan of them; every other channel is a one-channel VALU group.
@@ -1196,12 +1212,41 @@ void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
     h->n_groups = (uint32_t)(h->groups.size() / kGroupWords);
 }
 
+// Estimated cost of one tile of a group, in GPU-wide cycles: the larger of
+// its matrix-core or vector-ALU time spread over all SIMDs and its HBM bytes
+// at about 6 TB/s (2500 B per 2.4 GHz cycle).
+double tile_cost(const uint32_t *gw, uint32_t block)
+{
+    const uint32_t kind = gw[0], up = gw[1], down = gw[2], w = gw[3], m = gw[4], rows = gw[5];
+    const double simds = 1024.0, bytes_per_cycle = 2500.0;
+    double compute, in_samples;
+    if (kind == kGroupMfma) {
+        compute = std::ceil(rows / 16.0) * (2.0 * (w / 4)) * 32.0 + rows / 16.0 * 200.0;
+        in_samples = down < w ? (double)rows * down + w : (double)rows * w;
+    } else {
+        // four FMAs per complex tap per output, 64 lanes per 4-cycle wave instruction, 2x for the rest
+        compute = (double)rows * w * up * 4.0 / 64.0 * 4.0 * 2.0 / (up > 1 ? up : 1);
+        in_samples = kind == kGroupVSparse ? (double)rows * w : (double)rows * down / up + w;
+    }
+    const double bytes = 8.0 * in_samples + 8.0 * rows * m;
+    (void)block;
+    return std::max(compute / simds, bytes / bytes_per_cycle);
+}
+
 // Work split of a chunk of n inputs (a chunk emits at most floor(n*up/down)+1
-// outputs per channel). Each block runs a stripe of tps consecutive tiles of
-// one group covering about seg_len inputs. Stripes are dealt to the kXcds
-// queues by the input segment (seg_len inputs) they start in, so all groups'
-// stripes over the same input go to one XCD and share its L2; the queues are
-// then interleaved to follow the round-robin dispatch of workgroups to XCDs.
+// outputs per channel): each block runs a stripe of tps consecutive tiles of
+// one group.
+//  - FDSP_TILING_STRIDED (default) and FDSP_TILING_CLAIM: the grid is the
+//    number of blocks the GPU holds at once; each group gets blocks in
+//    proportion to its estimated cost. Strided: the group's tiles are split
+//    among its blocks (block s runs s, s + blocks, ...). Claimed: the table
+//    only names each block's first group; blocks take tiles one at a time
+//    from per-group counters and move on to the other groups when theirs
+//    runs out, so they all finish together whatever the cost estimate said.
+//  - seg_len > 0: stripes cover about seg_len inputs, dealt to the kXcds
+//    queues by the input segment they start in (all groups' stripes over the
+//    same input share one XCD's L2), the queues interleaved to follow the
+//    round-robin dispatch of workgroups to XCDs.
 // table: (group, stripe) per block, kNoGroup where a queue is short.
 void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table)
 {
@@ -1211,15 +1256,43 @@ void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table
         uint32_t g, s;
     };
     std::vector<Item> items;
+    std::vector<uint64_t> tiles(h->n_groups), span(h->n_groups);
+    double total = 0;
     for (uint32_t g = 0; g < h->n_groups; ++g) {
         const uint32_t *gw = h->groups.data() + g * kGroupWords;
         const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
-        const uint64_t tiles = (most + gw[5] - 1) / gw[5];
-        const uint64_t span = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
-        const uint64_t tps = std::min<uint64_t>(std::max<uint64_t>(1, h->seg_len / span), tiles);
+        tiles[g] = (most + gw[5] - 1) / gw[5];
+        span[g] = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
+        total += tiles[g] * tile_cost(gw, h->block);
+    }
+    for (uint32_t g = 0; g < h->n_groups; ++g) {
+        uint64_t tps;
+        if (h->seg_len <= FDSP_TILING_STRIDED) {
+            // strided split: the group's blocks sweep its tiles together, tps = block count
+            const double share = tiles[g] * tile_cost(h->groups.data() + g * kGroupWords, h->block) / total;
+            const uint64_t blocks = std::min<uint64_t>(
+                tiles[g], std::max<uint64_t>(1, (uint64_t)std::llround(share * h->capacity)));
+            h->tps[g] = (uint32_t)blocks;
+            for (uint64_t st = 0; st < blocks; ++st)
+                items.push_back({st * span[g], g, (uint32_t)st});
+            continue;
+        } else {
+            tps = std::min<uint64_t>(std::max<uint64_t>(1, h->seg_len / span[g]), tiles[g]);
+        }
         h->tps[g] = (uint32_t)tps;
-        for (uint64_t st = 0; st * tps < tiles; ++st)
-            items.push_back({st * tps * span, g, (uint32_t)st});
+        for (uint64_t st = 0; st * tps < tiles[g]; ++st)
+            items.push_back({st * tps * span[g], g, (uint32_t)st});
+    }
+    if (h->seg_len <= FDSP_TILING_STRIDED) {
+        // all blocks are resident together: interleave the groups so each
+        // XCD gets a share of every group
+        std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.start < y.start; });
+        table->clear();
+        for (const Item &it : items) {
+            table->push_back(it.g);
+            table->push_back(it.s);
+        }
+        return;
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.start < y.start; });
     std::vector<std::vector<uint32_t>> q(kXcds);
@@ -1302,6 +1375,8 @@ int groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool d
     if (int rc = table_for(h, n, device_pos, &f.table, &f.blocks))
         return rc;
     f.tps = h->tps.data();
+    f.strided = h->seg_len <= FDSP_TILING_STRIDED;
+    f.claims = h->seg_len == FDSP_TILING_CLAIM ? h->d_claims.as<uint32_t>() : nullptr;
     *out = f;
     return FDSP_OK;
 }
@@ -1410,6 +1485,8 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
         if (!rc)
             rc = size_groups(h->groups, h->block, &h->lds_bytes);
         if (!rc)
+            rc = update_capacity(h);
+        if (!rc)
             rc = h->d_groups.ensure(h->groups.size() * sizeof(uint32_t));
         if (!rc)
             rc = h->d_btab.ensure(std::max<size_t>(h->btab.size(), 1) * sizeof(float));
@@ -1429,8 +1506,11 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             rc = h->d_hist[1].ensure(hb);
         if (!rc)
             rc = h->d_pos.ensure(sizeof(ChanPos));
+        if (!rc)
+            rc = h->d_claims.ensure(kClaimWords * sizeof(uint32_t));
         if (!rc && (hipMemcpy(h->d_taps.p, taps.data(), taps.size() * sizeof(float2), hipMemcpyHostToDevice) !=
                         hipSuccess ||
+                    hipMemset(h->d_claims.p, 0, kClaimWords * sizeof(uint32_t)) != hipSuccess ||
                     hipMemset(h->d_hist[0].p, 0, hb) != hipSuccess || hipMemset(h->d_hist[1].p, 0, hb) != hipSuccess ||
                     hipMemset(h->d_pos.p, 0, sizeof(ChanPos)) != hipSuccess ||
                     hipHostMalloc((void **)&h->pinned_stat, sizeof(ChanStatic) * n_chan, hipHostMallocDefault) !=
@@ -1486,6 +1566,7 @@ extern "C" int fdsp_channelizer_destroy(fdsp_channelizer_t h)
     h->d_hist[0].release();
     h->d_hist[1].release();
     h->d_pos.release();
+    h->d_claims.release();
     h->d_groups.release();
     h->d_btab.release();
     h->d_table_graph.release();
@@ -1812,8 +1893,10 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
         return set_err(FDSP_ENOTINIT, "null handle");
     if (h->mode != FDSP_CHAN_BATCHED)
         return set_err(FDSP_EINVAL, "tiling applies to the batched mode");
-    if (segment_len < 256 || segment_len > (1u << 20) || (segment_len & (segment_len - 1)))
-        return set_err(FDSP_EINVAL, "segment_len must be a power of two in [256, 2^20]");
+    if (segment_len > FDSP_TILING_STRIDED &&
+        (segment_len < 256 || segment_len > (1u << 20) || (segment_len & (segment_len - 1))))
+        return set_err(FDSP_EINVAL, "segment_len must be FDSP_TILING_CLAIM, FDSP_TILING_STRIDED or a power of two in "
+                                    "[256, 2^20]");
     if (block_threads != 128 && block_threads != 256 && block_threads != 512)
         return set_err(FDSP_EINVAL, "block_threads must be 128, 256 or 512");
     if (h->graph_exec)
@@ -1832,6 +1915,28 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
     h->tables.clear();
     h->seg_len = segment_len;
     h->block = block_threads;
+    if (int rc = update_capacity(h))
+        return rc;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t *blocks,
+                                          uint32_t *capacity, uint32_t *groups, uint32_t *lds_bytes)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "plan info applies to the batched mode");
+    std::vector<uint32_t> t;
+    block_table(h, (uint32_t)chunk_len, &t);
+    if (blocks)
+        *blocks = (uint32_t)(t.size() / 2);
+    if (capacity)
+        *capacity = h->capacity;
+    if (groups)
+        *groups = h->n_groups;
+    if (lds_bytes)
+        *lds_bytes = h->lds_bytes;
     return FDSP_OK;
 }
 

@@ -67,7 +67,9 @@ struct GroupsDev {
     const uint32_t *groups;
     uint32_t n_groups;
     const uint32_t *table;  // (group, stripe) of each block
-    uint32_t tps[kGroupsInline];  // tiles per stripe of each group
+    uint32_t strided;       // 0: tps consecutive tiles per block; 1: every tps-th tile
+    uint32_t *claims;       // non-null: blocks claim tiles from per-group counters
+    uint32_t tps[kGroupsInline];  // tiles per stripe, or blocks, of each group
 };
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -221,6 +223,63 @@ __device__ __forceinline__ void gather_windows(const GroupsDev &a, const float2 
 }
 
 // ---------------------------------------------------------------------------
+// tile sequences
+// ---------------------------------------------------------------------------
+// A block's tiles of one group come either from a fixed sequence (tb, tb+step,
+// ... below end) or from the group's claim counters: shard s hands out tiles
+// s, s + kClaimShards, ... in turn, and a block draws from one shard until it
+// is spent, then from the next. Each claim is issued one tile ahead (its
+// result is read at the next claim), so its latency hides behind a tile of
+// work. claim() is called by all threads; only thread 0 touches the
+// counters. It leaves the next tile (or ~0 when there is none) in *slot,
+// which needs a barrier before it is read.
+struct TileSeq {
+    uint32_t *ctr;  // the group's counters, or null for the fixed sequence
+    uint32_t next, step, end;
+    uint32_t tiles;
+    uint32_t first;   // shard drawn from first
+    uint32_t *spent;  // LDS word (thread 0): shards found spent; the shard drawn from is first + *spent
+    uint32_t pend;    // thread 0: the claim in flight
+    __device__ __forceinline__ void start(uint32_t first_shard, uint32_t *spent_word)
+    {
+        first = first_shard;
+        spent = spent_word;
+        if (ctr && threadIdx.x == 0) {
+            *spent = 0;
+            pend = atomicAdd(ctr + (first % kClaimShards) * kClaimPitch, 1u);
+        }
+    }
+    __device__ __forceinline__ void claim(uint32_t *slot)
+    {
+        if (ctr) {
+            if (threadIdx.x == 0) {
+                uint32_t n_spent = *spent;
+                uint32_t shard = (first + n_spent) % kClaimShards;
+                uint32_t t = shard + kClaimShards * pend;
+                // at most kClaimShards - 1 more shards to try
+                while (t >= tiles && n_spent + 1 < kClaimShards) {
+                    ++n_spent;
+                    shard = (first + n_spent) % kClaimShards;
+                    t = shard + kClaimShards * atomicAdd(ctr + shard * kClaimPitch, 1u);
+                }
+                if (t < tiles) {
+                    *slot = t;
+                    pend = atomicAdd(ctr + shard * kClaimPitch, 1u);
+                } else {
+                    *slot = 0xffffffffu;
+                    n_spent = kClaimShards;
+                }
+                *spent = n_spent;
+            }
+        } else {
+            if (threadIdx.x == 0)
+                *slot = next < end ? next : 0xffffffffu;
+            next += step;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
 // MFMA groups
 // ---------------------------------------------------------------------------
 // A tile's input passes through registers (kStage samples per thread) on
@@ -311,12 +370,14 @@ __device__ __forceinline__ void mfma_commit(bool dense, uint32_t down, uint32_t 
     }
 }
 
-// Tiles tb .. tb_end-1 of an MFMA group (tile i: rows i*rows .. of the chunk's
-// `count` outputs, first output n0 of the stream).
+// The tiles of an MFMA group that seq hands this block, the first one already
+// in *slot (tile i: rows i*rows .. of the chunk's `count` outputs, first
+// output n0 of the stream). The next tile is claimed while the current one
+// is staged, so its loads overlap the current tile's MFMA work.
 template <int kBlock, uint32_t kS, bool kDense>
 __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P,
-                                            const uint32_t *gw, uint64_t n0, uint32_t count, uint32_t tb,
-                                            uint32_t tb_end, unsigned char *smem)
+                                            const uint32_t *gw, uint64_t n0, uint32_t count, TileSeq seq,
+                                            uint32_t *slot, unsigned char *smem)
 {
     constexpr uint32_t kW = 4 * kS;  // padded taps per window (zero taps in front)
     constexpr uint32_t kWaves = kBlock / 64;
@@ -391,23 +452,23 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
     const uint32_t du = 16 * kWaves * down;  // from one of the wave's sub-tiles to the next
     const float2 step = nco_phasor((int64_t)du, nu);
 
+    // tiles of this chunk: ceil(count / rows); the first one was claimed by
+    // the caller (in *slot)
+    const uint32_t tiles = (count + rows - 1) / rows;
+    uint32_t tb = uniform(*slot);
     float2 v[mfma_stage(kBlock)];
-    if (tb < tb_end && tb * rows < count)
-        mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + tb * rows) * down - P), down, v);
-    for (; tb < tb_end; ++tb) {
+    mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + tb * rows) * down - P), down, v);
+    for (;;) {
         const uint32_t j0 = tb * rows;
-        if (j0 >= count)
-            break;  // uniform
         const uint32_t nt = min(rows, count - j0);
         const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // chunk-local anchor of row 0
-        __syncthreads();  // the previous tile's reads of LDS are done
+        __syncthreads();  // the previous tile's reads of LDS (and of *slot) are done
         mfma_commit<kBlock, kW>(kDense, down, mc, v, xs);
+        seq.claim(slot);
         __syncthreads();
-        {
-            const uint32_t jn = j0 + rows;
-            if (tb + 1 < tb_end && jn < count)
-                mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + jn) * down - P), down, v);
-        }
+        const uint32_t tn = uniform(*slot);
+        if (tn < tiles)
+            mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + tn * rows) * down - P), down, v);
         uint64_t u[2];
         float2 ph[2];
         {
@@ -424,12 +485,15 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
 #pragma unroll
             for (uint32_t s = 0; s < kS; ++s)
                 x[s] = xa[s][st * 16 * (kDense ? 1 : kW + 1)];
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+            // two independent accumulation chains (real-part and imaginary-part
+            // steps), summed at the end: halves the chain of dependent MFMAs
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (uint32_t s = 0; s < kS; ++s) {
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].x, bv[s], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].y, bv[kS + s], acc, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].y, bv[kS + s], acc2, 0, 0, 0);
             }
+            acc += acc2;
             // all of the sub-tile's LDS reads first, then its MFMA chain: one
             // LDS latency per sub-tile instead of one per step
             __builtin_amdgcn_sched_group_barrier(0x100, kS, 0);
@@ -474,6 +538,9 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                 ph[e] = cmul(ph[e], step);
             }
         }
+        if (tn >= tiles)
+            break;  // uniform
+        tb = tn;
     }
 }
 
@@ -606,6 +673,64 @@ __device__ __forceinline__ void generic_group(const GroupsDev &a, const float2 *
     }
 }
 
+// All of this block's tiles of group g from seq.
+template <int kBlock, bool kWide>
+__device__ void run_group(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P, uint32_t g, TileSeq seq,
+                          uint32_t *slot, unsigned char *smem)
+{
+    const uint32_t *gw = a.groups + g * kGroupWords;
+    const uint32_t kind = gw[0], up = gw[1], down = gw[2], rows = gw[5];
+    const double inv_down = a.chans[gw[8]].inv_down;
+    const uint64_t n0 = cdiv(P * up, down, inv_down);
+    const uint32_t count = (uint32_t)(cdiv((P + a.n) * up, down, inv_down) - n0);
+    const uint32_t tiles = (count + rows - 1) / rows;
+    seq.tiles = tiles;
+    seq.start(blockIdx.x, slot + 1);
+    seq.claim(slot);
+    __syncthreads();
+    uint32_t tb = uniform(*slot);
+    if (tb >= tiles)
+        return;  // uniform: nothing (left) for this block
+    if (kind == kGroupMfma) {
+        if constexpr (kWide) {
+            if (gw[3] != 32) {
+                if (down < 64)
+                    mfma_stripe<kBlock, 16, true>(a, hist, P, gw, n0, count, seq, slot, smem);
+                else
+                    mfma_stripe<kBlock, 16, false>(a, hist, P, gw, n0, count, seq, slot, smem);
+                return;
+            }
+        }
+        if (down < 32)
+            mfma_stripe<kBlock, 8, true>(a, hist, P, gw, n0, count, seq, slot, smem);
+        else
+            mfma_stripe<kBlock, 8, false>(a, hist, P, gw, n0, count, seq, slot, smem);
+        return;
+    }
+    for (;;) {
+        const uint32_t j0 = tb * rows;
+        const uint32_t nt = min(rows, count - j0);
+        const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // up == 1: anchor of row 0
+        switch (kind) {
+        case kGroupVDense:
+            vdense_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
+            break;
+        case kGroupVSparse:
+            vsparse_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
+            break;
+        default:
+            generic_group<kBlock>(a, hist, P, gw, j0, nt, n0, smem);
+            break;
+        }
+        __syncthreads();  // this tile's reads of LDS (and of *slot) are done
+        seq.claim(slot);
+        __syncthreads();
+        tb = uniform(*slot);
+        if (tb >= tiles)
+            return;
+    }
+}
+
 // kWide: the plan holds 64-tap MFMA groups (their B operands and window
 // reads take 32 more registers, so plans without them get a leaner build
 // that fits 128 registers: four waves per SIMD)
@@ -634,52 +759,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWide ? 
             hw[t] = i >= 0 ? a.in[i] : hist[H + i];
         }
     }
-    // this block's stripe: tiles tb .. tb + tps - 1 of group g
-    const uint32_t g = uniform(a.table[2 * blockIdx.x]);
-    if (g == kNoGroup)
-        return;  // padding of an XCD's queue
-    uint32_t tb = uniform(a.table[2 * blockIdx.x + 1]) * a.tps[g];
-    const uint32_t tb_end = tb + a.tps[g];
-    const uint32_t *gw = a.groups + g * kGroupWords;
-    const uint32_t kind = gw[0], up = gw[1], down = gw[2], rows = gw[5];
-    const double inv_down = a.chans[gw[8]].inv_down;
-    const uint64_t n0 = cdiv(P * up, down, inv_down);
-    const uint32_t count = (uint32_t)(cdiv((P + a.n) * up, down, inv_down) - n0);
-    if (kind == kGroupMfma) {
-        if constexpr (kWide) {
-            if (gw[3] != 32) {
-                if (down < 64)
-                    mfma_stripe<kBlock, 16, true>(a, hist, P, gw, n0, count, tb, tb_end, smem);
-                else
-                    mfma_stripe<kBlock, 16, false>(a, hist, P, gw, n0, count, tb, tb_end, smem);
-                return;
-            }
-        }
-        if (down < 32)
-            mfma_stripe<kBlock, 8, true>(a, hist, P, gw, n0, count, tb, tb_end, smem);
-        else
-            mfma_stripe<kBlock, 8, false>(a, hist, P, gw, n0, count, tb, tb_end, smem);
+    __shared__ uint32_t s_tile[2];  // next tile, claim shards spent
+    const uint32_t g0 = uniform(a.table[2 * blockIdx.x]);
+    if (!a.claims) {
+        if (g0 == kNoGroup)
+            return;  // padding of an XCD's queue
+        // fixed split: the stripe s*tps .. s*tps + tps - 1, or (strided) every
+        // tps-th tile from s on
+        const uint32_t s_idx = uniform(a.table[2 * blockIdx.x + 1]);
+        TileSeq seq{nullptr, a.strided ? s_idx : s_idx * a.tps[g0], a.strided ? a.tps[g0] : 1u,
+                    a.strided ? 0xffffffffu : s_idx * a.tps[g0] + a.tps[g0]};
+        run_group<kBlock, kWide>(a, hist, P, g0, seq, s_tile, smem);
         return;
     }
-    for (const uint32_t first = tb; tb < tb_end; ++tb) {
-        const uint32_t j0 = tb * rows;
-        if (j0 >= count)
-            return;  // uniform across the block
-        const uint32_t nt = min(rows, count - j0);
-        const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // up == 1: anchor of row 0
-        if (tb != first)
-            __syncthreads();  // the previous tile's reads of LDS are done
-        switch (kind) {
-        case kGroupVDense:
-            vdense_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
-            break;
-        case kGroupVSparse:
-            vsparse_group<kBlock>(a, hist, P, gw, j0, nt, a0, smem);
-            break;
-        default:
-            generic_group<kBlock>(a, hist, P, gw, j0, nt, n0, smem);
-            break;
-        }
+    // claimed tiles: start with this block's group, then help the others;
+    // every block reaches the finished-block count below
+    for (uint32_t k = 0, g = g0 < a.n_groups ? g0 : 0; k < a.n_groups; ++k, g = g + 1 == a.n_groups ? 0 : g + 1) {
+        __syncthreads();  // the previous group's reads of LDS are done
+        run_group<kBlock, kWide>(a, hist, P, g, TileSeq{a.claims + g * kClaimShards * kClaimPitch}, s_tile,
+                                 smem);
+    }
+    // the last block out resets the counters for the next launch
+    uint32_t *const finished = a.claims + (size_t)kGroupsInline * kClaimShards * kClaimPitch;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_tile[0] = atomicAdd(finished, 1u) == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (uniform(s_tile[0])) {
+        for (uint32_t i = threadIdx.x; i < a.n_groups * kClaimShards; i += kBlock)
+            atomicExch(a.claims + i * kClaimPitch, 0u);
+        if (threadIdx.x == 0)
+            atomicExch(finished, 0u);
     }
 }
 
@@ -716,6 +828,8 @@ hipError_t launch_chan_groups(const GroupsArgs &f, hipStream_t s)
     d.groups = f.groups;
     d.n_groups = f.n_groups;
     d.table = f.table;
+    d.strided = f.strided ? 1u : 0u;
+    d.claims = f.claims;
     for (uint32_t k = 0; k < f.n_groups; ++k)
         d.tps[k] = f.tps[k];
     const uint32_t blocks = f.blocks;
@@ -743,6 +857,33 @@ hipError_t launch_chan_groups(const GroupsArgs &f, hipStream_t s)
         break;
     }
     return hipGetLastError();
+}
+
+int chan_groups_occupancy(uint32_t block, bool wide, size_t lds_bytes)
+{
+    int n = 0;
+    hipError_t e = hipSuccess;
+    switch (block * 2 + (wide ? 1 : 0)) {
+    case 256:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, chan_groups_kernel<128, false>, 128, lds_bytes);
+        break;
+    case 257:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, chan_groups_kernel<128, true>, 128, lds_bytes);
+        break;
+    case 1024:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, chan_groups_kernel<512, false>, 512, lds_bytes);
+        break;
+    case 1025:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, chan_groups_kernel<512, true>, 512, lds_bytes);
+        break;
+    case 513:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, chan_groups_kernel<256, true>, 256, lds_bytes);
+        break;
+    default:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, chan_groups_kernel<256, false>, 256, lds_bytes);
+        break;
+    }
+    return e == hipSuccess ? n : 0;
 }
 
 hipError_t launch_chan_advance(ChanPos *pos, uint32_t n, hipStream_t s)

@@ -132,6 +132,12 @@ constexpr uint32_t kGroupsInline = 256;
 // MI355X: 256 CUs in 8 XCDs, each with its own L2; consecutive workgroup ids
 // are dealt round-robin to the XCDs
 constexpr uint32_t kXcds = 8;
+// tile claim counters of the batched kernel: kClaimShards per group (shard s
+// hands out tiles s, s + kClaimShards, ...), each on its own 128-byte line,
+// then one finished-block counter
+constexpr uint32_t kClaimShards = 8;
+constexpr uint32_t kClaimPitch = 32;  // u32
+constexpr size_t kClaimWords = ((size_t)kGroupsInline * kClaimShards + 1) * kClaimPitch;
 // block-table entry of a block with no work (pads the per-XCD queues)
 constexpr uint32_t kNoGroup = 0xffffffffu;
 // taps per LDS pass of the one-channel sparse path
@@ -160,9 +166,14 @@ struct GroupsArgs {
     uint32_t parity;
     const uint32_t *table;      // device: (group, stripe) of each block, 2 x blocks u32
     uint32_t blocks;
-    const uint32_t *tps;        // host: tiles per block (one stripe) of each group
+    const uint32_t *tps;        // host: tiles per block (one stripe) of each group, or
+    bool strided;               //   (strided) the group's block count: block s runs tiles s, s+tps, ...
+    uint32_t *claims;           // device: kClaimWords zeroed words, or null: blocks then claim
+                                //   tiles one by one, starting in their table group
 };
 hipError_t launch_chan_groups(const GroupsArgs &a, hipStream_t s);
+// resident blocks of the fused kernel per CU for a block size, build and LDS size
+int chan_groups_occupancy(uint32_t block, bool wide, size_t lds_bytes);
 // advance the device position by n inputs and flip the history parity
 hipError_t launch_chan_advance(ChanPos *pos, uint32_t n, hipStream_t s);
 

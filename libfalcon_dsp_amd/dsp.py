@@ -34,6 +34,11 @@ from ._native import check, lib
 
 _vp = C.c_void_p
 
+# MultiRateChannelizer.set_tiling work splits other than a segment length
+# (FDSP_TILING_CLAIM / FDSP_TILING_STRIDED in include/fdsp.h)
+TILING_CLAIM = 0
+TILING_STRIDED = 1
+
 
 def _cf32(x) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(x, dtype=np.complex64))
@@ -418,6 +423,12 @@ class MultiRateChannelizer(_Handle):
         s, b = C.c_uint32(), C.c_uint32()
         check(lib().fdsp_channelizer_get_tiling(self.h, C.byref(s), C.byref(b)))
         return s.value, b.value
+
+    def plan_info(self, chunk_len: int) -> dict:
+        """Work split of the batched kernel for a chunk length (diagnostics)."""
+        v = [C.c_uint32() for _ in range(4)]
+        check(lib().fdsp_channelizer_plan_info(self.h, chunk_len, *[C.byref(x) for x in v]))
+        return dict(zip(("blocks", "capacity", "groups", "lds_bytes"), (x.value for x in v)))
 
     def apply(self, x) -> List[np.ndarray]:
         x = _cf32(x)

@@ -101,7 +101,8 @@ def test_grouped_channels_chunked():
     _compare(fs, _chans_grouped(fs), x, [434_780, 5, 1, 131_071, 3], True)
 
 
-@pytest.mark.parametrize("seg,block", [(256, 128), (2048, 512), (16384, 256)])
+@pytest.mark.parametrize("seg,block", [(256, 128), (2048, 512), (16384, 256), (fd.TILING_STRIDED, 256),
+                                       (fd.TILING_STRIDED, 512), (fd.TILING_CLAIM, 128), (fd.TILING_CLAIM, 512)])
 def test_grouped_tilings_agree(seg, block):
     fs = 1_000_000
     chans = _chans_grouped(fs)
@@ -198,7 +199,8 @@ def test_workload_chan_32_slice():
             assert_close_rms(ys[c], rs[c], xr, what=f"chan {c}")
 
 
-@pytest.mark.parametrize("seg,block", [(256, 128), (1024, 256), (2048, 512), (512, 512), (4096, 512)])
+@pytest.mark.parametrize("seg,block", [(256, 128), (1024, 256), (2048, 512), (512, 512), (4096, 512),
+                                       (fd.TILING_STRIDED, 128), (fd.TILING_CLAIM, 512)])
 def test_tilings_agree(seg, block):
     fs = 20_000_000
     chans = _chans_small(fs)
