@@ -243,6 +243,66 @@ int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uin
 int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t *blocks, uint32_t *capacity,
                                uint32_t *groups, uint32_t *lds_bytes);
 
+/* ---------------------------------------------------------------------------
+ * Serving: channelizer streams admitted as they arrive, their chunks batched
+ * continuously into shared launches. No counterpart in the reference, whose
+ * callers run one falcon_dsp_multi_rate_channelizer object per stream
+ * (hdr/transform/falcon_dsp_multi_rate_channelizer.h:106-110). A stream is
+ * one channel (fdsp_channel_desc, up to and including a discriminator tail)
+ * of an input stream at cfg.fs; each chunk's results equal that channel of a
+ * channelizer fed the same chunks. Its filter history and NCO state stay in
+ * device memory between chunks.
+ *
+ * Requests are asynchronous: fdsp_serve_submit queues a chunk and returns a
+ * ticket; a worker thread packs waiting chunks (one per stream per launch,
+ * streams taken in turn, at most cfg.max_batch input and output samples)
+ * into one launch while the previous launch runs. `in` and `out` must stay
+ * valid and untouched until fdsp_serve_poll reports the ticket done. Chunks
+ * of one stream are processed in submission order. All calls are
+ * thread-safe. */
+typedef struct fdsp_serve_s *fdsp_serve_t;
+typedef struct fdsp_serve_config {
+    uint32_t fs;          /* input sample rate of every stream (Hz) */
+    uint32_t max_streams; /* live streams at most */
+    uint32_t max_chunk;   /* samples per request at most */
+    uint32_t max_batch;   /* input (and output) samples per launch at most; >= max_chunk */
+    uint64_t hbm_budget;  /* device bytes for batch buffers plus stream state (0: no limit) */
+} fdsp_serve_config;
+typedef struct fdsp_serve_result {
+    size_t n_out;       /* samples written: fdsp_cf32, or float angles for a discriminator tail */
+    int status;         /* FDSP_OK, or the error the request ended with */
+    uint64_t submit_ns; /* steady-clock time of submission */
+    uint64_t done_ns;   /* steady-clock time the result was in `out` */
+    uint64_t batch;     /* launch sequence number */
+} fdsp_serve_result;
+typedef struct fdsp_serve_stats {
+    uint64_t hbm_used, hbm_budget; /* device bytes held: batch buffers + live streams */
+    uint32_t live_streams;
+    size_t queued;                 /* requests waiting or running */
+    uint64_t batches, chunks, in_samples, out_samples;
+} fdsp_serve_stats;
+
+/* Allocates the batch buffers (two staging slots) up front; FDSP_ENOMEM when
+ * they alone exceed the budget. */
+int fdsp_serve_create(const fdsp_serve_config *cfg, fdsp_serve_t *out);
+/* Finishes every queued request, then frees everything. */
+int fdsp_serve_destroy(fdsp_serve_t s);
+/* Admit a stream: FDSP_EAGAIN when max_streams are live or its state would
+ * exceed the budget (retry after a release). */
+int fdsp_serve_admit(fdsp_serve_t s, const fdsp_channel_desc *chan, uint32_t *stream);
+/* Output capacity a chunk of n samples may need (samples). */
+int fdsp_serve_max_out(fdsp_serve_t s, uint32_t stream, size_t n, size_t *count);
+/* Queue one chunk (1 <= n <= max_chunk, cap >= fdsp_serve_max_out);
+ * FDSP_EAGAIN when too many requests are outstanding. */
+int fdsp_serve_submit(fdsp_serve_t s, uint32_t stream, const fdsp_cf32 *in, size_t n, void *out, size_t cap,
+                      uint64_t *ticket);
+/* FDSP_EAGAIN while the request runs (after waiting up to wait_ms; < 0
+ * waits for it). A finished ticket is consumed and returns its status. */
+int fdsp_serve_poll(fdsp_serve_t s, uint64_t ticket, int wait_ms, fdsp_serve_result *res);
+/* End a stream and free its state; FDSP_EBUSY while it has requests. */
+int fdsp_serve_release(fdsp_serve_t s, uint32_t stream);
+int fdsp_serve_info(fdsp_serve_t s, fdsp_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,42 @@ class ChannelDesc(C.Structure):
     ]
 
 
+class ServeConfig(C.Structure):
+    """fdsp_serve_config."""
+    _fields_ = [
+        ("fs", C.c_uint32),
+        ("max_streams", C.c_uint32),
+        ("max_chunk", C.c_uint32),
+        ("max_batch", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+    ]
+
+
+class ServeResult(C.Structure):
+    """fdsp_serve_result."""
+    _fields_ = [
+        ("n_out", C.c_size_t),
+        ("status", C.c_int),
+        ("submit_ns", C.c_uint64),
+        ("done_ns", C.c_uint64),
+        ("batch", C.c_uint64),
+    ]
+
+
+class ServeStats(C.Structure):
+    """fdsp_serve_stats."""
+    _fields_ = [
+        ("hbm_used", C.c_uint64),
+        ("hbm_budget", C.c_uint64),
+        ("live_streams", C.c_uint32),
+        ("queued", C.c_size_t),
+        ("batches", C.c_uint64),
+        ("chunks", C.c_uint64),
+        ("in_samples", C.c_uint64),
+        ("out_samples", C.c_uint64),
+    ]
+
+
 _vp = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -129,6 +165,14 @@ _SIGS = [
     ("fdsp_channelizer_set_tiling", _i, [_vp, _u32, _u32]),
     ("fdsp_channelizer_get_tiling", _i, [_vp, _P(_u32), _P(_u32)]),
     ("fdsp_channelizer_plan_info", _i, [_vp, _sz, _P(_u32), _P(_u32), _P(_u32), _P(_u32)]),
+    ("fdsp_serve_create", _i, [_P(ServeConfig), _P(_vp)]),
+    ("fdsp_serve_destroy", _i, [_vp]),
+    ("fdsp_serve_admit", _i, [_vp, _P(ChannelDesc), _P(_u32)]),
+    ("fdsp_serve_max_out", _i, [_vp, _u32, _sz, _P(_sz)]),
+    ("fdsp_serve_submit", _i, [_vp, _u32, _vp, _sz, _vp, _sz, _P(_u64)]),
+    ("fdsp_serve_poll", _i, [_vp, _u64, _i, _P(ServeResult)]),
+    ("fdsp_serve_release", _i, [_vp, _u32]),
+    ("fdsp_serve_info", _i, [_vp, _P(ServeStats)]),
 ]
 
 SYMBOLS = [s[0] for s in _SIGS]

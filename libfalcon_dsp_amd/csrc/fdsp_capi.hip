@@ -19,99 +19,11 @@
 #include <vector>
 
 #include "../../include/fdsp.h"
+#include "fdsp_host.h"
 #include "fdsp_internal.h"
 
 using namespace fdsp;
 
-namespace {
-
-thread_local std::string g_err;
-
-int set_err(int code, const char *fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return set_err(FDSP_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                           __LINE__);                                                       \
-    } while (0)
-
-constexpr double kTwoPi = 6.283185307179586476925286766559;
-
-// device buffer grown on demand
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    int ensure(size_t want)
-    {
-        if (want <= bytes)
-            return FDSP_OK;
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (want == 0)
-            return FDSP_OK;
-        if (hipMalloc(&p, want) != hipSuccess)
-            return set_err(FDSP_ENOMEM, "hipMalloc(%zu) failed", want);
-        bytes = want;
-        return FDSP_OK;
-    }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
-
-// outputs a chunk of n inputs starting at global input p emits
-uint32_t chunk_count(uint64_t p, uint64_t n, uint32_t up, uint32_t down)
-{
-    return (uint32_t)(ceil_div((p + n) * up, down) - ceil_div(p * up, down));
-}
-
-// reference coeff_phase / xOffset after p inputs
-void resampler_fields(uint64_t p, uint32_t up, uint32_t down, uint32_t *phase, uint32_t *xoff, uint64_t *n_next)
-{
-    const uint64_t nn = ceil_div(p * up, down);
-    const uint64_t num = nn * down;
-    *phase = (uint32_t)(num % up);
-    *xoff = (uint32_t)(num / up - p);
-    *n_next = nn;
-}
-
-float2 phasor64(double revolutions)
-{
-    const double t = revolutions - std::floor(revolutions);
-    return make_float2((float)std::cos(kTwoPi * t), (float)std::sin(kTwoPi * t));
-}
-
-int check_chunk(size_t n, uint32_t up)
-{
-    // chunk-local index arithmetic runs in 32 bits
-    if (n >= (1ull << 31) || (uint64_t)n * up >= (1ull << 32))
-        return set_err(FDSP_ERANGE, "chunk of %zu samples is above the per-call limit (n < 2^31, n*up < 2^32)", n);
-    return FDSP_OK;
-}
-
-hipStream_t pick(hipStream_t own, fdsp_stream_t s) { return s ? static_cast<hipStream_t>(s) : own; }
-
-}  // namespace
 
 // ===========================================================================
 // misc + plumbing
@@ -338,27 +250,6 @@ extern "C" int fdsp_freq_shift_params(uint32_t fs, int32_t f_hz, uint32_t *rollo
     return FDSP_OK;
 }
 
-namespace {
-// per-shift NCO parameters shared by the freq-shift and plain channelizer paths
-NcoChan make_nco(uint32_t fs, int32_t f)
-{
-    NcoChan c{};
-    uint32_t r = 1;
-    double w = 0;
-    fdsp_freq_shift_params(fs, f, &r, &w);
-    if (f == 0) {
-        c.nu = 0.0;
-        c.r = 0xFFFFFFFFu;  // identity shift: the index never matters
-    } else {
-        c.nu = (double)f / (double)fs;
-        c.r = r;
-    }
-    c.inv_r = 1.0 / (double)c.r;
-    for (int i = 0; i < 4; ++i)
-        c.step[i] = phasor64(c.nu * i);
-    return c;
-}
-}  // namespace
 
 // ===========================================================================
 // NCO frequency shift
@@ -482,24 +373,6 @@ struct fdsp_resampler_s {
     hipStream_t stream = nullptr;
 };
 
-namespace {
-// transposed, flipped real bank: bank[ph*cpp + (cpp-1-j)] = Re h[j*up + ph]
-// (reference src/transform/falcon_dsp_polyphase_resampler.cc:131-157)
-std::vector<float> real_bank(const fdsp_cf32 *taps, uint32_t n_taps, uint32_t up, uint32_t *cpp_out)
-{
-    uint32_t padded = n_taps;
-    while (padded % up)
-        ++padded;
-    const uint32_t cpp = padded / up;
-    std::vector<float> bank((size_t)up * cpp, 0.f);
-    for (uint32_t ph = 0; ph < up; ++ph)
-        for (uint32_t j = 0; j < cpp; ++j)
-            if ((size_t)j * up + ph < n_taps)
-                bank[(size_t)ph * cpp + (cpp - 1 - j)] = taps[(size_t)j * up + ph].re;
-    *cpp_out = cpp;
-    return bank;
-}
-}  // namespace
 
 extern "C" int fdsp_resampler_create(uint32_t up, uint32_t down, const fdsp_cf32 *taps, uint32_t n_taps,
                                      fdsp_resampler_t *out)
@@ -1444,7 +1317,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             rc = set_err(FDSP_EHIP, "NCO table upload failed");
     }
     if (!rc && mode == FDSP_CHAN_BATCHED) {
-        // modulated taps: g[ph][t] = Re(bank[ph][t]) * e^{-j 2 pi nu d}, d = cpp-1-t
+        // modulated taps of every channel, back to back
         std::vector<float2> taps;
         h->stat.resize(n_chan);
         for (uint32_t c = 0; c < n_chan; ++c) {
@@ -1462,14 +1335,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             s.tail = k.tail;
             s.wrap = phasor64(s.nu * (double)s.r);
             s.out = nullptr;
-            for (uint32_t ph = 0; ph < k.up; ++ph)
-                for (uint32_t t = 0; t < k.cpp; ++t) {
-                    const double d = (double)(k.cpp - 1 - t);
-                    const double g = k.bank[(size_t)ph * k.cpp + t];
-                    const double rev = -s.nu * d;
-                    const double tt = rev - std::floor(rev);
-                    taps.push_back(make_float2((float)(g * std::cos(kTwoPi * tt)), (float)(g * std::sin(kTwoPi * tt))));
-                }
+            append_modulated_taps(k.bank, k.up, k.cpp, s.nu, &taps);
         }
         h->taps_total = (uint32_t)taps.size();
         // history: every window's reach back (MFMA windows are padded to 32/64),

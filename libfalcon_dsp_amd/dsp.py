@@ -473,6 +473,79 @@ class MultiRateChannelizer(_Handle):
 
 
 # ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+class Server(_Handle):
+    """Continuous-batching server of channelizer streams (fdsp_serve_*).
+
+    ``admit(channel)`` takes one channel dict (as MultiRateChannelizer's,
+    optionally with ``tail="discriminator"``) and returns a stream id, or
+    None when the stream limit or the device-memory budget is reached.
+    ``submit(stream, x)`` queues a chunk and returns a ticket; ``poll`` and
+    ``result`` collect it. The server keeps each request's input and output
+    arrays alive until the ticket is collected."""
+    _destroy = "fdsp_serve_destroy"
+
+    def __init__(self, fs: int, max_streams: int = 200, max_chunk: int = 1 << 18, max_batch: int = 1 << 22,
+                 hbm_budget: int = 64 << 30):
+        super().__init__()
+        cfg = N.ServeConfig(int(fs), int(max_streams), int(max_chunk), int(max_batch), int(hbm_budget))
+        check(lib().fdsp_serve_create(C.byref(cfg), C.byref(self.h)))
+        self.fs = int(fs)
+        self._tail = {}
+        self._live = {}
+
+    def admit(self, channel: dict) -> Optional[int]:
+        taps = _cf32(channel["taps"])
+        tail = 1 if channel.get("tail") in ("discriminator", N.FDSP_TAIL_DISCRIMINATOR) else 0
+        d = N.ChannelDesc(int(channel.get("output_sample_rate_in_sps", 0)), int(channel["freq_shift_in_hz"]),
+                          int(channel["up_rate"]), int(channel["down_rate"]), taps.ctypes.data, taps.size, tail)
+        sid = C.c_uint32()
+        rc = lib().fdsp_serve_admit(self.h, C.byref(d), C.byref(sid))
+        if rc == N.FDSP_EAGAIN:
+            return None
+        check(rc)
+        self._tail[sid.value] = tail
+        return sid.value
+
+    def max_out(self, stream: int, n: int) -> int:
+        v = C.c_size_t()
+        check(lib().fdsp_serve_max_out(self.h, stream, n, C.byref(v)))
+        return v.value
+
+    def submit(self, stream: int, x) -> int:
+        x = _cf32(x)
+        out = np.empty(self.max_out(stream, x.size), dtype=np.float32 if self._tail[stream] else np.complex64)
+        t = C.c_uint64()
+        check(lib().fdsp_serve_submit(self.h, stream, _ptr(x), x.size, _ptr(out), out.size, C.byref(t)))
+        self._live[t.value] = (x, out)
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: int = 0):
+        """(output array, fdsp_serve_result) when done, else None."""
+        res = N.ServeResult()
+        rc = lib().fdsp_serve_poll(self.h, ticket, int(wait_ms), C.byref(res))
+        if rc == N.FDSP_EAGAIN:
+            return None
+        entry = self._live.pop(ticket, None)  # a finished ticket is consumed, failed or not
+        check(rc)
+        return entry[1][: res.n_out], res
+
+    def result(self, ticket: int):
+        """Wait for a ticket: (output array, fdsp_serve_result)."""
+        return self.poll(ticket, -1)
+
+    def release(self, stream: int):
+        check(lib().fdsp_serve_release(self.h, stream))
+        self._tail.pop(stream, None)
+
+    def info(self) -> dict:
+        st = N.ServeStats()
+        check(lib().fdsp_serve_info(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in N.ServeStats._fields_}
+
+
+# ---------------------------------------------------------------------------
 # free functions (one-shot wrappers, as in the reference)
 # ---------------------------------------------------------------------------
 def freq_shift(in_sample_rate_in_sps: int, x, freq_shift_in_hz):

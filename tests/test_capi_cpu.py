@@ -67,6 +67,29 @@ def test_version_and_error_text():
     assert b"fs" in L.fdsp_last_error() or len(L.fdsp_last_error()) > 0
 
 
+def test_serve_argument_checks_before_any_device_work():
+    # the serving entry points validate their arguments before touching HIP
+    L = N.lib()
+    h = C.c_void_p()
+    bad = [
+        N.ServeConfig(0, 8, 1024, 4096, 0),           # fs 0
+        N.ServeConfig(25_000_000, 0, 1024, 4096, 0),  # no streams
+        N.ServeConfig(25_000_000, 8, 8192, 4096, 0),  # max_batch < max_chunk
+    ]
+    for cfg in bad:
+        assert L.fdsp_serve_create(C.byref(cfg), C.byref(h)) == N.FDSP_EINVAL
+        assert not h.value
+    big = N.ServeConfig(25_000_000, 8, 1024, 1 << 31, 0)
+    assert L.fdsp_serve_create(C.byref(big), C.byref(h)) == N.FDSP_ERANGE
+    assert L.fdsp_serve_create(None, C.byref(h)) == N.FDSP_EINVAL
+    res = N.ServeResult()
+    assert L.fdsp_serve_poll(None, 1, 0, C.byref(res)) == N.FDSP_ENOTINIT
+    assert L.fdsp_serve_release(None, 0) == N.FDSP_ENOTINIT
+    assert L.fdsp_serve_destroy(None) == N.FDSP_OK
+    # the ctypes mirrors of the header's structs
+    assert C.sizeof(N.ServeConfig) == 24 and C.sizeof(N.ServeResult) == 40 and C.sizeof(N.ServeStats) == 64
+
+
 def test_python_rate_helpers():
     import libfalcon_dsp_amd as fd
     assert fd.calculate_gcd(840, 210) == 210
