@@ -3,11 +3,13 @@
 replayed as a HIP graph) with p99 chunk latency, roofline and the reference
 CPU channelizer timed beside it.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu] [--no-plain]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu] [--no-plain] [--no-serving] [--no-others]
 
 A step is one pass of the batched channelizer over the whole 2^24-sample
-input, resident in HBM before timing starts. Prints ONE JSON line on stdout;
-progress goes to stderr.
+input, resident in HBM before timing starts. The same line carries the
+serving workload (p50/p99 request latency, aggregate Msps) and the
+resampler, multi-shift NCO and FIR-bank workloads. Prints ONE JSON line on
+stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -108,6 +110,151 @@ def measured_traffic():
         return None, None
 
 
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+
+def timed_launches(fd, stream, launch, reps):
+    """Average ms of `reps` back-to-back launches between one event pair on `stream`."""
+    launch()
+    stream.sync()
+    e0, e1 = fd.Event(), fd.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        launch()
+    e1.record(stream)
+    e1.sync()
+    return e0.elapsed_ms(e1) / reps
+
+
+def bench_other_configs(fd, stream, steps):
+    """The other workloads, each with data resident in HBM: throughput and
+    roofline fraction of one pass."""
+    from libfalcon_dsp_amd.workloads import FirBank256, Nco16, Resampler1ch
+    out = {}
+
+    # resampler_1ch: 16 chunks of 2^16 through the streaming resampler
+    w1 = Resampler1ch()
+    x1 = w1.input()
+    r = fd.PolyphaseResampler(w1.up, w1.down, w1.taps())
+    d1 = fd.DeviceBuffer.from_array(x1)
+    o1 = fd.DeviceBuffer(w1.n * w1.up // w1.down + 16)
+    cnt = [0]
+
+    def pass1():
+        # the stream continues across passes; counts differ by at most one per chunk
+        cnt[0] = 0
+        for k in range(w1.n // w1.chunk):
+            cnt[0] += r.apply_device(d1, w1.chunk, o1, stream, in_offset=k * w1.chunk)
+
+    ms1 = timed_launches(fd, stream, pass1, steps)
+    b1 = 8 * w1.n + 8 * cnt[0]
+    out["resampler_1ch"] = {
+        "value": round(cnt[0] / (ms1 / 1e3) / 1e6, 1), "unit": "output Msps", "ms_per_pass": round(ms1, 4),
+        "outputs": cnt[0], "roofline": {"bound": "hbm", "achieved": round(b1 / (ms1 / 1e3) / 1e9, 1),
+                                         "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+        "note": "16 launches of 2^16 inputs; launch-bound at this size"}
+    out["resampler_1ch"]["roofline"]["frac"] = round(out["resampler_1ch"]["roofline"]["achieved"] / HBM_PEAK_GBS, 4)
+    log(f"resampler_1ch: {ms1:.3f} ms per 16-chunk pass, {out['resampler_1ch']['value']} Msps")
+    del d1, o1, r
+
+    # nco_16shift: 64 chunks of 2^18, 16 shifted copies each
+    w2 = Nco16()
+    x2 = w2.input()
+    f = fd.FreqShift(w2.fs, w2.shifts())
+    d2 = fd.DeviceBuffer.from_array(x2)
+    o2 = fd.DeviceBuffer(16 * w2.chunk)
+
+    def pass2():
+        for k in range(w2.n // w2.chunk):
+            f.apply_device(d2, w2.chunk, o2, w2.chunk, stream, in_offset=k * w2.chunk)
+
+    ms2 = timed_launches(fd, stream, pass2, max(2, steps // 4))
+    b2 = 8 * w2.n + 8 * 16 * w2.n
+    out["nco_16shift"] = {
+        "value": round(16 * w2.n / (ms2 / 1e3) / 1e6, 1), "unit": "output Msps", "ms_per_pass": round(ms2, 4),
+        "roofline": {"bound": "hbm", "achieved": round(b2 / (ms2 / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(b2 / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "note": "2^24 inputs in 64 chunks of 2^18, 16 shifts, 136 B per input sample"}
+    log(f"nco_16shift: {ms2:.3f} ms per pass, {out['nco_16shift']['roofline']['achieved']} GB/s")
+    del d2, o2, f, x2
+
+    # fir_bank_256: 256 channels x 2^20, 512 real taps
+    w4 = FirBank256()
+    taps = w4.taps()
+    fb = fd.FirBank(taps)
+    one = w4.input(0)
+    d4 = fd.DeviceBuffer(w4.n_chan * w4.n)
+    for c in range(w4.n_chan):
+        d4.upload(one, offset=c * w4.n)  # the same samples in every channel (timing does not depend on them)
+    o4 = fd.DeviceBuffer(w4.n_chan * w4.n)
+    ms4 = timed_launches(fd, stream, lambda: fb.apply_device(d4, w4.n, w4.n, o4, w4.n, stream),
+                         max(2, steps // 4))
+    flops = 4.0 * w4.n_taps * w4.n * w4.n_chan
+    tf = flops / (ms4 / 1e3) / 1e12
+    out["fir_bank_256"] = {
+        "value": round(tf, 2), "unit": "TFLOP/s", "ms_per_pass": round(ms4, 4),
+        "roofline": {"bound": "fp32 valu", "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(tf / 157.3, 4)},
+        "note": "4 flops per real tap per complex sample; input is channel 0's samples in all 256 channels"}
+    log(f"fir_bank_256: {ms4:.3f} ms per pass, {tf:.1f} TFLOP/s")
+    del d4, o4, fb
+    return out
+
+
+def bench_serving(fd):
+    """Serving workload: Poisson arrivals (2000 req/s) of single-channel
+    streams at 25 Msps, chunks U[4k, 256k], a quarter FM streams with a
+    discriminator tail. Two runs: paced at the arrival times (latency at the
+    offered load), then all requests at once (capacity)."""
+    from libfalcon_dsp_amd.signals import cwgn
+    from libfalcon_dsp_amd.workloads import Serving
+    w = Serving()
+    reqs = w.requests()
+    pool = cwgn(w.seed, 1 << 23)
+
+    def run(paced):
+        srv = fd.Server(w.fs, max_streams=w.max_streams, max_chunk=w.chunk_hi, max_batch=1 << 22,
+                        hbm_budget=w.hbm_budget)
+        sids, tickets = {}, []
+        t0 = time.perf_counter()
+        for i, rq in enumerate(reqs):
+            if paced:
+                while True:
+                    dt = t0 + rq.t_arrival - time.perf_counter()
+                    if dt <= 0:
+                        break
+                    if dt > 2e-3:
+                        time.sleep(dt - 1e-3)
+            if rq.stream not in sids:
+                sids[rq.stream] = srv.admit(w.stream_channel(rq.stream))
+            off = (i * 2654435761) % (pool.size - rq.chunk)
+            tickets.append((srv.submit(sids[rq.stream], pool[off:off + rq.chunk]), rq.chunk))
+        res = [srv.result(t)[1] for t, _ in tickets]
+        info = srv.info()
+        srv.close()
+        lat = [(r.done_ns - r.submit_ns) / 1e6 for r in res]
+        span = (max(r.done_ns for r in res) - min(r.submit_ns for r in res)) / 1e9
+        n_in = sum(m for _, m in tickets)
+        n_out = sum(r.n_out for r in res)
+        return {"p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
+                "in_msps": round(n_in / span / 1e6, 1), "out_msps": round(n_out / span / 1e6, 2),
+                "seconds": round(span, 3), "batches": info["batches"],
+                "chunks_per_batch": round(info["chunks"] / max(1, info["batches"]), 2),
+                "streams": len(sids)}
+
+    paced = run(True)
+    log(f"serving (paced): p50 {paced['p50_ms']} ms, p99 {paced['p99_ms']} ms, {paced['in_msps']} Msps in")
+    sat = run(False)
+    log(f"serving (all at once): p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms, {sat['in_msps']} Msps in")
+    return {"workload": f"{len(reqs)} requests, Poisson {w.rate_hz:g}/s over {w.max_streams} streams at "
+                        f"{w.fs / 1e6:g} Msps, chunks U[{w.chunk_lo}, {w.chunk_hi}] from host memory, "
+                        f"{w.fm_share:g} of streams FM with a discriminator tail, budget {w.hbm_budget >> 30} GiB",
+            "latency": "submit to result in the caller's host buffer (server clock)",
+            "paced": paced, "saturated": sat}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,8 +264,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-channel plain GPU form")
     ap.add_argument("--latency-chunk", type=int, default=1 << 18)
     ap.add_argument("--latency-chunks", type=int, default=200)
-    ap.add_argument("--seg", type=int, default=0, help="input samples per block of the fused kernel (0 = default)")
-    ap.add_argument("--block", type=int, default=0, help="fused-kernel block threads (0 = default)")
+    ap.add_argument("--seg", type=int, default=0,
+                    help="fused-kernel work split (1 strided, or a segment length; 0: autotune)")
+    ap.add_argument("--block", type=int, default=0, help="fused-kernel block threads (0: autotune)")
+    ap.add_argument("--no-others", action="store_true", help="skip the resampler, NCO and FIR-bank workloads")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device-to-device copies of 2^24 complex samples (128 MiB), the known-size "
                          "copy that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -152,12 +302,16 @@ def main():
     n = x.size
 
     g = fd.MultiRateChannelizer(w.fs, chans, batched=True)
+    stream = fd.Stream()
+    d_in = fd.DeviceBuffer.from_array(x)
+    trials = None
     if args.seg or args.block:
         s0, b0 = g.get_tiling()
         g.set_tiling(args.seg or s0, args.block or b0)
+    else:
+        trials = g.autotune(d_in, n, reps=5)
+        log("autotune: " + ", ".join(f"{t['segment_len']}/{t['block_threads']}: {t['us']} us" for t in trials))
     seg, block = g.get_tiling()
-    stream = fd.Stream()
-    d_in = fd.DeviceBuffer.from_array(x)
     outs = [fd.DeviceBuffer(m) for m in g.max_out(n)]
     g.graph_create(n, d_in, outs)
     fd.device_sync()
@@ -249,6 +403,9 @@ def main():
         log(f"plain per-channel GPU form: {plain['value']} Msps ({plain['ms_per_step']} ms/step)")
         del gp, pouts
 
+    others = None if args.no_others else bench_other_configs(fd, stream, args.steps)
+    serving = None if args.no_serving else bench_serving(fd)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference channelizer on the host cores)")
@@ -273,7 +430,7 @@ def main():
                                "{12.5k,25k,100k,250k,1M,2.5M} sps, 32 Kaiser taps, one fused launch per step "
                                "in a HIP graph",
                    "input_samples": n, "channels": len(chans), "outputs_per_step": int(round(total_out / args.steps)),
-                   "segment_len": seg, "block_threads": block},
+                   "segment_len": seg, "block_threads": block, "autotune_trials": trials},
         "p50_chunk_latency_ms": round(p50, 5),
         "p99_chunk_latency_ms": round(p99, 5),
         "latency_chunk_samples": lc,
@@ -285,6 +442,8 @@ def main():
         "plain_gpu": plain,
         "batched_vs_plain": round(value / plain["value"], 1) if plain else None,
         "cpu_baseline": cpu,
+        "serving": serving,
+        "other_workloads": others,
     }
     print(json.dumps(rec), flush=True)
 

@@ -1806,6 +1806,96 @@ extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len
     return FDSP_OK;
 }
 
+extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len,
+                                         uint32_t reps, fdsp_tiling_trial *trials, size_t cap, size_t *n_trials)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "autotuning applies to the batched mode");
+    if (!d_in || chunk_len == 0 || reps == 0)
+        return set_err(FDSP_EINVAL, "autotune: need a device chunk and reps > 0");
+    if (h->graph_exec)
+        return set_err(FDSP_EBUSY, "destroy the graph before tuning");
+    uint32_t max_up = 1;
+    for (auto &k : h->ch)
+        max_up = std::max(max_up, k.up);
+    if (int rc = check_chunk(chunk_len, max_up))
+        return rc;
+    const uint32_t n = (uint32_t)chunk_len;
+    // scratch outputs: the stream position (p, parity) is passed to every
+    // launch unchanged and never advanced, so the tuning launches leave the
+    // channelizer's state as it was
+    const size_t C = h->ch.size();
+    std::vector<DevBuf> outs(C);
+    std::vector<void *> ptrs(C);
+    for (size_t c = 0; c < C; ++c) {
+        ChanHost &k = h->ch[c];
+        const size_t most = (size_t)chunk_len * k.up / k.down + 1;
+        DevBuf &dst = k.tail ? k.d_scratch : outs[c];
+        if (int rc = dst.ensure(most * sizeof(float2)))
+            return rc;
+        ptrs[c] = dst.p;
+    }
+    if (int rc = chan_upload_outs(h, ptrs.data()))
+        return rc;
+    static const uint32_t kCand[][2] = {
+        {FDSP_TILING_STRIDED, 256}, {FDSP_TILING_STRIDED, 512}, {FDSP_TILING_STRIDED, 128},
+        {FDSP_TILING_CLAIM, 256},   {FDSP_TILING_CLAIM, 512},   {FDSP_TILING_CLAIM, 128},
+        {4096, 256},                {16384, 256},               {65536, 256},
+    };
+    const uint32_t seg0 = h->seg_len, block0 = h->block;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_err(FDSP_EHIP, "event creation failed");
+    }
+    int rc = FDSP_OK;
+    float best = -1.f;
+    uint32_t best_seg = seg0, best_block = block0;
+    size_t k = 0;
+    for (const auto &cand : kCand) {
+        float us = -1.f;
+        if (fdsp_channelizer_set_tiling(h, cand[0], cand[1]) == FDSP_OK) {
+            GroupsArgs f;
+            rc = groups_args(h, d_in, n, false, &f);
+            // one untimed launch (table upload, caches), then reps back to back
+            if (!rc && launch_chan_groups(f, h->stream) != hipSuccess)
+                rc = set_err(FDSP_EHIP, "autotune launch failed");
+            if (!rc && hipEventRecord(e0, h->stream) != hipSuccess)
+                rc = set_err(FDSP_EHIP, "event record failed");
+            for (uint32_t r = 0; r < reps && !rc; ++r)
+                if (launch_chan_groups(f, h->stream) != hipSuccess)
+                    rc = set_err(FDSP_EHIP, "autotune launch failed");
+            float ms = 0.f;
+            if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = set_err(FDSP_EHIP, "autotune timing failed");
+            if (rc)
+                break;
+            us = 1e3f * ms / (float)reps;
+            if (best < 0.f || us < best) {
+                best = us;
+                best_seg = cand[0];
+                best_block = cand[1];
+            }
+        }
+        if (trials && k < cap)
+            trials[k] = fdsp_tiling_trial{cand[0], cand[1], us};
+        ++k;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the device table points at the scratch outputs: the next call re-uploads its own
+    h->stat_valid = false;
+    const int rc2 = fdsp_channelizer_set_tiling(h, rc ? seg0 : best_seg, rc ? block0 : best_block);
+    if (n_trials)
+        *n_trials = k;
+    return rc ? rc : rc2;
+}
+
 extern "C" int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads)
 {
     if (!h)

@@ -424,6 +424,16 @@ class MultiRateChannelizer(_Handle):
         check(lib().fdsp_channelizer_get_tiling(self.h, C.byref(s), C.byref(b)))
         return s.value, b.value
 
+    def autotune(self, d_in: DeviceBuffer, chunk_len: int, reps: int = 10, in_offset: int = 0) -> List[dict]:
+        """Time every candidate work split on a device chunk and keep the
+        fastest (fdsp_channelizer_autotune); returns the trials."""
+        cap = 32
+        tr = (N.TilingTrial * cap)()
+        k = C.c_size_t()
+        check(lib().fdsp_channelizer_autotune(self.h, d_in.at(in_offset), chunk_len, reps, tr, cap, C.byref(k)))
+        return [{"segment_len": t.segment_len, "block_threads": t.block_threads, "us": round(t.us, 2)}
+                for t in tr[: k.value]]
+
     def plan_info(self, chunk_len: int) -> dict:
         """Work split of the batched kernel for a chunk length (diagnostics)."""
         v = [C.c_uint32() for _ in range(4)]

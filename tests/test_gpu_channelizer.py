@@ -199,6 +199,28 @@ def test_workload_chan_32_slice():
             assert_close_rms(ys[c], rs[c], xr, what=f"chan {c}")
 
 
+def test_autotune_keeps_stream_state():
+    # tuning launches run on a chunk of the caller's data without moving the
+    # stream: the chunks around it still continue the oracle's stream
+    fs = 1_000_000
+    chans = _chans_grouped(fs)
+    x = cwgn(23, 300_000)
+    g = fd.MultiRateChannelizer(fs, chans)
+    ref = O.Channelizer(fs, chans)
+    for lo, hi in [(0, 100_000), (100_000, 300_000)]:
+        ys, rs = g.apply(x[lo:hi]), ref.apply(x[lo:hi])
+        for c in range(len(chans)):
+            assert ys[c].size == rs[c].size
+            assert_close_rms(ys[c], rs[c], rms(x), what=f"chan {c} [{lo},{hi})")
+        if lo == 0:
+            d = fd.DeviceBuffer.from_array(x[:131_072])
+            trials = g.autotune(d, 131_072, reps=3)
+            timed = [t for t in trials if t["us"] > 0]
+            assert len(trials) == 9 and timed
+            best = min(timed, key=lambda t: t["us"])
+            assert g.get_tiling() == (best["segment_len"], best["block_threads"])
+
+
 @pytest.mark.parametrize("seg,block", [(256, 128), (1024, 256), (2048, 512), (512, 512), (4096, 512),
                                        (fd.TILING_STRIDED, 128), (fd.TILING_CLAIM, 512)])
 def test_tilings_agree(seg, block):
