@@ -13,6 +13,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -866,12 +867,14 @@ struct fdsp_channelizer_s {
     bool pos_dirty = false;         // device position behind the host's (eager calls since the last graph launch)
     uint32_t taps_total = 0, hist_len = 0;
     uint32_t seg_len = FDSP_TILING_STRIDED, block = 256;
+    bool bf16_planes = true;  // MFMA groups with 8-aligned windows on bf16 matrix cores (FDSP_MFMA_FP32=1: off)
     uint32_t capacity = 1024;  // fused-kernel blocks resident at once (occupancy x CUs)
     // work plan: channel groups (kGroupWords u32 each) and their MFMA B tables
     std::vector<uint32_t> groups;
     std::vector<float> btab;
     DevBuf d_groups, d_btab;
     DevBuf d_claims;  // per-group tile claim counters + a finished-block counter (zero between launches)
+    DevBuf d_sink;    // stores of rows and lanes with no output
     uint32_t n_groups = 0, lds_bytes = 0;
     std::vector<uint32_t> tps;
     // block tables by chunk length (eager calls), and the graph's own
@@ -934,6 +937,7 @@ size_t group_lds(uint32_t kind, uint32_t up, uint32_t down, uint32_t w, uint32_t
         // transposed by down (mc columns, rounded up to odd), sparse windows
         // at a pitch of w + 1
         if (down < w) {
+            // (bf16 planes: 4 planes of cap samples of 2 bytes, never more)
             const size_t cap = (size_t)mfma_stage(block) * block;
             return (kMfmaMaxChan * w + down * (((cap + down - 1) / down) | 1u)) * sizeof(float2);
         }
@@ -969,7 +973,9 @@ int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t *lds)
             switch (kind) {
             case kGroupMfma: {
                 const uint32_t stage = mfma_stage(block) * block;
-                rows = down < w ? ((stage - w) / down + 1) : stage / w;
+                // bf16 planes start up to 7 samples before the first window
+                const uint32_t reach = mfma_bf16_planes(w, down) ? w + 8 : w;
+                rows = down < w ? ((stage - reach) / down + 1) : stage / w;
                 step = lo = 16;
                 rows = std::max(lo, rows / step * step);
                 break;
@@ -1019,6 +1025,22 @@ int update_capacity(fdsp_channelizer_s *h)
 // window (32 or 64) form MFMA groups of up to kMfmaMaxChan when there are at
 // least kMfmaMinChan of them; every other channel is a one-channel VALU group.
 // `taps` holds every channel's modulated taps (ChanStatic::taps_off).
+// round-to-nearest-even bf16 of a finite float, and back
+uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf16_to_float(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, sizeof(f));
+    return f;
+}
+
 void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
 {
     const uint32_t C = (uint32_t)h->ch.size();
@@ -1052,6 +1074,42 @@ void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
                     }
                     h->btab.push_back(v);
                 }
+            if (mfma_bf16_planes(w, k.down) && h->bf16_planes) {
+                // bf16 form: for each alignment base b (the windows start b
+                // samples past an 8-aligned slot), K step q and lane l, the 8
+                // values B[kk = 32q + 8(l/16) + e][j = l mod 16], e = 0..7, as
+                // hi and lo bf16 parts. K slot blocks 0-4 are the real parts of
+                // window slots 0..39, blocks 5-9 the imaginary parts, 10-11 zero;
+                // slot s holds window tap s - b.
+                gw[7] = (uint32_t)h->btab.size() + 1;
+                for (uint32_t base = 0; base < 8; ++base)
+                    for (uint32_t q = 0; q < 3; ++q)
+                        for (uint32_t part = 0; part < 2; ++part)
+                            for (uint32_t l = 0; l < 64; ++l)
+                                for (uint32_t wd = 0; wd < 4; ++wd) {
+                                    uint32_t packed = 0;
+                                    for (uint32_t hlf = 0; hlf < 2; ++hlf) {
+                                        const uint32_t kb = 4 * q + l / 16, e = 2 * wd + hlf, j = l % 16, c = j / 2;
+                                        float v = 0.f;
+                                        if (kb < 10 && c < members.size()) {
+                                            const bool re = kb < 5;
+                                            const int t = (int)(8 * (re ? kb : kb - 5) + e) - (int)base;
+                                            const ChanStatic &st = h->stat[members[c]];
+                                            const int pad = (int)(w - st.cpp);
+                                            if (t >= pad && t < (int)w) {
+                                                const float2 g = taps[st.taps_off + t - pad];
+                                                v = re ? ((j & 1) ? g.y : g.x) : ((j & 1) ? g.x : -g.y);
+                                            }
+                                        }
+                                        const uint16_t hi = bf16_rne(v);
+                                        const uint16_t bits = part == 0 ? hi : bf16_rne(v - bf16_to_float(hi));
+                                        packed |= (uint32_t)bits << (16 * hlf);
+                                    }
+                                    float word;
+                                    std::memcpy(&word, &packed, sizeof(word));
+                                    h->btab.push_back(word);
+                                }
+            }
         }
         h->groups.insert(h->groups.end(), gw, gw + kGroupWords);
     };
@@ -1250,6 +1308,11 @@ int groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool d
     f.tps = h->tps.data();
     f.strided = h->seg_len <= FDSP_TILING_STRIDED;
     f.claims = h->seg_len == FDSP_TILING_CLAIM ? h->d_claims.as<uint32_t>() : nullptr;
+    f.sink = h->d_sink.as<float2>();
+    f.out16 = true;
+    for (const ChanStatic &st : h->stat)
+        f.out16 = f.out16 && reinterpret_cast<uintptr_t>(st.out) % 16 == 0;
+    f.in16 = reinterpret_cast<uintptr_t>(d_in) % 16 == 0;
     *out = f;
     return FDSP_OK;
 }
@@ -1343,8 +1406,14 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
         uint32_t max_w = max_cpp;
         for (auto &k : h->ch)
             max_w = std::max(max_w, mfma_window(k));
-        h->hist_len = max_w & ~1u;
+        if (const char *e = std::getenv("FDSP_MFMA_FP32"))
+            h->bf16_planes = e[0] == '0';
         build_groups(h, taps);
+        // bf16-plane tiles start up to 7 samples before their first window
+        for (uint32_t g = 0; g < h->n_groups; ++g)
+            if (h->groups[g * kGroupWords + 7])
+                max_w = std::max(max_w, h->groups[g * kGroupWords + 3] + 8);
+        h->hist_len = (max_w + 1) & ~1u;
         if (h->n_groups > kGroupsInline)
             rc = set_err(FDSP_ERANGE, "batched channelizer: %u channel groups exceed %u; use FDSP_CHAN_PLAIN",
                          h->n_groups, kGroupsInline);
@@ -1374,6 +1443,8 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             rc = h->d_pos.ensure(sizeof(ChanPos));
         if (!rc)
             rc = h->d_claims.ensure(kClaimWords * sizeof(uint32_t));
+        if (!rc)
+            rc = h->d_sink.ensure(kSinkWords * sizeof(float2));
         if (!rc && (hipMemcpy(h->d_taps.p, taps.data(), taps.size() * sizeof(float2), hipMemcpyHostToDevice) !=
                         hipSuccess ||
                     hipMemset(h->d_claims.p, 0, kClaimWords * sizeof(uint32_t)) != hipSuccess ||
@@ -1433,6 +1504,7 @@ extern "C" int fdsp_channelizer_destroy(fdsp_channelizer_t h)
     h->d_hist[1].release();
     h->d_pos.release();
     h->d_claims.release();
+    h->d_sink.release();
     h->d_groups.release();
     h->d_btab.release();
     h->d_table_graph.release();
@@ -1705,6 +1777,9 @@ extern "C" int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_
     if (int rc = groups_args(h, d_in, (uint32_t)chunk_len, true, &f))
         return rc;
     f.chans = h->d_stat_graph.as<ChanStatic>();
+    f.out16 = true;
+    for (const ChanStatic &st : gstat)
+        f.out16 = f.out16 && reinterpret_cast<uintptr_t>(st.out) % 16 == 0;
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = launch_chan_groups(f, h->stream);
     hipError_t e2 = launch_chan_advance(h->d_pos.as<ChanPos>(), (uint32_t)chunk_len, h->stream);

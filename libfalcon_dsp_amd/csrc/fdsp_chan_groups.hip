@@ -35,6 +35,8 @@
 //            per-output filter phase.
 // Every staging step issues its loads as one batch (a handful per thread)
 // before writing LDS, so memory latency overlaps across the batch.
+#include <type_traits>
+
 #include "fdsp_internal.h"
 
 namespace fdsp {
@@ -47,6 +49,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // the LDS wait counter and stall every following LDS read until they land.
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) fvec2 *gvec2_ptr;
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) fvec4 *gvec4_ptr;
 __device__ __forceinline__ void store_out(void *base, uint32_t j, float2 v)
 {
     ((gvec2_ptr)base)[j] = fvec2{v.x, v.y};
@@ -69,6 +73,9 @@ struct GroupsDev {
     const uint32_t *table;  // (group, stripe) of each block
     uint32_t strided;       // 0: tps consecutive tiles per block; 1: every tps-th tile
     uint32_t *claims;       // non-null: blocks claim tiles from per-group counters
+    float2 *sink;           // kSinkWords samples that absorb the stores of rows and lanes with no output
+    uint32_t out16;         // every channel's output is 16-byte aligned
+    uint32_t in16;          // the chunk's input is 16-byte aligned
     uint32_t tps[kGroupsInline];  // tiles per stripe, or blocks, of each group
 };
 
@@ -321,10 +328,14 @@ __device__ __forceinline__ void mfma_load(const GroupsDev &a, const float2 *__re
     // of the tile loop into registers that would stay live across it.
     asm volatile("" : "+v"(e0));
     const uint32_t stride = dense ? kBlock * 8u : kRowsPer * down * 8u;
-    if (lo >= 0)
+    if (lo >= 0) {
         v[0] = buffer_sample(r, e0 * 8u, 0);
-    else
-        v[0] = sample(a, hist, lo + (int64_t)e0);
+    } else {
+        // from the history, or from the chunk through a descriptor bounded
+        // by the chunk (short chunks end before the tile's span does)
+        const int64_t i = lo + (int64_t)e0;
+        v[0] = i < 0 ? hist[a.hist_len + i] : buffer_sample(buffer_of(a.in, a.n), (uint32_t)i * 8u, 0);
+    }
     // the whole offset goes in the VGPR operand, which the range check covers
 #pragma unroll
     for (uint32_t s = 1; s < kStage; ++s)
@@ -370,16 +381,93 @@ __device__ __forceinline__ void mfma_commit(bool dense, uint32_t down, uint32_t 
     }
 }
 
+// bf16 planes of a staged tile: the span's real and imaginary parts, each as
+// a hi and a lo bf16 part (x = hi + lo to about 2^-16), in natural order from
+// an 8-aligned start s0, planes [re hi | im hi | re lo | im lo] of kPlane
+// samples each. A thread stages pairs of samples (16-byte loads): pair
+// p = e*kBlock + tid holds samples s0 + 2p, s0 + 2p + 1.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bf16_split(float a, float b, uint32_t &hi, uint32_t &lo)
+{
+    const bf16x2 h = __builtin_convertvector(fvec2{a, b}, bf16x2);
+    const bf16x2 l = __builtin_convertvector(fvec2{a - (float)h[0], b - (float)h[1]}, bf16x2);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+__device__ __forceinline__ float bf16_word_half(uint32_t w, uint32_t i)
+{
+    return __uint_as_float(i ? (w & 0xffff0000u) : (w << 16));
+}
+
+template <int kBlock, uint32_t kPairs = mfma_stage(kBlock) / 2>
+__device__ __forceinline__ void bf_load(const GroupsDev &a, const float2 *__restrict__ hist, int64_t s0,
+                                        u32x4 (&v)[kPairs])
+{
+    const __amdgpu_buffer_rsrc_t r = buffer_of(a.in + s0, (int64_t)a.n - s0);
+    uint32_t p0 = threadIdx.x;
+    asm volatile("" : "+v"(p0));
+    if (s0 >= 0) {
+        v[0] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(p0 * 16u), 0, 0));
+    } else {
+        // s0 and the history length are even, so a pair never straddles the
+        // chunk start; only pair 0 can precede it (s0 >= -38 > -2*kBlock)
+        const int64_t i = s0 + 2 * (int64_t)p0;
+        if (i < 0)
+            v[0] = *reinterpret_cast<const u32x4 *>(hist + (a.hist_len + i));
+        else  // bounded by the chunk, which may end before the span does
+            v[0] = __builtin_bit_cast(
+                u32x4, __builtin_amdgcn_raw_buffer_load_b128(buffer_of(a.in, a.n), (int)((uint32_t)i * 8u), 0, 0));
+    }
+#pragma unroll
+    for (uint32_t e = 1; e < kPairs; ++e)
+        v[e] = __builtin_bit_cast(u32x4,
+                                  __builtin_amdgcn_raw_buffer_load_b128(r, (int)(p0 * 16u + e * kBlock * 16u), 0, 0));
+}
+
+template <int kBlock, uint32_t kPairs = mfma_stage(kBlock) / 2>
+__device__ __forceinline__ void bf_commit(const u32x4 (&v)[kPairs], uint32_t *planes)
+{
+    constexpr uint32_t kWords = kPairs * kBlock;  // words per plane
+    uint32_t p0 = threadIdx.x;
+    asm volatile("" : "+v"(p0));
+#pragma unroll
+    for (uint32_t e = 0; e < kPairs; ++e) {
+        uint32_t rh, rl, ih, il;
+        bf16_split(__uint_as_float(v[e][0]), __uint_as_float(v[e][2]), rh, rl);
+        bf16_split(__uint_as_float(v[e][1]), __uint_as_float(v[e][3]), ih, il);
+        const uint32_t p = p0 + e * kBlock;
+        planes[p] = rh;
+        planes[kWords + p] = ih;
+        planes[2 * kWords + p] = rl;
+        planes[3 * kWords + p] = il;
+    }
+}
+
+// sample i of a bf16-plane tile
+template <uint32_t kWords>
+__device__ __forceinline__ float2 bf_sample(const uint32_t *planes, uint32_t i)
+{
+    const uint32_t w = i >> 1, h = i & 1;
+    return make_float2(bf16_word_half(planes[w], h) + bf16_word_half(planes[2 * kWords + w], h),
+                       bf16_word_half(planes[kWords + w], h) + bf16_word_half(planes[3 * kWords + w], h));
+}
+
 // The tiles of an MFMA group that seq hands this block, the first one already
 // in *slot (tile i: rows i*rows .. of the chunk's `count` outputs, first
 // output n0 of the stream). The next tile is claimed while the current one
 // is staged, so its loads overlap the current tile's MFMA work.
-template <int kBlock, uint32_t kS, bool kDense>
+template <int kBlock, uint32_t kS, bool kDense, bool kBf = false>
 __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P,
                                             const uint32_t *gw, uint64_t n0, uint32_t count, TileSeq seq,
                                             uint32_t *slot, unsigned char *smem)
 {
+    static_assert(!kBf || (kS == 8 && kDense), "bf16 planes: 32-tap dense windows");
     constexpr uint32_t kW = 4 * kS;  // padded taps per window (zero taps in front)
+    constexpr uint32_t kPairs = mfma_stage(kBlock) / 2;
+    constexpr uint32_t kWords = kPairs * kBlock;  // bf16 plane length in words
     constexpr uint32_t kWaves = kBlock / 64;
     const uint32_t down = gw[2], m = gw[4], rows = gw[5];
     // LDS: the group's modulated taps [channel][w] (zero-padded in front, as
@@ -397,12 +485,35 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
 
     // B operands: step s holds B[k = 4s + lane/16][j = lane%16]; steps 0..kS-1
     // cover the real parts of the window, kS..2kS-1 the imaginary parts
-    float bv[2 * kS];
-    {
+    float bv[kBf ? 1 : 2 * kS];
+    bf16x8 bh[3], bl[3];  // bf16 planes: B of K step q, hi and lo parts
+    // bf16 planes: the windows start `base` samples past an 8-aligned slot,
+    // the same for every tile (tile anchors differ by multiples of down)
+    const uint32_t base = (uint32_t)((int64_t)((n0 + uniform(*slot) * rows) * down - P) - (int64_t)(kW - 1)) & 7u;
+    if constexpr (kBf) {
+        const u32x4 *bt = reinterpret_cast<const u32x4 *>(a.btab + (gw[7] - 1) + base * kBfWordsPerBase);
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            bh[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 0) * 64 + lane]);
+            bl[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 1) * 64 + lane]);
+        }
+    } else {
         const float *__restrict__ bt = a.btab + gw[6];
 #pragma unroll
         for (uint32_t s = 0; s < 2 * kS; ++s)
             bv[s] = bt[s * 64 + lane];
+    }
+    uint32_t *planes = reinterpret_cast<uint32_t *>(xs);
+    // bf16 A operand of K step q: row lane%16, K block kb = 4q + lane/16: the
+    // real (kb < 5) or imaginary (kb 5..9) plane from slot 8*(kb mod 5); the
+    // zero-B blocks 10, 11 read the real plane's slot 0 (finite)
+    uint32_t aoff[3];
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        const uint32_t kb = 4 * q + (lane >> 4);
+        const uint32_t plane = kb >= 5 && kb < 10 ? 1u : 0u;
+        const uint32_t slot = kb < 5 ? 8 * kb : kb < 10 ? 8 * (kb - 5) : 0u;
+        aoff[q] = plane * kWords * 4 + (8 * (lane & 15) + slot) * 2;  // bytes, hi part
     }
     // A operand of step s: row r = st*16 + lane%16, window tap 4s + lane/16,
     // read at xa[s] + 16*st
@@ -441,103 +552,192 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
     const uint32_t cpp1 = ch->cpp - 1;
     const double nu = ch->nu, inv_r = ch->inv_r;
     const float2 wrap = ch->wrap;
-    const float2 unwrap = make_float2(wrap.x, -wrap.y);
     gvec2_ptr out = (gvec2_ptr)ch->out;
+    // rows and lanes with no output store into this block's part of the sink
+    // (16 bytes per thread)
+    gvec2_ptr sink = (gvec2_ptr)(a.sink + 2 * ((blockIdx.x % (kSinkWords / (2 * kBlock))) * kBlock + threadIdx.x));
+    const bool out16 = a.out16 != 0;
     const float2 *my_taps = taps_l + cl * kW;
     const uint32_t ro = 4 * (lane >> 4) + (odd ? 2 : 0);
-    // NCO bookkeeping per output row: u = k(row 16*wave of the tile) + (row -
-    // 16*wave)*down, unwrapped (a tile spans fewer than R inputs, so at most
-    // one rollover: u < 2R); the phasor runs on as e^{j2pi nu u} and picks up
-    // e^{-j2pi nu R} past the rollover.
-    const uint32_t du = 16 * kWaves * down;  // from one of the wave's sub-tiles to the next
-    const float2 step = nco_phasor((int64_t)du, nu);
+    // NCO index k of the lane's first row (ro) in its current sub-tile, in
+    // [0, R), with the phasor e^{j2pi nu k}; the lane's second row is one
+    // row (down inputs) later and the wave's next sub-tile du inputs later.
+    // While cpp-1 <= k < R - du - down neither row's window holds a rollover
+    // and neither index nor the next sub-tile's wraps: the fast path steps
+    // the phasor by recurrence. Otherwise the wave takes the exact path.
+    const uint32_t du = 16 * kWaves * down;
+    const float2 s_row = nco_phasor((int64_t)down, nu);
+    const float2 s_sub = nco_phasor((int64_t)du, nu);
+    const uint32_t k_hi = R > (uint64_t)du + down ? (uint32_t)(R - du - down) : 0u;
+
+    // A operands and MFMA chains of one sub-tile: two independent chains (the
+    // real-part and the imaginary-part steps), summed by the consumer
+    auto mma = [&](uint32_t st, floatx4 &c0, floatx4 &c1) {
+        if constexpr (kBf) {
+            const unsigned char *pb = reinterpret_cast<const unsigned char *>(planes) + st * 16 * 16;
+            bf16x8 ah[3], al[3];
+#pragma unroll
+            for (uint32_t q = 0; q < 3; ++q) {
+                ah[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q]);
+                al[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q] + 2 * kWords * 4);
+            }
+            c0 = floatx4{0.f, 0.f, 0.f, 0.f};
+            c1 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (uint32_t q = 0; q < 3; ++q) {
+                c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh[q], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[q], c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[q], c1, 0, 0, 0);
+            }
+            return;
+        }
+        float2 x[kS];
+#pragma unroll
+        for (uint32_t s = 0; s < kS; ++s)
+            x[s] = xa[s][st * 16 * (kDense ? 1 : kW + 1)];
+        c0 = floatx4{0.f, 0.f, 0.f, 0.f};
+        c1 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (uint32_t s = 0; s < kS; ++s) {
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].x, bv[s], c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].y, bv[kS + s], c1, 0, 0, 0);
+        }
+    };
 
     // tiles of this chunk: ceil(count / rows); the first one was claimed by
     // the caller (in *slot)
     const uint32_t tiles = (count + rows - 1) / rows;
     uint32_t tb = uniform(*slot);
-    float2 v[mfma_stage(kBlock)];
-    mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + tb * rows) * down - P), down, v);
+    float2 v[kBf ? 1 : mfma_stage(kBlock)];
+    u32x4 vb[kBf ? kPairs : 1];
+    // bf16 planes: the staged span of a tile starts at its first window's
+    // first sample less `base`
+    auto load = [&](uint32_t t) {
+        const int64_t a0t = (int64_t)((n0 + t * rows) * down - P);
+        if constexpr (kBf)
+            bf_load<kBlock>(a, hist, a0t - (int64_t)(kW - 1) - (int64_t)base, vb);
+        else
+            mfma_load<kBlock, kW>(a, hist, kDense, a0t, down, v);
+    };
+    load(tb);
     for (;;) {
         const uint32_t j0 = tb * rows;
         const uint32_t nt = min(rows, count - j0);
         const int64_t a0 = (int64_t)((n0 + j0) * down - P);  // chunk-local anchor of row 0
         __syncthreads();  // the previous tile's reads of LDS (and of *slot) are done
-        mfma_commit<kBlock, kW>(kDense, down, mc, v, xs);
+        if constexpr (kBf)
+            bf_commit<kBlock>(vb, planes);
+        else
+            mfma_commit<kBlock, kW>(kDense, down, mc, v, xs);
         seq.claim(slot);
         __syncthreads();
         const uint32_t tn = uniform(*slot);
         if (tn < tiles)
-            mfma_load<kBlock, kW>(a, hist, kDense, (int64_t)((n0 + tn * rows) * down - P), down, v);
-        uint64_t u[2];
-        float2 ph[2];
-        {
-            const uint64_t k0 = mod_u64(P + (uint64_t)a0 + (uint64_t)(wave * 16) * down, (uint32_t)R, inv_r);
-#pragma unroll
-            for (uint32_t e = 0; e < 2; ++e) {
-                u[e] = k0 + (uint64_t)(ro + e) * down;
-                ph[e] = nco_phasor((int64_t)u[e], nu);
-            }
-        }
+            load(tn);
         const uint32_t subtiles = (nt + 15) >> 4;
-        for (uint32_t st = wave; st < subtiles; st += kWaves) {
-            float2 x[kS];
+        if (wave >= subtiles)
+            goto next_tile;  // uniform
+        {
+            uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(wave * 16 + ro) * down, (uint32_t)R, inv_r);
+            float2 ph = nco_phasor((int64_t)k, nu);
+            gvec2_ptr out_t = out + j0;
+            // Finish sub-tile st from its two chains: combine real and imaginary
+            // parts across the lane pair, apply the NCO phasor, store. With
+            // kPipe, the next sub-tile's LDS reads and MFMAs were issued just
+            // before in the same block of code and interleave with this work.
+            auto finish = [&](auto pipe, uint32_t st, const floatx4 &c0, const floatx4 &c1) {
+                const floatx4 acc = c0 + c1;
+                // partner lane's values (quad_perm 1,0,3,2 swaps lanes 2c and 2c+1)
+                float oth[4];
 #pragma unroll
-            for (uint32_t s = 0; s < kS; ++s)
-                x[s] = xa[s][st * 16 * (kDense ? 1 : kW + 1)];
-            // two independent accumulation chains (real-part and imaginary-part
-            // steps), summed at the end: halves the chain of dependent MFMAs
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+                for (uint32_t e = 0; e < 4; ++e)
+                    oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
+                const float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
+                const float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
+                const uint32_t r0 = st * 16 + ro;
+                float2 o0 = cmul(ph, y0);
+                float2 o1 = cmul(cmul(ph, s_row), y1);
+                // fast path: a whole sub-tile of rows (the lane's two rows are
+                // then one 16-byte store), no rollover near the lane's rows
+                const bool fast = out16 && st * 16 + 16 <= nt && k >= cpp1 && k < k_hi;
+                if constexpr (decltype(pipe)::value) {
+                    __builtin_amdgcn_sched_group_barrier(0x100, kS, 0);
 #pragma unroll
-            for (uint32_t s = 0; s < kS; ++s) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].x, bv[s], acc, 0, 0, 0);
-                acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s].y, bv[kS + s], acc2, 0, 0, 0);
-            }
-            acc += acc2;
-            // all of the sub-tile's LDS reads first, then its MFMA chain: one
-            // LDS latency per sub-tile instead of one per step
-            __builtin_amdgcn_sched_group_barrier(0x100, kS, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 2 * kS, 0);
-            // partner lane's values (quad_perm 1,0,3,2 swaps lanes 2c and 2c+1)
-            float oth[4];
-#pragma unroll
-            for (uint32_t e = 0; e < 4; ++e)
-                oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
-#pragma unroll
-            for (uint32_t e = 0; e < 2; ++e) {
-                const uint32_t row = st * 16 + ro + e;
-                if (live && row < nt) {
-                    float2 y = odd ? make_float2(oth[2 + e], acc[2 + e]) : make_float2(acc[e], oth[e]);
-                    float2 p = ph[e];
-                    uint64_t k = u[e];
-                    if (k >= R) {
-                        k -= R;
-                        p = cmul(p, unwrap);
+                    for (uint32_t i = 0; i < 2 * kS; ++i) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
                     }
-                    if (k < cpp1) {
-                        // a rollover inside the window: window taps before it
-                        // (t < w-1-k) carry e^{j2pi nu R}; add (e^{j2pi nu R} - 1)
-                        // times their partial sum, from the staged tile in LDS
-                        const uint32_t split = kW - 1 - (uint32_t)k;
-                        float2 part = make_float2(0.f, 0.f);
-                        uint32_t q = 0, rr = 0;
-                        for (uint32_t t = 0; t < split; ++t) {
-                            const float2 xv = kDense ? xs[rr * mc + q + row] : xs[row * (kW + 1) + t];
-                            cmac(part, my_taps[t], xv);
-                            if (++rr == down) {
-                                rr = 0;
-                                ++q;
-                            }
-                        }
-                        cmac(y, make_float2(wrap.x - 1.f, wrap.y), part);
-                    }
-                    const float2 o = cmul(p, y);
-                    out[j0 + row] = fvec2{o.x, o.y};
                 }
-                u[e] += du;
-                ph[e] = cmul(ph[e], step);
+                if (__builtin_amdgcn_ballot_w64(!fast) != 0) {
+                    // exact path (uniform branch, rare): FP64 NCO index and phasor
+                    // of both rows, the rollover correction from the staged tile,
+                    // and a fresh start for the next sub-tile
+                    if (!fast) {
+#pragma unroll
+                        for (uint32_t e = 0; e < 2; ++e) {
+                            const uint32_t row = r0 + e;
+                            const uint32_t kk =
+                                (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)row * down, (uint32_t)R, inv_r);
+                            float2 y = e ? y1 : y0;
+                            if (kk < cpp1) {
+                                // window taps before the rollover (t < w-1-kk) carry
+                                // e^{j2pi nu R}: add (e^{j2pi nu R} - 1) times their sum
+                                const uint32_t split = kW - 1 - kk;
+                                const uint32_t rw = min(row, rows - 1);
+                                float2 part = make_float2(0.f, 0.f);
+                                uint32_t q = 0, rr = 0;
+                                for (uint32_t t = 0; t < split; ++t) {
+                                    const float2 xv = kBf      ? bf_sample<kWords>(planes, rw * down + base + t)
+                                                      : kDense ? xs[rr * mc + q + rw]
+                                                               : xs[rw * (kW + 1) + t];
+                                    cmac(part, my_taps[t], xv);
+                                    if (++rr == down) {
+                                        rr = 0;
+                                        ++q;
+                                    }
+                                }
+                                cmac(y, make_float2(wrap.x - 1.f, wrap.y), part);
+                            }
+                            const float2 o = cmul(nco_phasor((int64_t)kk, nu), y);
+                            if (e)
+                                o1 = o;
+                            else
+                                o0 = o;
+                        }
+                        k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(r0 + 16 * kWaves) * down, (uint32_t)R,
+                                              inv_r);
+                        ph = nco_phasor((int64_t)k, nu);
+                    } else {
+                        k += du;
+                        ph = cmul(ph, s_sub);
+                    }
+                    (live && r0 < nt ? out_t + r0 : sink)[0] = fvec2{o0.x, o0.y};
+                    (live && r0 + 1 < nt ? out_t + r0 + 1 : sink + 1)[0] = fvec2{o1.x, o1.y};
+                } else {
+                    k += du;
+                    ph = cmul(ph, s_sub);
+                    // rows r0, r0+1 of one channel: the 8 lanes of a channel
+                    // write its 16 rows, 128 contiguous bytes
+                    ((gvec4_ptr)(live ? out_t + r0 : sink))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+                }
+            };
+            floatx4 pa, pb;  // the pending sub-tile's chains
+            mma(wave, pa, pb);
+            for (uint32_t st = wave;;) {
+                const uint32_t sn = st + kWaves;
+                if (sn >= subtiles) {
+                    finish(std::false_type{}, st, pa, pb);
+                    break;
+                }
+                floatx4 na, nb;
+                mma(sn, na, nb);
+                finish(std::true_type{}, st, pa, pb);
+                pa = na;
+                pb = nb;
+                st = sn;
             }
         }
+    next_tile:
         if (tn >= tiles)
             break;  // uniform
         tb = tn;
@@ -675,7 +875,7 @@ __device__ __forceinline__ void generic_group(const GroupsDev &a, const float2 *
 
 // All of this block's tiles of group g from seq.
 template <int kBlock, bool kWide>
-__device__ void run_group(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P, uint32_t g, TileSeq seq,
+__device__ __forceinline__ void run_group(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P, uint32_t g, TileSeq seq,
                           uint32_t *slot, unsigned char *smem)
 {
     const uint32_t *gw = a.groups + g * kGroupWords;
@@ -701,9 +901,12 @@ __device__ void run_group(const GroupsDev &a, const float2 *__restrict__ hist, u
                 return;
             }
         }
-        if (down < 32)
-            mfma_stripe<kBlock, 8, true>(a, hist, P, gw, n0, count, seq, slot, smem);
-        else
+        if (down < 32) {
+            if (gw[7] && a.in16)
+                mfma_stripe<kBlock, 8, true, true>(a, hist, P, gw, n0, count, seq, slot, smem);
+            else
+                mfma_stripe<kBlock, 8, true>(a, hist, P, gw, n0, count, seq, slot, smem);
+        } else
             mfma_stripe<kBlock, 8, false>(a, hist, P, gw, n0, count, seq, slot, smem);
         return;
     }
@@ -830,6 +1033,9 @@ hipError_t launch_chan_groups(const GroupsArgs &f, hipStream_t s)
     d.table = f.table;
     d.strided = f.strided ? 1u : 0u;
     d.claims = f.claims;
+    d.sink = f.sink;
+    d.out16 = f.out16 ? 1u : 0u;
+    d.in16 = f.in16 ? 1u : 0u;
     for (uint32_t k = 0; k < f.n_groups; ++k)
         d.tps[k] = f.tps[k];
     const uint32_t blocks = f.blocks;

@@ -138,14 +138,28 @@ constexpr uint32_t kXcds = 8;
 constexpr uint32_t kClaimShards = 8;
 constexpr uint32_t kClaimPitch = 32;  // u32
 constexpr size_t kClaimWords = ((size_t)kGroupsInline * kClaimShards + 1) * kClaimPitch;
+// samples of the batched kernel's sink: MFMA lanes and rows with no output
+// store there instead of branching around the store
+constexpr uint32_t kSinkWords = 1u << 16;
 // block-table entry of a block with no work (pads the per-XCD queues)
 constexpr uint32_t kNoGroup = 0xffffffffu;
 // taps per LDS pass of the one-channel sparse path
 constexpr uint32_t kVSparsePass = 8;
 // outputs per lane of the one-channel dense path
 constexpr uint32_t kVDenseOpl = 2;
+// MFMA groups whose windows of 32 taps advance by a multiple of 8 samples
+// (down in {8, 16, 24}) can run on bf16 matrix cores: the staged span is kept
+// as bf16 hi/lo planes from an 8-aligned start, and each product is
+// hi*hi + hi*lo + lo*hi (error about 2^-16 of each term). Their group record
+// holds the offset of those B tables + 1 in word 7.
+__host__ __device__ constexpr bool mfma_bf16_planes(uint32_t w, uint32_t down)
+{
+    return w == 32 && down < 32 && down % 8 == 0;
+}
+// MFMA B words per base (8 bf16 per lane, 3 K steps of 32, hi and lo)
+constexpr uint32_t kBfWordsPerBase = 3 * 2 * 64 * 4;
 // input samples per thread staged for one MFMA tile (4096 per block at most)
-constexpr uint32_t mfma_stage(uint32_t block) { return block >= 512 ? 8 : 16; }
+constexpr uint32_t mfma_stage(uint32_t block) { return block >= 256 ? 8 : 16; }
 
 struct GroupsArgs {
     const float2 *in;
@@ -168,6 +182,9 @@ struct GroupsArgs {
     uint32_t blocks;
     const uint32_t *tps;        // host: tiles per block (one stripe) of each group, or
     bool strided;               //   (strided) the group's block count: block s runs tiles s, s+tps, ...
+    float2 *sink;               // device: kSinkWords samples, written and never read
+    bool out16;                 // every channel's output is 16-byte aligned (paired-row stores)
+    bool in16;                  // the chunk's input is 16-byte aligned (bf16-plane staging)
     uint32_t *claims;           // device: kClaimWords zeroed words, or null: blocks then claim
                                 //   tiles one by one, starting in their table group
 };
