@@ -973,10 +973,13 @@ int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t *lds)
             switch (kind) {
             case kGroupMfma: {
                 const uint32_t stage = mfma_stage(block) * block;
-                // bf16 planes start up to 7 samples before the first window
-                const uint32_t reach = mfma_bf16_planes(w, down) ? w + 8 : w;
+                // bf16 planes: the span starts up to 7 samples before the first
+                // window and class windows read 40 slots; rows come in whole
+                // sub-tiles of every class
+                const bool bf = mfma_bf16_planes(w, down);
+                const uint32_t reach = bf ? w + 16 : w;
                 rows = down < w ? ((stage - reach) / down + 1) : stage / w;
-                step = lo = 16;
+                step = lo = bf ? 16 * mfma_bf16_classes(down) : 16;
                 rows = std::max(lo, rows / step * step);
                 break;
             }
@@ -1123,7 +1126,8 @@ void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
             for (uint32_t d = c; d < C; ++d)
                 if (!done[d] && mfma_window(h->ch[d]) == w && h->ch[d].down == k.down)
                     same.push_back(d);
-        if (same.size() >= kMfmaMinChan && k.down <= kMfmaMaxDown) {
+        const bool bf = h->bf16_planes && w && mfma_bf16_planes(w, k.down);
+        if (same.size() >= (bf ? 1u : kMfmaMinChan) && k.down <= kMfmaMaxDown) {
             // near-equal split into ceil(n / 8) groups
             const size_t ng = (same.size() + kMfmaMaxChan - 1) / kMfmaMaxChan;
             for (size_t i = 0, at = 0; i < ng; ++i) {
@@ -1152,7 +1156,9 @@ double tile_cost(const uint32_t *gw, uint32_t block)
     const double simds = 1024.0, bytes_per_cycle = 2500.0;
     double compute, in_samples;
     if (kind == kGroupMfma) {
-        compute = std::ceil(rows / 16.0) * (2.0 * (w / 4)) * 32.0 + rows / 16.0 * 200.0;
+        // bf16 planes: 9 bf16 steps of 16 cycles per 16 rows; else 2w/4 FP32 steps of 32
+        const double steps = gw[7] ? 9.0 * 16.0 : (2.0 * (w / 4)) * 32.0;
+        compute = std::ceil(rows / 16.0) * steps + rows / 16.0 * 200.0;
         in_samples = down < w ? (double)rows * down + w : (double)rows * w;
     } else {
         // four FMAs per complex tap per output, 64 lanes per 4-cycle wave instruction, 2x for the rest

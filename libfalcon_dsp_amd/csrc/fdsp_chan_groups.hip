@@ -477,6 +477,13 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = uniform(threadIdx.x >> 6);
     const uint32_t mc = kDense ? mfma_dense_mc(mfma_stage(kBlock) * kBlock, down) : 0;
+    // Rows of a tile fall in nc residue classes (bf16 planes: nc = 8/gcd(down,
+    // 8), whose rows' windows share their offset from an 8-aligned slot;
+    // otherwise nc = 1): class c holds rows c, c+nc, c+2nc, ... Wave w works
+    // on class w mod nc, on every wpc-th sub-tile of 16 of its rows.
+    const uint32_t nc = kBf ? 8u >> min(3u, (uint32_t)__builtin_ctz(down)) : 1u;
+    const uint32_t cls = wave % nc, wpc = kWaves / nc, wi = wave / nc;
+    const uint32_t dn = down * nc;  // inputs between consecutive rows of a class
     for (uint32_t e = threadIdx.x; e < m * kW; e += kBlock) {
         const ChanStatic &c = a.chans[gw[8 + e / kW]];
         const uint32_t t = e % kW, pad = kW - c.cpp;
@@ -490,8 +497,11 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
     // bf16 planes: the windows start `base` samples past an 8-aligned slot,
     // the same for every tile (tile anchors differ by multiples of down)
     const uint32_t base = (uint32_t)((int64_t)((n0 + uniform(*slot) * rows) * down - P) - (int64_t)(kW - 1)) & 7u;
+    // this wave's class: windows start cstart + dn*m + cbase samples into the
+    // staged span (cstart a multiple of 8, cbase < 8)
+    const uint32_t cstart = (base + down * cls) & ~7u, cbase = (base + down * cls) & 7u;
     if constexpr (kBf) {
-        const u32x4 *bt = reinterpret_cast<const u32x4 *>(a.btab + (gw[7] - 1) + base * kBfWordsPerBase);
+        const u32x4 *bt = reinterpret_cast<const u32x4 *>(a.btab + (gw[7] - 1) + cbase * kBfWordsPerBase);
 #pragma unroll
         for (uint32_t q = 0; q < 3; ++q) {
             bh[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 0) * 64 + lane]);
@@ -504,16 +514,17 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
             bv[s] = bt[s * 64 + lane];
     }
     uint32_t *planes = reinterpret_cast<uint32_t *>(xs);
-    // bf16 A operand of K step q: row lane%16, K block kb = 4q + lane/16: the
-    // real (kb < 5) or imaginary (kb 5..9) plane from slot 8*(kb mod 5); the
-    // zero-B blocks 10, 11 read the real plane's slot 0 (finite)
+    // bf16 A operand of K step q: class row lane%16, K block kb = 4q +
+    // lane/16: the real (kb < 5) or imaginary (kb 5..9) plane from slot
+    // 8*(kb mod 5) of the row's aligned window; the zero-B blocks 10, 11 read
+    // the real plane's slot 0 (finite)
     uint32_t aoff[3];
 #pragma unroll
     for (uint32_t q = 0; q < 3; ++q) {
         const uint32_t kb = 4 * q + (lane >> 4);
         const uint32_t plane = kb >= 5 && kb < 10 ? 1u : 0u;
         const uint32_t slot = kb < 5 ? 8 * kb : kb < 10 ? 8 * (kb - 5) : 0u;
-        aoff[q] = plane * kWords * 4 + (8 * (lane & 15) + slot) * 2;  // bytes, hi part
+        aoff[q] = plane * kWords * 4 + (cstart + dn * (lane & 15) + slot) * 2;  // bytes, hi part
     }
     // A operand of step s: row r = st*16 + lane%16, window tap 4s + lane/16,
     // read at xa[s] + 16*st
@@ -559,22 +570,22 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
     const bool out16 = a.out16 != 0;
     const float2 *my_taps = taps_l + cl * kW;
     const uint32_t ro = 4 * (lane >> 4) + (odd ? 2 : 0);
-    // NCO index k of the lane's first row (ro) in its current sub-tile, in
-    // [0, R), with the phasor e^{j2pi nu k}; the lane's second row is one
-    // row (down inputs) later and the wave's next sub-tile du inputs later.
-    // While cpp-1 <= k < R - du - down neither row's window holds a rollover
-    // and neither index nor the next sub-tile's wraps: the fast path steps
-    // the phasor by recurrence. Otherwise the wave takes the exact path.
-    const uint32_t du = 16 * kWaves * down;
-    const float2 s_row = nco_phasor((int64_t)down, nu);
+    // NCO index k of the lane's first row (class row ro) in its current
+    // sub-tile, in [0, R), with the phasor e^{j2pi nu k}; the lane's second
+    // row is dn inputs later and the wave's next sub-tile du inputs later.
+    // While cpp-1 <= k < R - du - dn neither row's window holds a rollover and
+    // neither index nor the next sub-tile's wraps: the fast path steps the
+    // phasor by recurrence. Otherwise the wave takes the exact path.
+    const uint32_t du = 16 * wpc * dn;
+    const float2 s_row = nco_phasor((int64_t)dn, nu);
     const float2 s_sub = nco_phasor((int64_t)du, nu);
-    const uint32_t k_hi = R > (uint64_t)du + down ? (uint32_t)(R - du - down) : 0u;
+    const uint32_t k_hi = R > (uint64_t)du + dn ? (uint32_t)(R - du - dn) : 0u;
 
     // A operands and MFMA chains of one sub-tile: two independent chains (the
     // real-part and the imaginary-part steps), summed by the consumer
     auto mma = [&](uint32_t st, floatx4 &c0, floatx4 &c1) {
         if constexpr (kBf) {
-            const unsigned char *pb = reinterpret_cast<const unsigned char *>(planes) + st * 16 * 16;
+            const unsigned char *pb = reinterpret_cast<const unsigned char *>(planes) + st * 16 * dn * 2;
             bf16x8 ah[3], al[3];
 #pragma unroll
             for (uint32_t q = 0; q < 3; ++q) {
@@ -634,11 +645,14 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
         const uint32_t tn = uniform(*slot);
         if (tn < tiles)
             load(tn);
-        const uint32_t subtiles = (nt + 15) >> 4;
-        if (wave >= subtiles)
+        // rows of this wave's class in the tile, and their sub-tiles of 16
+        const uint32_t crows = nt > cls ? (nt - cls + nc - 1) / nc : 0u;
+        const uint32_t subtiles = (crows + 15) >> 4;
+        if (wi >= subtiles)
             goto next_tile;  // uniform
         {
-            uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(wave * 16 + ro) * down, (uint32_t)R, inv_r);
+            uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wi * 16 + ro)) * down, (uint32_t)R,
+                                           inv_r);
             float2 ph = nco_phasor((int64_t)k, nu);
             gvec2_ptr out_t = out + j0;
             // Finish sub-tile st from its two chains: combine real and imaginary
@@ -654,12 +668,12 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                     oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
                 const float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
                 const float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
-                const uint32_t r0 = st * 16 + ro;
+                const uint32_t r0 = cls + nc * (st * 16 + ro);  // tile row of the lane's first output
                 float2 o0 = cmul(ph, y0);
                 float2 o1 = cmul(cmul(ph, s_row), y1);
                 // fast path: a whole sub-tile of rows (the lane's two rows are
                 // then one 16-byte store), no rollover near the lane's rows
-                const bool fast = out16 && st * 16 + 16 <= nt && k >= cpp1 && k < k_hi;
+                const bool fast = cls + nc * (st * 16 + 15) < nt && k >= cpp1 && k < k_hi;
                 if constexpr (decltype(pipe)::value) {
                     __builtin_amdgcn_sched_group_barrier(0x100, kS, 0);
 #pragma unroll
@@ -675,7 +689,7 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                     if (!fast) {
 #pragma unroll
                         for (uint32_t e = 0; e < 2; ++e) {
-                            const uint32_t row = r0 + e;
+                            const uint32_t row = r0 + e * nc;
                             const uint32_t kk =
                                 (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)row * down, (uint32_t)R, inv_r);
                             float2 y = e ? y1 : y0;
@@ -704,7 +718,7 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                             else
                                 o0 = o;
                         }
-                        k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(r0 + 16 * kWaves) * down, (uint32_t)R,
+                        k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(r0 + 16 * wpc * nc) * down, (uint32_t)R,
                                               inv_r);
                         ph = nco_phasor((int64_t)k, nu);
                     } else {
@@ -712,19 +726,24 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                         ph = cmul(ph, s_sub);
                     }
                     (live && r0 < nt ? out_t + r0 : sink)[0] = fvec2{o0.x, o0.y};
-                    (live && r0 + 1 < nt ? out_t + r0 + 1 : sink + 1)[0] = fvec2{o1.x, o1.y};
+                    (live && r0 + nc < nt ? out_t + r0 + nc : sink + 1)[0] = fvec2{o1.x, o1.y};
                 } else {
                     k += du;
                     ph = cmul(ph, s_sub);
-                    // rows r0, r0+1 of one channel: the 8 lanes of a channel
-                    // write its 16 rows, 128 contiguous bytes
-                    ((gvec4_ptr)(live ? out_t + r0 : sink))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+                    if (nc == 1 && out16) {
+                        // rows r0, r0+1 of one channel: the 8 lanes of a
+                        // channel write its 16 rows, 128 contiguous bytes
+                        ((gvec4_ptr)(live ? out_t + r0 : sink))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+                    } else {
+                        (live ? out_t + r0 : sink)[0] = fvec2{o0.x, o0.y};
+                        (live ? out_t + r0 + nc : sink + 1)[0] = fvec2{o1.x, o1.y};
+                    }
                 }
             };
             floatx4 pa, pb;  // the pending sub-tile's chains
-            mma(wave, pa, pb);
-            for (uint32_t st = wave;;) {
-                const uint32_t sn = st + kWaves;
+            mma(wi, pa, pb);
+            for (uint32_t st = wi;;) {
+                const uint32_t sn = st + wpc;
                 if (sn >= subtiles) {
                     finish(std::false_type{}, st, pa, pb);
                     break;
@@ -902,7 +921,7 @@ __device__ __forceinline__ void run_group(const GroupsDev &a, const float2 *__re
             }
         }
         if (down < 32) {
-            if (gw[7] && a.in16)
+            if (gw[7] && a.in16 && (8u >> min(3u, (uint32_t)__builtin_ctz(down))) <= kBlock / 64)
                 mfma_stripe<kBlock, 8, true, true>(a, hist, P, gw, n0, count, seq, slot, smem);
             else
                 mfma_stripe<kBlock, 8, true>(a, hist, P, gw, n0, count, seq, slot, smem);

@@ -147,14 +147,20 @@ constexpr uint32_t kNoGroup = 0xffffffffu;
 constexpr uint32_t kVSparsePass = 8;
 // outputs per lane of the one-channel dense path
 constexpr uint32_t kVDenseOpl = 2;
-// MFMA groups whose windows of 32 taps advance by a multiple of 8 samples
-// (down in {8, 16, 24}) can run on bf16 matrix cores: the staged span is kept
-// as bf16 hi/lo planes from an 8-aligned start, and each product is
-// hi*hi + hi*lo + lo*hi (error about 2^-16 of each term). Their group record
-// holds the offset of those B tables + 1 in word 7.
+// MFMA groups of 32-tap windows with an even down below 32 can run on bf16
+// matrix cores: the staged span is kept as bf16 hi/lo planes from an
+// 8-aligned start, rows are taken in nc = 8/gcd(down, 8) <= 4 classes whose
+// windows share one offset from an 8-aligned slot, and each product is
+// hi*hi + hi*lo + lo*hi (error about 2^-16 of each term). Such channels form
+// groups from one channel up. Their group record holds the offset of the
+// bf16 B tables + 1 in word 7.
 __host__ __device__ constexpr bool mfma_bf16_planes(uint32_t w, uint32_t down)
 {
-    return w == 32 && down < 32 && down % 8 == 0;
+    return w == 32 && down < 32 && down % 2 == 0;
+}
+__host__ __device__ constexpr uint32_t mfma_bf16_classes(uint32_t down)
+{
+    return down % 8 == 0 ? 1u : down % 4 == 0 ? 2u : 4u;
 }
 // MFMA B words per base (8 bf16 per lane, 3 K steps of 32, hi and lo)
 constexpr uint32_t kBfWordsPerBase = 3 * 2 * 64 * 4;
