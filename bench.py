@@ -310,7 +310,8 @@ def main():
         g.set_tiling(args.seg or s0, args.block or b0)
     else:
         trials = g.autotune(d_in, n, reps=5)
-        log("autotune: " + ", ".join(f"{t['segment_len']}/{t['block_threads']}: {t['us']} us" for t in trials))
+        log("autotune: " + ", ".join(f"{t['segment_len']}/{t['block_threads']}{'m' if t['measured_weights'] else ''}: "
+                                     f"{t['us']} us" for t in trials))
     seg, block = g.get_tiling()
     outs = [fd.DeviceBuffer(m) for m in g.max_out(n)]
     g.graph_create(n, d_in, outs)

@@ -242,17 +242,21 @@ int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uin
  * (diagnostics for the autotuner; any output may be null). */
 int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t *blocks, uint32_t *capacity,
                                uint32_t *groups, uint32_t *lds_bytes);
-/* Autotuner: times the batched kernel on the device chunk d_in (chunk_len
+/* Autotuner: first times each channel group alone (on all and on half of
+ * the resident blocks) to weigh the groups' shares of the blocks by measured
+ * work, then times the batched kernel on the device chunk d_in (chunk_len
  * samples, the caller's real input) under each candidate work split
- * (FDSP_TILING_STRIDED / FDSP_TILING_CLAIM at 128, 256 and 512 threads,
- * segments of 4096, 16384 and 65536 inputs at 256), reps launches each after
- * a warm-up, and keeps the fastest with fdsp_channelizer_set_tiling. The
- * launches write scratch outputs and leave the stream position and history
- * as they were. trials (optional, cap entries) receives every candidate's
- * average kernel time (us < 0: the split does not fit this channel set). */
+ * (FDSP_TILING_STRIDED / FDSP_TILING_CLAIM at 128, 256 and 512 threads with
+ * estimated group weights, at 256 and 512 with measured ones; segments of
+ * 4096 ... 65536 inputs at 256 threads and of 16384 at 512), reps launches
+ * each after a warm-up, and keeps the fastest. The launches write scratch
+ * outputs and leave the stream position and history as they were. trials
+ * (optional, cap entries, up to 16) receives every candidate's average kernel
+ * time (us < 0: the split does not fit this channel set). */
 typedef struct fdsp_tiling_trial {
     uint32_t segment_len, block_threads;
     float us;
+    uint32_t measured_weights; /* 1: the split weighed the groups by measured work */
 } fdsp_tiling_trial;
 int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len, uint32_t reps,
                               fdsp_tiling_trial *trials, size_t cap, size_t *n_trials);

@@ -90,7 +90,8 @@ class ServeStats(C.Structure):
 
 class TilingTrial(C.Structure):
     """fdsp_tiling_trial."""
-    _fields_ = [("segment_len", C.c_uint32), ("block_threads", C.c_uint32), ("us", C.c_float)]
+    _fields_ = [("segment_len", C.c_uint32), ("block_threads", C.c_uint32), ("us", C.c_float),
+                ("measured_weights", C.c_uint32)]
 
 
 _vp = C.c_void_p

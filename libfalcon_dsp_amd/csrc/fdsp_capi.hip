@@ -868,6 +868,11 @@ struct fdsp_channelizer_s {
     uint32_t taps_total = 0, hist_len = 0;
     uint32_t seg_len = FDSP_TILING_STRIDED, block = 256;
     bool bf16_planes = true;  // MFMA groups with 8-aligned windows on bf16 matrix cores (FDSP_MFMA_FP32=1: off)
+    // block-time per tile of each group, measured by the autotuner (empty:
+    // the tile_cost estimate weighs the groups)
+    std::vector<double> gcost;
+    int only_group = -1;       // autotuner: only this group, on only_blocks blocks
+    uint32_t only_blocks = 0;
     uint32_t capacity = 1024;  // fused-kernel blocks resident at once (occupancy x CUs)
     // work plan: channel groups (kGroupWords u32 each) and their MFMA B tables
     std::vector<uint32_t> groups;
@@ -1200,15 +1205,19 @@ void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table
         const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
         tiles[g] = (most + gw[5] - 1) / gw[5];
         span[g] = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
-        total += tiles[g] * tile_cost(gw, h->block);
+        total += tiles[g] * (h->gcost.empty() ? tile_cost(gw, h->block) : h->gcost[g]);
     }
     for (uint32_t g = 0; g < h->n_groups; ++g) {
         uint64_t tps;
         if (h->seg_len <= FDSP_TILING_STRIDED) {
             // strided split: the group's blocks sweep its tiles together, tps = block count
-            const double share = tiles[g] * tile_cost(h->groups.data() + g * kGroupWords, h->block) / total;
-            const uint64_t blocks = std::min<uint64_t>(
-                tiles[g], std::max<uint64_t>(1, (uint64_t)std::llround(share * h->capacity)));
+            const double share =
+                tiles[g] * (h->gcost.empty() ? tile_cost(h->groups.data() + g * kGroupWords, h->block) : h->gcost[g]) /
+                total;
+            uint64_t blocks = std::min<uint64_t>(tiles[g],
+                                                 std::max<uint64_t>(1, (uint64_t)std::llround(share * h->capacity)));
+            if (h->only_group >= 0)
+                blocks = (int)g == h->only_group ? std::min<uint64_t>(tiles[g], h->only_blocks) : 0;
             h->tps[g] = (uint32_t)blocks;
             for (uint64_t st = 0; st < blocks; ++st)
                 items.push_back({st * span[g], g, (uint32_t)st});
@@ -1920,10 +1929,14 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
     }
     if (int rc = chan_upload_outs(h, ptrs.data()))
         return rc;
-    static const uint32_t kCand[][2] = {
-        {FDSP_TILING_STRIDED, 256}, {FDSP_TILING_STRIDED, 512}, {FDSP_TILING_STRIDED, 128},
-        {FDSP_TILING_CLAIM, 256},   {FDSP_TILING_CLAIM, 512},   {FDSP_TILING_CLAIM, 128},
-        {4096, 256},                {16384, 256},               {65536, 256},
+    // candidates: {split, threads, group weights measured (1) or estimated (0)}
+    static const uint32_t kCand[][3] = {
+        {FDSP_TILING_STRIDED, 256, 0}, {FDSP_TILING_STRIDED, 512, 0}, {FDSP_TILING_STRIDED, 128, 0},
+        {FDSP_TILING_CLAIM, 256, 0},   {FDSP_TILING_CLAIM, 512, 0},   {FDSP_TILING_CLAIM, 128, 0},
+        {4096, 256, 0},                {8192, 256, 0},                {16384, 256, 0},
+        {32768, 256, 0},               {65536, 256, 0},               {16384, 512, 0},
+        {FDSP_TILING_STRIDED, 256, 1}, {FDSP_TILING_STRIDED, 512, 1}, {FDSP_TILING_CLAIM, 256, 1},
+        {FDSP_TILING_CLAIM, 512, 1},
     };
     const uint32_t seg0 = h->seg_len, block0 = h->block;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1933,39 +1946,87 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
         return set_err(FDSP_EHIP, "event creation failed");
     }
     int rc = FDSP_OK;
+    // average time of reps back-to-back launches of the current plan
+    auto time_plan = [&](float *us) -> int {
+        for (auto &e : h->tables)
+            e.second.release();
+        h->tables.clear();
+        GroupsArgs f;
+        if (int r = groups_args(h, d_in, n, false, &f))
+            return r;
+        if (launch_chan_groups(f, h->stream) != hipSuccess || hipEventRecord(e0, h->stream) != hipSuccess)
+            return set_err(FDSP_EHIP, "autotune launch failed");
+        for (uint32_t r = 0; r < reps; ++r)
+            if (launch_chan_groups(f, h->stream) != hipSuccess)
+                return set_err(FDSP_EHIP, "autotune launch failed");
+        float ms = 0.f;
+        if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return set_err(FDSP_EHIP, "autotune timing failed");
+        *us = 1e3f * ms / (float)reps;
+        return FDSP_OK;
+    };
+    // Group weights: each group alone (strided split, 256 threads) on every
+    // block the GPU holds and on half of them. The difference of the two times
+    // is the group's work per block at full occupancy, free of the launch's
+    // fixed costs; the splits below give each group blocks in proportion to it.
+    h->gcost.clear();
+    std::vector<double> measured;
+    if (h->n_groups > 1 && (rc = fdsp_channelizer_set_tiling(h, FDSP_TILING_STRIDED, 256)) == FDSP_OK) {
+        std::vector<double> work(h->n_groups, 0.0);
+        double most = 0;
+        for (uint32_t g = 0; g < h->n_groups && !rc; ++g) {
+            h->only_group = (int)g;
+            float full = 0.f, half = 0.f;
+            h->only_blocks = h->capacity;
+            rc = time_plan(&full);
+            h->only_blocks = std::max(1u, h->capacity / 2);
+            if (!rc)
+                rc = time_plan(&half);
+            work[g] = std::max(0.0, (double)half - (double)full);
+            most = std::max(most, work[g]);
+        }
+        h->only_group = -1;
+        if (!rc && most > 0) {
+            measured.resize(h->n_groups);
+            for (uint32_t g = 0; g < h->n_groups; ++g) {
+                const uint32_t *gw = h->groups.data() + g * kGroupWords;
+                const uint64_t tiles = ((uint64_t)n * gw[1] / gw[2] + 1 + gw[5] - 1) / gw[5];
+                measured[g] = std::max(work[g], 0.02 * most) / (double)std::max<uint64_t>(1, tiles);
+            }
+        }
+    }
     float best = -1.f;
     uint32_t best_seg = seg0, best_block = block0;
+    bool best_measured = false;
     size_t k = 0;
     for (const auto &cand : kCand) {
         float us = -1.f;
-        if (fdsp_channelizer_set_tiling(h, cand[0], cand[1]) == FDSP_OK) {
-            GroupsArgs f;
-            rc = groups_args(h, d_in, n, false, &f);
+        if (cand[2] && measured.empty())
+            continue;  // one group, or no weights: nothing to compare
+        if (cand[2])
+            h->gcost = measured;
+        else
+            h->gcost.clear();
+        if (!rc && fdsp_channelizer_set_tiling(h, cand[0], cand[1]) == FDSP_OK) {
             // one untimed launch (table upload, caches), then reps back to back
-            if (!rc && launch_chan_groups(f, h->stream) != hipSuccess)
-                rc = set_err(FDSP_EHIP, "autotune launch failed");
-            if (!rc && hipEventRecord(e0, h->stream) != hipSuccess)
-                rc = set_err(FDSP_EHIP, "event record failed");
-            for (uint32_t r = 0; r < reps && !rc; ++r)
-                if (launch_chan_groups(f, h->stream) != hipSuccess)
-                    rc = set_err(FDSP_EHIP, "autotune launch failed");
-            float ms = 0.f;
-            if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = set_err(FDSP_EHIP, "autotune timing failed");
-            if (rc)
+            if ((rc = time_plan(&us)) != FDSP_OK)
                 break;
-            us = 1e3f * ms / (float)reps;
             if (best < 0.f || us < best) {
                 best = us;
                 best_seg = cand[0];
                 best_block = cand[1];
+                best_measured = cand[2] != 0;
             }
         }
         if (trials && k < cap)
-            trials[k] = fdsp_tiling_trial{cand[0], cand[1], us};
+            trials[k] = fdsp_tiling_trial{cand[0], cand[1], us, cand[2]};
         ++k;
     }
+    if (best_measured && !rc)
+        h->gcost = measured;
+    else
+        h->gcost.clear();
     (void)hipStreamSynchronize(h->stream);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -431,8 +431,8 @@ class MultiRateChannelizer(_Handle):
         tr = (N.TilingTrial * cap)()
         k = C.c_size_t()
         check(lib().fdsp_channelizer_autotune(self.h, d_in.at(in_offset), chunk_len, reps, tr, cap, C.byref(k)))
-        return [{"segment_len": t.segment_len, "block_threads": t.block_threads, "us": round(t.us, 2)}
-                for t in tr[: k.value]]
+        return [{"segment_len": t.segment_len, "block_threads": t.block_threads, "us": round(t.us, 2),
+                 "measured_weights": bool(t.measured_weights)} for t in tr[: k.value]]
 
     def plan_info(self, chunk_len: int) -> dict:
         """Work split of the batched kernel for a chunk length (diagnostics)."""

@@ -216,7 +216,7 @@ def test_autotune_keeps_stream_state():
             d = fd.DeviceBuffer.from_array(x[:131_072])
             trials = g.autotune(d, 131_072, reps=3)
             timed = [t for t in trials if t["us"] > 0]
-            assert len(trials) == 9 and timed
+            assert len(trials) >= 9 and timed
             best = min(timed, key=lambda t: t["us"])
             assert g.get_tiling() == (best["segment_len"], best["block_threads"])
 
