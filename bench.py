@@ -310,8 +310,8 @@ def main():
         g.set_tiling(args.seg or s0, args.block or b0)
     else:
         trials = g.autotune(d_in, n, reps=5)
-        log("autotune: " + ", ".join(f"{t['segment_len']}/{t['block_threads']}{'m' if t['measured_weights'] else ''}: "
-                                     f"{t['us']} us" for t in trials))
+        log("autotune: " + ", ".join(f"{t['segment_len']}/{t['block_threads']}{'m' if t['measured_weights'] else ''}"
+                                     f"{'f' if t['fused'] else ''}: {t['us']} us" for t in trials))
     seg, block = g.get_tiling()
     outs = [fd.DeviceBuffer(m) for m in g.max_out(n)]
     g.graph_create(n, d_in, outs)
@@ -431,7 +431,8 @@ def main():
                                "{12.5k,25k,100k,250k,1M,2.5M} sps, 32 Kaiser taps, one fused launch per step "
                                "in a HIP graph",
                    "input_samples": n, "channels": len(chans), "outputs_per_step": int(round(total_out / args.steps)),
-                   "segment_len": seg, "block_threads": block, "autotune_trials": trials},
+                   "segment_len": seg, "block_threads": block, "fused_groups": g.get_fusion()[0],
+                   "autotune_trials": trials},
         "p50_chunk_latency_ms": round(p50, 5),
         "p99_chunk_latency_ms": round(p99, 5),
         "latency_chunk_samples": lc,

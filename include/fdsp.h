@@ -249,17 +249,26 @@ int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t 
  * (FDSP_TILING_STRIDED / FDSP_TILING_CLAIM at 128, 256 and 512 threads with
  * estimated group weights, at 256 and 512 with measured ones; segments of
  * 4096 ... 65536 inputs at 256 threads and of 16384 at 512), reps launches
- * each after a warm-up, and keeps the fastest. The launches write scratch
+ * each after a warm-up, and keeps the fastest; then it times that split with
+ * fusion on (bit 1 of the trial flags) and keeps it if faster. The launches
+ * write scratch
  * outputs and leave the stream position and history as they were. trials
- * (optional, cap entries, up to 16) receives every candidate's average kernel
+ * (optional, cap entries, up to 17) receives every candidate's average kernel
  * time (us < 0: the split does not fit this channel set). */
 typedef struct fdsp_tiling_trial {
     uint32_t segment_len, block_threads;
     float us;
-    uint32_t measured_weights; /* 1: the split weighed the groups by measured work */
+    uint32_t flags; /* bit 0: groups weighed by measured work; bit 1: bf16-plane groups fused */
 } fdsp_tiling_trial;
 int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len, uint32_t reps,
                               fdsp_tiling_trial *trials, size_t cap, size_t *n_trials);
+/* Phase grouping: with fusion on, the bf16-plane channel groups whose rows
+ * tile 1920-sample spans evenly share one staged span per tile (the input is
+ * read and converted once for all of them); it applies to 16-byte aligned
+ * input. Off by default; the autotuner turns it on when it measures faster.
+ * fusable_groups: groups in the fused record (0: nothing to fuse). */
+int fdsp_channelizer_set_fusion(fdsp_channelizer_t h, int on);
+int fdsp_channelizer_get_fusion(fdsp_channelizer_t h, int *on, uint32_t *fusable_groups);
 
 /* ---------------------------------------------------------------------------
  * Serving: channelizer streams admitted as they arrive, their chunks batched

@@ -91,7 +91,7 @@ class ServeStats(C.Structure):
 class TilingTrial(C.Structure):
     """fdsp_tiling_trial."""
     _fields_ = [("segment_len", C.c_uint32), ("block_threads", C.c_uint32), ("us", C.c_float),
-                ("measured_weights", C.c_uint32)]
+                ("flags", C.c_uint32)]
 
 
 _vp = C.c_void_p
@@ -172,6 +172,8 @@ _SIGS = [
     ("fdsp_channelizer_get_tiling", _i, [_vp, _P(_u32), _P(_u32)]),
     ("fdsp_channelizer_plan_info", _i, [_vp, _sz, _P(_u32), _P(_u32), _P(_u32), _P(_u32)]),
     ("fdsp_channelizer_autotune", _i, [_vp, _vp, _sz, _u32, _P(TilingTrial), _sz, _P(_sz)]),
+    ("fdsp_channelizer_set_fusion", _i, [_vp, _i]),
+    ("fdsp_channelizer_get_fusion", _i, [_vp, _P(_i), _P(_u32)]),
     ("fdsp_serve_create", _i, [_P(ServeConfig), _P(_vp)]),
     ("fdsp_serve_destroy", _i, [_vp]),
     ("fdsp_serve_admit", _i, [_vp, _P(ChannelDesc), _P(_u32)]),

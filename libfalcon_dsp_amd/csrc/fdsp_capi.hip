@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -881,9 +882,15 @@ struct fdsp_channelizer_s {
     DevBuf d_claims;  // per-group tile claim counters + a finished-block counter (zero between launches)
     DevBuf d_sink;    // stores of rows and lanes with no output
     uint32_t n_groups = 0, lds_bytes = 0;
+    uint32_t lds_plan[2] = {0, 0};  // launch LDS without / with the fused record scheduled
     std::vector<uint32_t> tps;
     // block tables by chunk length (eager calls), and the graph's own
-    std::vector<std::pair<uint32_t, DevBuf>> tables;
+    std::vector<std::pair<uint32_t, DevBuf>> tables;  // key: chunk length * 2 + input 16-byte aligned
+    // groups each launch schedules: [1] when the input is 16-byte aligned (a
+    // fused record in place of its members), [0] otherwise (the members)
+    std::vector<uint32_t> sched[2];
+    bool fuse = true;      // FDSP_FUSE=0: no fused records are built
+    bool fuse_on = false;  // launches schedule the fused record (fdsp_channelizer_set_fusion, autotuner)
     DevBuf d_table_graph;
     // host-pointer path
     DevBuf d_in;
@@ -959,11 +966,12 @@ size_t group_lds(uint32_t kind, uint32_t up, uint32_t down, uint32_t w, uint32_t
 }
 
 // Rows per tile of each group for `block` threads, written into word 5 of
-// the group records; the launch's LDS size (the largest need) goes to *lds.
+// the group records; the launch's LDS size (the largest need) goes to lds[0]
+// for the member groups alone and to lds[1] with the fused record as well.
 // MFMA tiles hold what one staging pass of mfma_stage(block) samples per
 // thread brings in; the one-channel tiles are sized to the same LDS budget where
 // they can be (all blocks of the launch get the same LDS allocation).
-int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t *lds)
+int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t lds[2])
 {
     const size_t G = groups.size() / kGroupWords;
     size_t budget = 32 * 1024;
@@ -1011,7 +1019,20 @@ int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t *lds)
             budget = std::max(budget, need);
         }
     }
-    *lds = (uint32_t)budget;
+    lds[0] = (uint32_t)budget;
+    for (size_t g = 0; g < G; ++g)
+        if (groups[g * kGroupWords] == kGroupFused) {
+            // B tables (6 KB per member class), planes, taps, NCO steps, channels
+            const uint32_t *fw = groups.data() + g * kGroupWords;
+            size_t nb = 0;
+            for (uint32_t i = 0; i < fw[4]; ++i)
+                nb += mfma_bf16_classes(groups[fw[8 + i] * kGroupWords + 2]);
+            const size_t need = nb * kBfWordsPerBase * 4 + (size_t)mfma_stage(block) * block * 8 +
+                                fw[4] * kMfmaMaxChan * (32 * sizeof(float2) + 2 * sizeof(float2) + sizeof(ChanStatic)) +
+                                512;
+            budget = std::max(budget, need);
+        }
+    lds[1] = (uint32_t)budget;
     return FDSP_OK;
 }
 
@@ -1175,6 +1196,101 @@ double tile_cost(const uint32_t *gw, uint32_t block)
     return std::max(compute / simds, bytes / bytes_per_cycle);
 }
 
+// A fused record runs on 16-byte aligned input only, and needs a wave per
+// residue class of each member
+bool fused_ok(const fdsp_channelizer_s *h, bool in16)
+{
+    if (!h->fuse_on || !in16 || h->sched[1] == h->sched[0])
+        return false;
+    for (uint32_t g : h->sched[1]) {
+        const uint32_t *gw = h->groups.data() + g * kGroupWords;
+        if (gw[0] == kGroupFused)
+            for (uint32_t i = 0; i < gw[4]; ++i)
+                if (mfma_bf16_classes(h->groups[gw[8 + i] * kGroupWords + 2]) > h->block / 64)
+                    return false;
+    }
+    return true;
+}
+
+// Schedule the fused record (on) or its members (off): the launch's LDS
+// size and the resident-block capacity follow, and the cached block tables
+// are dropped. The caller makes sure no queued launch still reads them.
+int use_fusion(fdsp_channelizer_s *h, bool on)
+{
+    h->fuse_on = on;
+    h->lds_bytes = h->lds_plan[on ? 1 : 0];
+    for (auto &e : h->tables)
+        e.second.release();
+    h->tables.clear();
+    return update_capacity(h);
+}
+
+// estimated cost of one tile of group g (a fused tile: its members' work on L inputs)
+double group_tile_cost(const fdsp_channelizer_s *h, uint32_t g)
+{
+    const uint32_t *gw = h->groups.data() + g * kGroupWords;
+    if (gw[0] != kGroupFused)
+        return tile_cost(gw, h->block);
+    double c = 0;
+    for (uint32_t i = 0; i < gw[4]; ++i) {
+        const uint32_t *mw = h->groups.data() + gw[8 + i] * kGroupWords;
+        c += tile_cost(mw, h->block) * gw[2] / ((double)mw[5] * mw[2]);
+    }
+    return c * 0.8;  // one staging pass for all members
+}
+
+// Fuse the bf16-plane groups whose rows per tile of L inputs are whole
+// sub-tiles of every residue class (L a multiple of 16 * classes * down,
+// at most 1920 so a tile's span fits the staging capacity with its windows):
+// the largest such set up to kFusedMax members, largest groups first.
+void fuse_groups(fdsp_channelizer_s *h)
+{
+    std::vector<uint32_t> all(h->n_groups);
+    for (uint32_t g = 0; g < h->n_groups; ++g)
+        all[g] = g;
+    h->sched[0] = h->sched[1] = all;
+    if (!h->fuse || h->n_groups + 1 > kGroupsInline)
+        return;
+    std::vector<uint32_t> cand;
+    for (uint32_t g = 0; g < h->n_groups; ++g) {
+        const uint32_t *gw = h->groups.data() + g * kGroupWords;
+        if (gw[0] == kGroupMfma && gw[7] && gw[3] == 32)
+            cand.push_back(g);
+    }
+    // largest first: most outputs per input
+    std::stable_sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
+        const uint32_t *a = h->groups.data() + x * kGroupWords, *b = h->groups.data() + y * kGroupWords;
+        return (double)a[4] / a[2] > (double)b[4] / b[2];
+    });
+    constexpr uint64_t kMaxL = 1920;
+    std::vector<uint32_t> members;
+    uint64_t M = 1;
+    for (uint32_t g : cand) {
+        if (members.size() == kFusedMax)
+            break;
+        const uint32_t down = h->groups[g * kGroupWords + 2];
+        const uint64_t need = 16ull * mfma_bf16_classes(down) * down;
+        const uint64_t l = std::lcm(M, need);
+        if (l <= kMaxL) {
+            M = l;
+            members.push_back(g);
+        }
+    }
+    if (members.size() < 2)
+        return;
+    const uint32_t L = (uint32_t)(kMaxL / M * M);
+    uint32_t fw[kGroupWords] = {kGroupFused, 1, L, 32, (uint32_t)members.size(), L, 0, 0};
+    for (size_t i = 0; i < members.size(); ++i)
+        fw[8 + i] = members[i];
+    const uint32_t fg = h->n_groups++;
+    h->groups.insert(h->groups.end(), fw, fw + kGroupWords);
+    h->sched[1].clear();
+    for (uint32_t g = 0; g < fg; ++g)
+        if (std::find(members.begin(), members.end(), g) == members.end())
+            h->sched[1].push_back(g);
+    h->sched[1].push_back(fg);
+}
+
 // Work split of a chunk of n inputs (a chunk emits at most floor(n*up/down)+1
 // outputs per channel): each block runs a stripe of tps consecutive tiles of
 // one group.
@@ -1190,9 +1306,10 @@ double tile_cost(const uint32_t *gw, uint32_t block)
 //    same input share one XCD's L2), the queues interleaved to follow the
 //    round-robin dispatch of workgroups to XCDs.
 // table: (group, stripe) per block, kNoGroup where a queue is short.
-void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table)
+void block_table(fdsp_channelizer_s *h, uint32_t n, bool in16, std::vector<uint32_t> *table)
 {
     h->tps.assign(h->n_groups, 1);
+    const std::vector<uint32_t> &sched = h->sched[fused_ok(h, in16) ? 1 : 0];
     struct Item {
         uint64_t start;
         uint32_t g, s;
@@ -1200,20 +1317,24 @@ void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table
     std::vector<Item> items;
     std::vector<uint64_t> tiles(h->n_groups), span(h->n_groups);
     double total = 0;
-    for (uint32_t g = 0; g < h->n_groups; ++g) {
+    for (uint32_t g : sched) {
         const uint32_t *gw = h->groups.data() + g * kGroupWords;
-        const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
-        tiles[g] = (most + gw[5] - 1) / gw[5];
-        span[g] = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
-        total += tiles[g] * (h->gcost.empty() ? tile_cost(gw, h->block) : h->gcost[g]);
+        if (gw[0] == kGroupFused) {
+            // global tiles of L inputs: the chunk's first and last may be partial
+            tiles[g] = n / gw[2] + 2;
+            span[g] = gw[2];
+        } else {
+            const uint64_t most = (uint64_t)n * gw[1] / gw[2] + 1;
+            tiles[g] = (most + gw[5] - 1) / gw[5];
+            span[g] = std::max<uint64_t>(1, (uint64_t)gw[5] * gw[2] / gw[1]);  // inputs per tile
+        }
+        total += tiles[g] * (h->gcost.empty() ? group_tile_cost(h, g) : h->gcost[g]);
     }
-    for (uint32_t g = 0; g < h->n_groups; ++g) {
+    for (uint32_t g : sched) {
         uint64_t tps;
         if (h->seg_len <= FDSP_TILING_STRIDED) {
             // strided split: the group's blocks sweep its tiles together, tps = block count
-            const double share =
-                tiles[g] * (h->gcost.empty() ? tile_cost(h->groups.data() + g * kGroupWords, h->block) : h->gcost[g]) /
-                total;
+            const double share = tiles[g] * (h->gcost.empty() ? group_tile_cost(h, g) : h->gcost[g]) / total;
             uint64_t blocks = std::min<uint64_t>(tiles[g],
                                                  std::max<uint64_t>(1, (uint64_t)std::llround(share * h->capacity)));
             if (h->only_group >= 0)
@@ -1264,17 +1385,18 @@ void block_table(fdsp_channelizer_s *h, uint32_t n, std::vector<uint32_t> *table
 }
 
 // device copy of the block table for chunk length n (cached per length)
-int table_for(fdsp_channelizer_s *h, uint32_t n, bool for_graph, const uint32_t **dev, uint32_t *blocks)
+int table_for(fdsp_channelizer_s *h, uint32_t n, bool in16, bool for_graph, const uint32_t **dev, uint32_t *blocks)
 {
     std::vector<uint32_t> t;
-    block_table(h, n, &t);
+    block_table(h, n, in16, &t);
+    const uint32_t key = n * 2 + (in16 ? 1 : 0);
     *blocks = (uint32_t)(t.size() / 2);
     DevBuf *buf = nullptr;
     if (for_graph) {
         buf = &h->d_table_graph;
     } else {
         for (auto &e : h->tables)
-            if (e.first == n) {
+            if (e.first == key) {
                 *dev = e.second.as<uint32_t>();
                 return FDSP_OK;
             }
@@ -1285,7 +1407,7 @@ int table_for(fdsp_channelizer_s *h, uint32_t n, bool for_graph, const uint32_t 
                 e.second.release();
             h->tables.clear();
         }
-        h->tables.emplace_back(n, DevBuf{});
+        h->tables.emplace_back(key, DevBuf{});
         buf = &h->tables.back().second;
     }
     if (int rc = buf->ensure(t.size() * sizeof(uint32_t)))
@@ -1318,8 +1440,11 @@ int groups_args(fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool d
     f.pos = device_pos ? h->d_pos.as<ChanPos>() : nullptr;
     f.p = h->p;
     f.parity = h->parity;
-    if (int rc = table_for(h, n, device_pos, &f.table, &f.blocks))
+    const bool in16 = reinterpret_cast<uintptr_t>(d_in) % 16 == 0;
+    if (int rc = table_for(h, n, in16, device_pos, &f.table, &f.blocks))
         return rc;
+    f.sched = h->sched[fused_ok(h, in16) ? 1 : 0].data();
+    f.n_sched = (uint32_t)h->sched[fused_ok(h, in16) ? 1 : 0].size();
     f.tps = h->tps.data();
     f.strided = h->seg_len <= FDSP_TILING_STRIDED;
     f.claims = h->seg_len == FDSP_TILING_CLAIM ? h->d_claims.as<uint32_t>() : nullptr;
@@ -1423,7 +1548,10 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             max_w = std::max(max_w, mfma_window(k));
         if (const char *e = std::getenv("FDSP_MFMA_FP32"))
             h->bf16_planes = e[0] == '0';
+        if (const char *e = std::getenv("FDSP_FUSE"))
+            h->fuse = e[0] != '0';
         build_groups(h, taps);
+        fuse_groups(h);
         // bf16-plane tiles start up to 7 samples before their first window
         for (uint32_t g = 0; g < h->n_groups; ++g)
             if (h->groups[g * kGroupWords + 7])
@@ -1433,9 +1561,9 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             rc = set_err(FDSP_ERANGE, "batched channelizer: %u channel groups exceed %u; use FDSP_CHAN_PLAIN",
                          h->n_groups, kGroupsInline);
         if (!rc)
-            rc = size_groups(h->groups, h->block, &h->lds_bytes);
+            rc = size_groups(h->groups, h->block, h->lds_plan);
         if (!rc)
-            rc = update_capacity(h);
+            rc = use_fusion(h, false);
         if (!rc)
             rc = h->d_groups.ensure(h->groups.size() * sizeof(uint32_t));
         if (!rc)
@@ -1858,22 +1986,18 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
     if (h->graph_exec)
         return set_err(FDSP_EBUSY, "destroy the graph before retiling");
     std::vector<uint32_t> groups = h->groups;
-    uint32_t lds = 0;
-    if (int rc = size_groups(groups, block_threads, &lds))
+    uint32_t lds[2] = {0, 0};
+    if (int rc = size_groups(groups, block_threads, lds))
         return rc;
     // launches already queued read the device plan: let them finish first
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(h->d_groups.p, groups.data(), groups.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     h->groups.swap(groups);
-    h->lds_bytes = lds;
-    for (auto &e : h->tables)
-        e.second.release();
-    h->tables.clear();
+    h->lds_plan[0] = lds[0];
+    h->lds_plan[1] = lds[1];
     h->seg_len = segment_len;
     h->block = block_threads;
-    if (int rc = update_capacity(h))
-        return rc;
-    return FDSP_OK;
+    return use_fusion(h, h->fuse_on);
 }
 
 extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t *blocks,
@@ -1884,7 +2008,7 @@ extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len
     if (h->mode != FDSP_CHAN_BATCHED)
         return set_err(FDSP_EINVAL, "plan info applies to the batched mode");
     std::vector<uint32_t> t;
-    block_table(h, (uint32_t)chunk_len, &t);
+    block_table(h, (uint32_t)chunk_len, true, &t);
     if (blocks)
         *blocks = (uint32_t)(t.size() / 2);
     if (capacity)
@@ -1911,6 +2035,10 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
     for (auto &k : h->ch)
         max_up = std::max(max_up, k.up);
     if (int rc = check_chunk(chunk_len, max_up))
+        return rc;
+    // the splits are timed unfused; fusion is a trial of its own at the end
+    HIP_TRY(hipDeviceSynchronize());
+    if (int rc = use_fusion(h, false))
         return rc;
     const uint32_t n = (uint32_t)chunk_len;
     // scratch outputs: the stream position (p, parity) is passed to every
@@ -1975,7 +2103,12 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
     if (h->n_groups > 1 && (rc = fdsp_channelizer_set_tiling(h, FDSP_TILING_STRIDED, 256)) == FDSP_OK) {
         std::vector<double> work(h->n_groups, 0.0);
         double most = 0;
-        for (uint32_t g = 0; g < h->n_groups && !rc; ++g) {
+        // the groups a launch on this input schedules
+        const std::vector<uint32_t> sched =
+            h->sched[fused_ok(h, reinterpret_cast<uintptr_t>(d_in) % 16 == 0) ? 1 : 0];
+        for (uint32_t g : sched) {
+            if (rc)
+                break;
             h->only_group = (int)g;
             float full = 0.f, half = 0.f;
             h->only_blocks = h->capacity;
@@ -1988,10 +2121,11 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
         }
         h->only_group = -1;
         if (!rc && most > 0) {
-            measured.resize(h->n_groups);
-            for (uint32_t g = 0; g < h->n_groups; ++g) {
+            measured.assign(h->n_groups, 0.0);
+            for (uint32_t g : sched) {
                 const uint32_t *gw = h->groups.data() + g * kGroupWords;
-                const uint64_t tiles = ((uint64_t)n * gw[1] / gw[2] + 1 + gw[5] - 1) / gw[5];
+                const uint64_t tiles = gw[0] == kGroupFused ? n / gw[2] + 2
+                                                            : ((uint64_t)n * gw[1] / gw[2] + 1 + gw[5] - 1) / gw[5];
                 measured[g] = std::max(work[g], 0.02 * most) / (double)std::max<uint64_t>(1, tiles);
             }
         }
@@ -2027,15 +2161,64 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
         h->gcost = measured;
     else
         h->gcost.clear();
+    // phase grouping: the best split again with the bf16-plane groups fused
+    // into shared tiles, kept if faster
+    bool best_fused = false;
+    if (!rc && h->sched[1] != h->sched[0] && reinterpret_cast<uintptr_t>(d_in) % 16 == 0 &&
+        fdsp_channelizer_set_tiling(h, best_seg, best_block) == FDSP_OK) {
+        float us = -1.f;
+        if ((rc = use_fusion(h, true)) == FDSP_OK && fused_ok(h, true)) {
+            rc = time_plan(&us);
+            if (!rc && us < best) {
+                best = us;
+                best_fused = true;
+            }
+        }
+        (void)hipStreamSynchronize(h->stream);
+        if (int r = use_fusion(h, false))
+            rc = rc ? rc : r;
+        if (trials && k < cap)
+            trials[k] = fdsp_tiling_trial{best_seg, best_block, us, (best_measured ? 1u : 0u) | 2u};
+        ++k;
+    }
     (void)hipStreamSynchronize(h->stream);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // the device table points at the scratch outputs: the next call re-uploads its own
     h->stat_valid = false;
+    h->fuse_on = !rc && best_fused;
     const int rc2 = fdsp_channelizer_set_tiling(h, rc ? seg0 : best_seg, rc ? block0 : best_block);
     if (n_trials)
         *n_trials = k;
     return rc ? rc : rc2;
+}
+
+extern "C" int fdsp_channelizer_set_fusion(fdsp_channelizer_t h, int on)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "fusion applies to the batched mode");
+    if (h->graph_exec)
+        return set_err(FDSP_EBUSY, "destroy the graph before changing the plan");
+    // launches already queued read the current tables
+    HIP_TRY(hipDeviceSynchronize());
+    return use_fusion(h, on != 0);
+}
+
+extern "C" int fdsp_channelizer_get_fusion(fdsp_channelizer_t h, int *on, uint32_t *fusable_groups)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (on)
+        *on = h->fuse_on ? 1 : 0;
+    if (fusable_groups) {
+        *fusable_groups = 0;
+        for (uint32_t g : h->sched[1])
+            if (h->groups[g * kGroupWords] == kGroupFused)
+                *fusable_groups = h->groups[g * kGroupWords + 4];
+    }
+    return FDSP_OK;
 }
 
 extern "C" int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads)

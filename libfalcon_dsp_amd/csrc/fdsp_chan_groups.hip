@@ -77,6 +77,8 @@ struct GroupsDev {
     uint32_t out16;         // every channel's output is 16-byte aligned
     uint32_t in16;          // the chunk's input is 16-byte aligned
     uint32_t tps[kGroupsInline];  // tiles per stripe, or blocks, of each group
+    uint16_t sched[kGroupsInline];  // the groups this launch schedules
+    uint32_t n_sched;
 };
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -764,6 +766,273 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
 }
 
 // ---------------------------------------------------------------------------
+// fused bf16-plane tiles: every bf16-plane group on one staged span
+// ---------------------------------------------------------------------------
+// A fused tile is L consecutive inputs of the stream (global tile T covers
+// stream inputs T*L .. T*L+L-1). Its span is staged once as bf16 planes
+// (from s0 = T*L - P - 40, even, not before the 40-sample history) and every
+// member group then computes its outputs whose anchors fall in the tile.
+
+// The rows jt .. jt+nt-1 of bf16-plane group gw, whose windows lie in the
+// staged span (s0: chunk-local index of plane sample 0); this wave's class.
+template <int kBlock>
+__device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const uint32_t *gw, uint64_t n0, uint32_t jt,
+                                        uint32_t nt, int64_t s0, const uint32_t *planes, const float2 *taps_m,
+                                        const float2 *steps_m, const ChanStatic *chans_m, const u32x4 *b_m,
+                                        uint32_t b_base, gvec2_ptr sink)
+{
+    constexpr uint32_t kW = 32, kS = 8, kWaves = kBlock / 64;
+    constexpr uint32_t kWords = mfma_stage(kBlock) / 2 * kBlock;
+    const uint32_t down = gw[2], m = gw[4];
+    const uint32_t lane = threadIdx.x & 63, wave = uniform(threadIdx.x >> 6);
+    const uint32_t nc = mfma_bf16_classes(down);
+    const uint32_t cls = wave % nc, wpc = kWaves / nc, wi = wave / nc, dn = down * nc;
+    const uint32_t crows = nt > cls ? (nt - cls + nc - 1) / nc : 0u;
+    const uint32_t subtiles = (crows + 15) >> 4;
+    if (wi >= subtiles)
+        return;  // uniform
+    const int64_t a0 = (int64_t)((n0 + jt) * down - P);  // chunk-local anchor of row jt
+    const uint32_t w0 = (uint32_t)(a0 - (int64_t)(kW - 1) - s0);  // its window's start in the span
+    const uint32_t cstart = (w0 + down * cls) & ~7u, cbase = (w0 + down * cls) & 7u;
+    bf16x8 bh[3], bl[3];
+    {
+        // the class's B operands: from the copy in LDS when the tile has the
+        // stripe's usual window offset (every tile but a chunk's first), else
+        // from the group's tables in memory
+        const u32x4 *bt = cbase == ((b_base + down * cls) & 7u)
+                              ? b_m + cls * (kBfWordsPerBase / 4)
+                              : reinterpret_cast<const u32x4 *>(a.btab + (gw[7] - 1) + cbase * kBfWordsPerBase);
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            bh[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 0) * 64 + lane]);
+            bl[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 1) * 64 + lane]);
+        }
+    }
+    uint32_t aoff[3];
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        const uint32_t kb = 4 * q + (lane >> 4);
+        const uint32_t plane = kb >= 5 && kb < 10 ? 1u : 0u;
+        const uint32_t slot = kb < 5 ? 8 * kb : kb < 10 ? 8 * (kb - 5) : 0u;
+        aoff[q] = plane * kWords * 4 + (cstart + dn * (lane & 15) + slot) * 2;
+    }
+    const uint32_t col = lane & 15;
+    const bool odd = col & 1, live = (col >> 1) < m;
+    const uint32_t cl = min(col >> 1, m - 1);
+    const ChanStatic *ch = chans_m + cl;  // LDS copy
+    const uint64_t R = ch->r;
+    const uint32_t cpp1 = ch->cpp - 1;
+    const double nu = ch->nu, inv_r = ch->inv_r;
+    const float2 wrap = ch->wrap;
+    gvec2_ptr out_t = (gvec2_ptr)ch->out + jt;
+    const bool pair16 = nc == 1 && a.out16 && (jt & 1) == 0;
+    const float2 *my_taps = taps_m + cl * kW;
+    const float2 s_row = steps_m[2 * cl], s_sub = steps_m[2 * cl + 1];
+    const uint32_t ro = 4 * (lane >> 4) + (odd ? 2 : 0);
+    const uint32_t du = 16 * wpc * dn;
+    const uint32_t k_hi = R > (uint64_t)du + dn ? (uint32_t)(R - du - dn) : 0u;
+
+    auto mma = [&](uint32_t st, floatx4 &c0, floatx4 &c1) {
+        const unsigned char *pb = reinterpret_cast<const unsigned char *>(planes) + st * 16 * dn * 2;
+        bf16x8 ah[3], al[3];
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            ah[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q]);
+            al[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q] + 2 * kWords * 4);
+        }
+        c0 = floatx4{0.f, 0.f, 0.f, 0.f};
+        c1 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh[q], c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[q], c1, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[q], c1, 0, 0, 0);
+        }
+    };
+
+    uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wi * 16 + ro)) * down, (uint32_t)R, inv_r);
+    float2 ph = nco_phasor((int64_t)k, nu);
+    // as in mfma_stripe: combine across the lane pair, phasor, store; the
+    // next sub-tile's MFMAs interleave with this work when kPipe
+    auto finish = [&](auto pipe, uint32_t st, const floatx4 &c0, const floatx4 &c1) {
+        const floatx4 acc = c0 + c1;
+        float oth[4];
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e)
+            oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
+        const float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
+        const float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
+        const uint32_t r0 = cls + nc * (st * 16 + ro);
+        float2 o0 = cmul(ph, y0);
+        float2 o1 = cmul(cmul(ph, s_row), y1);
+        const bool fast = cls + nc * (st * 16 + 15) < nt && k >= cpp1 && k < k_hi;
+        if constexpr (decltype(pipe)::value) {
+            __builtin_amdgcn_sched_group_barrier(0x100, kS, 0);
+#pragma unroll
+            for (uint32_t i = 0; i < 2 * kS; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            }
+        }
+        if (__builtin_amdgcn_ballot_w64(!fast) != 0) {
+            if (!fast) {
+#pragma unroll
+                for (uint32_t e = 0; e < 2; ++e) {
+                    const uint32_t row = r0 + e * nc;
+                    const uint32_t kk = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)row * down, (uint32_t)R, inv_r);
+                    float2 y = e ? y1 : y0;
+                    if (kk < cpp1) {
+                        const uint32_t split = kW - 1 - kk;
+                        const uint32_t rw = min(row, nt - 1);  // rows past nt store nothing
+                        float2 part = make_float2(0.f, 0.f);
+                        for (uint32_t t = 0; t < split; ++t)
+                            cmac(part, my_taps[t], bf_sample<kWords>(planes, w0 + rw * down + t));
+                        cmac(y, make_float2(wrap.x - 1.f, wrap.y), part);
+                    }
+                    const float2 o = cmul(nco_phasor((int64_t)kk, nu), y);
+                    if (e)
+                        o1 = o;
+                    else
+                        o0 = o;
+                }
+                k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(r0 + 16 * wpc * nc) * down, (uint32_t)R, inv_r);
+                ph = nco_phasor((int64_t)k, nu);
+            } else {
+                k += du;
+                ph = cmul(ph, s_sub);
+            }
+            (live && r0 < nt ? out_t + r0 : sink)[0] = fvec2{o0.x, o0.y};
+            (live && r0 + nc < nt ? out_t + r0 + nc : sink + 1)[0] = fvec2{o1.x, o1.y};
+        } else {
+            k += du;
+            ph = cmul(ph, s_sub);
+            if (pair16) {
+                ((gvec4_ptr)(live ? out_t + r0 : sink))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+            } else {
+                (live ? out_t + r0 : sink)[0] = fvec2{o0.x, o0.y};
+                (live ? out_t + r0 + nc : sink + 1)[0] = fvec2{o1.x, o1.y};
+            }
+        }
+    };
+    floatx4 pa, pb;
+    mma(wi, pa, pb);
+    for (uint32_t st = wi;;) {
+        const uint32_t sn = st + wpc;
+        if (sn >= subtiles) {
+            finish(std::false_type{}, st, pa, pb);
+            break;
+        }
+        floatx4 na, nb;
+        mma(sn, na, nb);
+        finish(std::true_type{}, st, pa, pb);
+        pa = na;
+        pb = nb;
+        st = sn;
+    }
+}
+
+// the stream's global fused tiles that reach into this chunk: T0 ..
+__device__ __forceinline__ uint32_t fused_tiles(uint64_t P, uint32_t n, uint32_t L)
+{
+    return (uint32_t)((P + n - 1) / L - P / L + 1);
+}
+
+template <int kBlock>
+__device__ __forceinline__ void fused_stripe(const GroupsDev &a, const float2 *__restrict__ hist, uint64_t P,
+                                             const uint32_t *fw, TileSeq seq, uint32_t *slot, unsigned char *smem)
+{
+    constexpr uint32_t kW = 32, kWaves = kBlock / 64;
+    constexpr uint32_t kPairs = mfma_stage(kBlock) / 2;
+    const uint32_t L = fw[2], nm = fw[4];
+    // LDS: members' B operands (one 6 KB table per residue class, for the
+    // stripe's usual window offset), the planes, taps, NCO steps, channels
+    u32x4 *btl = reinterpret_cast<u32x4 *>(smem);
+    uint32_t nb = 0;  // B tables
+    for (uint32_t i = 0; i < nm; ++i)
+        nb += mfma_bf16_classes(a.groups[uniform(fw[8 + i]) * kGroupWords + 2]);
+    uint32_t *planes = reinterpret_cast<uint32_t *>(btl + nb * (kBfWordsPerBase / 4));
+    float2 *taps_l = reinterpret_cast<float2 *>(planes + mfma_stage(kBlock) * kBlock * 2);
+    float2 *steps = taps_l + nm * kMfmaMaxChan * kW;
+    ChanStatic *chl = reinterpret_cast<ChanStatic *>(steps + nm * kMfmaMaxChan * 2);
+    gvec2_ptr sink = (gvec2_ptr)(a.sink + 2 * ((blockIdx.x % (kSinkWords / (2 * kBlock))) * kBlock + threadIdx.x));
+    // a full tile's first row has its window at w_std = 9 or 10 samples into
+    // the span (the span starts 40 samples before the tile, rounded down to
+    // an even index)
+    const uint32_t w_std = 9u + (uint32_t)(P & 1u);
+    // members' taps (zero-padded in front), channels, B tables, and NCO steps:
+    // one row of the member's class (down*nc inputs) and one sub-tile of its
+    // wave (16*wpc rows)
+    for (uint32_t i = 0, bi = 0; i < nm; ++i) {
+        const uint32_t *gw = a.groups + uniform(fw[8 + i]) * kGroupWords;
+        const uint32_t m = gw[4], down = gw[2];
+        for (uint32_t e = threadIdx.x; e < m * (sizeof(ChanStatic) / 4); e += kBlock)
+            reinterpret_cast<uint32_t *>(chl + i * kMfmaMaxChan)[e] =
+                reinterpret_cast<const uint32_t *>(a.chans + gw[8 + e / (sizeof(ChanStatic) / 4)])[e % (sizeof(ChanStatic) / 4)];
+        const uint32_t nc = mfma_bf16_classes(down);
+        for (uint32_t c = 0; c < nc; ++c, ++bi) {
+            const u32x4 *src =
+                reinterpret_cast<const u32x4 *>(a.btab + (gw[7] - 1) + ((w_std + down * c) & 7u) * kBfWordsPerBase);
+            for (uint32_t e = threadIdx.x; e < kBfWordsPerBase / 4; e += kBlock)
+                btl[bi * (kBfWordsPerBase / 4) + e] = src[e];
+        }
+        for (uint32_t e = threadIdx.x; e < m * kW; e += kBlock) {
+            const ChanStatic &c = a.chans[gw[8 + e / kW]];
+            const uint32_t t = e % kW, pad = kW - c.cpp;
+            taps_l[i * kMfmaMaxChan * kW + e] = t < pad ? make_float2(0.f, 0.f) : a.taps[c.taps_off + t - pad];
+        }
+        if (threadIdx.x < m) {
+            const double nu = a.chans[gw[8 + threadIdx.x]].nu;
+            const uint32_t nc = mfma_bf16_classes(down);
+            steps[(i * kMfmaMaxChan + threadIdx.x) * 2] = nco_phasor((int64_t)(down * nc), nu);
+            steps[(i * kMfmaMaxChan + threadIdx.x) * 2 + 1] = nco_phasor((int64_t)(16 * (kWaves / nc) * down * nc), nu);
+        }
+    }
+    const uint64_t T0 = P / L;
+    const uint32_t tiles = fused_tiles(P, a.n, L);
+    // span start of tile t (relative to T0): 40 samples before the tile, even,
+    // and not before the history
+    auto span0 = [&](uint32_t t) -> int64_t {
+        int64_t s0 = (int64_t)((T0 + t) * L) - (int64_t)P - 40;
+        s0 = s0 < -40 ? -40 : s0;
+        return s0 & ~(int64_t)1;
+    };
+    uint32_t tb = uniform(*slot);
+    u32x4 vb[kPairs];
+    bf_load<kBlock>(a, hist, span0(tb), vb);
+    for (;;) {
+        const int64_t s0 = span0(tb);
+        __syncthreads();  // the previous tile's reads of LDS (and of *slot) are done; taps and steps written
+        bf_commit<kBlock>(vb, planes);
+        seq.claim(slot);
+        __syncthreads();
+        const uint32_t tn = uniform(*slot);
+        if (tn < tiles)
+            bf_load<kBlock>(a, hist, span0(tn), vb);
+        const uint64_t Tg = T0 + tb;
+        for (uint32_t i = 0, bi = 0; i < nm; ++i) {
+            const uint32_t *gw = a.groups + uniform(fw[8 + i]) * kGroupWords;
+            const uint32_t bi_i = bi;
+            bi += mfma_bf16_classes(gw[2]);
+            const uint32_t down = gw[2];
+            const double inv_down = a.chans[gw[8]].inv_down;
+            const uint64_t n0 = cdiv(P, down, inv_down);
+            const uint32_t count = (uint32_t)(cdiv(P + a.n, down, inv_down) - n0);
+            // outputs whose stream anchors (n0 + j)*down fall in the tile
+            const uint64_t jl = cdiv(Tg * L, down, inv_down), jh = cdiv((Tg + 1) * L, down, inv_down);
+            const uint32_t jt = jl > n0 ? (uint32_t)min<uint64_t>(jl - n0, count) : 0u;
+            const uint32_t je = jh > n0 ? (uint32_t)min<uint64_t>(jh - n0, count) : 0u;
+            if (je > jt)
+                bf_rows<kBlock>(a, P, gw, n0, jt, je - jt, s0, planes, taps_l + i * kMfmaMaxChan * kW,
+                                steps + i * kMfmaMaxChan * 2, chl + i * kMfmaMaxChan, btl + bi_i * (kBfWordsPerBase / 4),
+                                w_std, sink);
+        }
+        if (tn >= tiles)
+            break;  // uniform
+        tb = tn;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // one channel, dense (VALU, transposed span)
 // ---------------------------------------------------------------------------
 constexpr uint32_t kOpl = kVDenseOpl;
@@ -899,6 +1168,15 @@ __device__ __forceinline__ void run_group(const GroupsDev &a, const float2 *__re
 {
     const uint32_t *gw = a.groups + g * kGroupWords;
     const uint32_t kind = gw[0], up = gw[1], down = gw[2], rows = gw[5];
+    if (kind == kGroupFused) {
+        seq.tiles = fused_tiles(P, a.n, down);
+        seq.start(blockIdx.x, slot + 1);
+        seq.claim(slot);
+        __syncthreads();
+        if (uniform(*slot) < seq.tiles)
+            fused_stripe<kBlock>(a, hist, P, gw, seq, slot, smem);
+        return;
+    }
     const double inv_down = a.chans[gw[8]].inv_down;
     const uint64_t n0 = cdiv(P * up, down, inv_down);
     const uint32_t count = (uint32_t)(cdiv((P + a.n) * up, down, inv_down) - n0);
@@ -996,7 +1274,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWide ? 
     }
     // claimed tiles: start with this block's group, then help the others;
     // every block reaches the finished-block count below
-    for (uint32_t k = 0, g = g0 < a.n_groups ? g0 : 0; k < a.n_groups; ++k, g = g + 1 == a.n_groups ? 0 : g + 1) {
+    // this block's group's place in the schedule (a padding entry starts at 0)
+    uint32_t i0 = 0;
+    for (uint32_t i = 0; i < a.n_sched; ++i)
+        if (a.sched[i] == g0)
+            i0 = i;
+    for (uint32_t k = 0, i = i0; k < a.n_sched; ++k, i = i + 1 == a.n_sched ? 0 : i + 1) {
+        const uint32_t g = a.sched[i];
         __syncthreads();  // the previous group's reads of LDS are done
         run_group<kBlock, kWide>(a, hist, P, g, TileSeq{a.claims + g * kClaimShards * kClaimPitch}, s_tile,
                                  smem);
@@ -1049,6 +1333,9 @@ hipError_t launch_chan_groups(const GroupsArgs &f, hipStream_t s)
     d.parity = f.parity;
     d.groups = f.groups;
     d.n_groups = f.n_groups;
+    d.n_sched = f.n_sched;
+    for (uint32_t i = 0; i < f.n_sched && i < kGroupsInline; ++i)
+        d.sched[i] = (uint16_t)f.sched[i];
     d.table = f.table;
     d.strided = f.strided ? 1u : 0u;
     d.claims = f.claims;

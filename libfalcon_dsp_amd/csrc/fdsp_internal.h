@@ -121,7 +121,11 @@ enum : uint32_t {
     kGroupVDense = 1,  // one channel, up 1, down < cpp: VALU over a transposed span
     kGroupVSparse = 2, // one channel, up 1, down >= cpp: VALU over gathered windows
     kGroupGeneric = 3, // one channel, up > 1: VALU with per-output filter phase
+    kGroupFused = 4,   // bf16-plane MFMA groups sharing staged spans: [2] L inputs per
+                       // tile, [4] member count, [8 ..] member group indices
 };
+// member groups of a fused record at most
+constexpr uint32_t kFusedMax = 4;
 constexpr uint32_t kGroupWords = 16;
 constexpr uint32_t kMfmaMaxChan = 8;
 constexpr uint32_t kMfmaMinChan = 3;
@@ -191,6 +195,8 @@ struct GroupsArgs {
     float2 *sink;               // device: kSinkWords samples, written and never read
     bool out16;                 // every channel's output is 16-byte aligned (paired-row stores)
     bool in16;                  // the chunk's input is 16-byte aligned (bf16-plane staging)
+    const uint32_t *sched;      // host: the groups this launch schedules (claimed tiles visit only these)
+    uint32_t n_sched;
     uint32_t *claims;           // device: kClaimWords zeroed words, or null: blocks then claim
                                 //   tiles one by one, starting in their table group
 };

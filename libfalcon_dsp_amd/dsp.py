@@ -432,7 +432,17 @@ class MultiRateChannelizer(_Handle):
         k = C.c_size_t()
         check(lib().fdsp_channelizer_autotune(self.h, d_in.at(in_offset), chunk_len, reps, tr, cap, C.byref(k)))
         return [{"segment_len": t.segment_len, "block_threads": t.block_threads, "us": round(t.us, 2),
-                 "measured_weights": bool(t.measured_weights)} for t in tr[: k.value]]
+                 "measured_weights": bool(t.flags & 1), "fused": bool(t.flags & 2)} for t in tr[: k.value]]
+
+    def set_fusion(self, on: bool):
+        """Phase grouping: bf16-plane groups share staged spans (fdsp_channelizer_set_fusion)."""
+        check(lib().fdsp_channelizer_set_fusion(self.h, 1 if on else 0))
+
+    def get_fusion(self):
+        """(on, number of groups in the fused record)."""
+        on, k = C.c_int(), C.c_uint32()
+        check(lib().fdsp_channelizer_get_fusion(self.h, C.byref(on), C.byref(k)))
+        return bool(on.value), k.value
 
     def plan_info(self, chunk_len: int) -> dict:
         """Work split of the batched kernel for a chunk length (diagnostics)."""

@@ -199,6 +199,38 @@ def test_workload_chan_32_slice():
             assert_close_rms(ys[c], rs[c], xr, what=f"chan {c}")
 
 
+def test_fused_groups_match_oracle():
+    # two bf16-plane groups (down 8 and down 20) share staged spans when fused;
+    # chunks of every size, including ones shorter than a tile, and an
+    # unaligned input start (which takes the unfused form)
+    w = Chan32(n=1 << 19)
+    chans = [c for c in w.channels() if c["down_rate"] in (8, 20)]
+    x = w.input()
+    g = fd.MultiRateChannelizer(w.fs, chans)
+    on, members = g.get_fusion()
+    assert not on and members == 2
+    g.set_fusion(True)
+    ref = O.Channelizer(w.fs, chans)
+    xr = rms(x)
+    for lo, hi in chunks(x.size, [100_003, 1_920, 7, 65_536, 1]):
+        ys, rs = g.apply(x[lo:hi]), ref.apply(x[lo:hi])
+        for c in range(len(chans)):
+            assert ys[c].size == rs[c].size
+            assert_close_rms(ys[c], rs[c], xr, what=f"chan {c} [{lo},{hi})")
+    # device form on an odd sample offset (8-byte aligned input)
+    n = 70_001
+    d = fd.DeviceBuffer.from_array(x[: n + 1])
+    g2 = fd.MultiRateChannelizer(w.fs, chans)
+    g2.set_fusion(True)
+    outs = [fd.DeviceBuffer(m) for m in g2.max_out(n)]
+    counts = g2.apply_device(d, n, outs, in_offset=1)
+    fd.device_sync()
+    rs = O.Channelizer(w.fs, chans).apply(x[1: n + 1])
+    for c in range(len(chans)):
+        assert counts[c] == rs[c].size
+        assert_close_rms(outs[c].download(counts[c]), rs[c], xr, what=f"unaligned chan {c}")
+
+
 def test_autotune_keeps_stream_state():
     # tuning launches run on a chunk of the caller's data without moving the
     # stream: the chunks around it still continue the oracle's stream
