@@ -1022,7 +1022,8 @@ int size_groups(std::vector<uint32_t> &groups, uint32_t block, uint32_t lds[2])
     lds[0] = (uint32_t)budget;
     for (size_t g = 0; g < G; ++g)
         if (groups[g * kGroupWords] == kGroupFused) {
-            // B tables (6 KB per member class), planes, taps, NCO steps, channels
+            // B tables (6 KB per member class), planes, taps, NCO steps,
+            // channels, and (in the 512 spare bytes) 32-byte member descriptors
             const uint32_t *fw = groups.data() + g * kGroupWords;
             size_t nb = 0;
             for (uint32_t i = 0; i < fw[4]; ++i)

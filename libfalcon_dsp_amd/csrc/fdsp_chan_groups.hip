@@ -775,19 +775,30 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
 
 // The rows jt .. jt+nt-1 of bf16-plane group gw, whose windows lie in the
 // staged span (s0: chunk-local index of plane sample 0); this wave's class.
+// A fused record's member group, as its stripe uses it on every tile
+struct FusedMember {
+    uint64_t n0;        // the chunk's first output of the group
+    uint32_t count;     // the chunk's outputs of the group
+    uint32_t per_tile;  // outputs per fused tile (L / down)
+    uint32_t down, m;   // decimation, channels
+    uint32_t lnc;       // log2 of the residue classes
+    uint32_t btab;      // offset of the group's B tables in btab
+};
+
 template <int kBlock>
-__device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const uint32_t *gw, uint64_t n0, uint32_t jt,
+__device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const FusedMember &fm, uint32_t jt,
                                         uint32_t nt, int64_t s0, const uint32_t *planes, const float2 *taps_m,
                                         const float2 *steps_m, const ChanStatic *chans_m, const u32x4 *b_m,
                                         uint32_t b_base, gvec2_ptr sink)
 {
     constexpr uint32_t kW = 32, kS = 8, kWaves = kBlock / 64;
     constexpr uint32_t kWords = mfma_stage(kBlock) / 2 * kBlock;
-    const uint32_t down = gw[2], m = gw[4];
+    const uint32_t down = fm.down, m = fm.m, lnc = fm.lnc;
+    const uint64_t n0 = fm.n0;
     const uint32_t lane = threadIdx.x & 63, wave = uniform(threadIdx.x >> 6);
-    const uint32_t nc = mfma_bf16_classes(down);
-    const uint32_t cls = wave % nc, wpc = kWaves / nc, wi = wave / nc, dn = down * nc;
-    const uint32_t crows = nt > cls ? (nt - cls + nc - 1) / nc : 0u;
+    const uint32_t nc = 1u << lnc;
+    const uint32_t cls = wave & (nc - 1), wpc = kWaves >> lnc, wi = wave >> lnc, dn = down << lnc;
+    const uint32_t crows = nt > cls ? (nt - cls + nc - 1) >> lnc : 0u;
     const uint32_t subtiles = (crows + 15) >> 4;
     if (wi >= subtiles)
         return;  // uniform
@@ -795,17 +806,24 @@ __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const ui
     const uint32_t w0 = (uint32_t)(a0 - (int64_t)(kW - 1) - s0);  // its window's start in the span
     const uint32_t cstart = (w0 + down * cls) & ~7u, cbase = (w0 + down * cls) & 7u;
     bf16x8 bh[3], bl[3];
-    {
-        // the class's B operands: from the copy in LDS when the tile has the
-        // stripe's usual window offset (every tile but a chunk's first), else
-        // from the group's tables in memory
-        const u32x4 *bt = cbase == ((b_base + down * cls) & 7u)
-                              ? b_m + cls * (kBfWordsPerBase / 4)
-                              : reinterpret_cast<const u32x4 *>(a.btab + (gw[7] - 1) + cbase * kBfWordsPerBase);
+    // the class's B operands: from the copy in LDS when the tile has the
+    // stripe's usual window offset (every tile but a chunk's first), else
+    // from the group's tables in memory (uniform branch)
+    if (cbase == ((b_base + down * cls) & 7u)) {
+        const u32x4 *bt = b_m + cls * (kBfWordsPerBase / 4);
 #pragma unroll
         for (uint32_t q = 0; q < 3; ++q) {
             bh[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 0) * 64 + lane]);
             bl[q] = __builtin_bit_cast(bf16x8, bt[(q * 2 + 1) * 64 + lane]);
+        }
+    } else {
+        // (buffer loads: a pointer that could be either would make flat loads of both)
+        const __amdgpu_buffer_rsrc_t bt = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(a.btab + fm.btab + cbase * kBfWordsPerBase), 0, (int)(kBfWordsPerBase * 4), kBufferFlags);
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            bh[q] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(bt, (int)(((q * 2 + 0) * 64 + lane) * 16), 0, 0));
+            bl[q] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(bt, (int)(((q * 2 + 1) * 64 + lane) * 16), 0, 0));
         }
     }
     uint32_t aoff[3];
@@ -954,6 +972,7 @@ __device__ __forceinline__ void fused_stripe(const GroupsDev &a, const float2 *_
     float2 *taps_l = reinterpret_cast<float2 *>(planes + mfma_stage(kBlock) * kBlock * 2);
     float2 *steps = taps_l + nm * kMfmaMaxChan * kW;
     ChanStatic *chl = reinterpret_cast<ChanStatic *>(steps + nm * kMfmaMaxChan * 2);
+    FusedMember *mem = reinterpret_cast<FusedMember *>(chl + nm * kMfmaMaxChan);
     gvec2_ptr sink = (gvec2_ptr)(a.sink + 2 * ((blockIdx.x % (kSinkWords / (2 * kBlock))) * kBlock + threadIdx.x));
     // a full tile's first row has its window at w_std = 9 or 10 samples into
     // the span (the span starts 40 samples before the tile, rounded down to
@@ -979,6 +998,13 @@ __device__ __forceinline__ void fused_stripe(const GroupsDev &a, const float2 *_
             const ChanStatic &c = a.chans[gw[8 + e / kW]];
             const uint32_t t = e % kW, pad = kW - c.cpp;
             taps_l[i * kMfmaMaxChan * kW + e] = t < pad ? make_float2(0.f, 0.f) : a.taps[c.taps_off + t - pad];
+        }
+        if (threadIdx.x == 0) {
+            // L is a multiple of down: the tile's outputs are per_tile whole rows
+            const double inv_down = a.chans[gw[8]].inv_down;
+            const uint64_t n0 = cdiv(P, down, inv_down);
+            mem[i] = FusedMember{n0, (uint32_t)(cdiv(P + a.n, down, inv_down) - n0), L / down, down, m,
+                                 3u - min(3u, (uint32_t)__builtin_ctz(down)), gw[7] - 1};
         }
         if (threadIdx.x < m) {
             const double nu = a.chans[gw[8 + threadIdx.x]].nu;
@@ -1010,19 +1036,25 @@ __device__ __forceinline__ void fused_stripe(const GroupsDev &a, const float2 *_
             bf_load<kBlock>(a, hist, span0(tn), vb);
         const uint64_t Tg = T0 + tb;
         for (uint32_t i = 0, bi = 0; i < nm; ++i) {
-            const uint32_t *gw = a.groups + uniform(fw[8 + i]) * kGroupWords;
+            FusedMember fm;
+            {
+                const FusedMember &f = mem[i];
+                fm.n0 = ((uint64_t)uniform((uint32_t)(f.n0 >> 32)) << 32) | uniform((uint32_t)f.n0);
+                fm.count = uniform(f.count);
+                fm.per_tile = uniform(f.per_tile);
+                fm.down = uniform(f.down);
+                fm.m = uniform(f.m);
+                fm.lnc = uniform(f.lnc);
+                fm.btab = uniform(f.btab);
+            }
             const uint32_t bi_i = bi;
-            bi += mfma_bf16_classes(gw[2]);
-            const uint32_t down = gw[2];
-            const double inv_down = a.chans[gw[8]].inv_down;
-            const uint64_t n0 = cdiv(P, down, inv_down);
-            const uint32_t count = (uint32_t)(cdiv(P + a.n, down, inv_down) - n0);
+            bi += 1u << fm.lnc;
             // outputs whose stream anchors (n0 + j)*down fall in the tile
-            const uint64_t jl = cdiv(Tg * L, down, inv_down), jh = cdiv((Tg + 1) * L, down, inv_down);
-            const uint32_t jt = jl > n0 ? (uint32_t)min<uint64_t>(jl - n0, count) : 0u;
-            const uint32_t je = jh > n0 ? (uint32_t)min<uint64_t>(jh - n0, count) : 0u;
+            const uint64_t jl = Tg * fm.per_tile, jh = jl + fm.per_tile;
+            const uint32_t jt = jl > fm.n0 ? (uint32_t)min<uint64_t>(jl - fm.n0, fm.count) : 0u;
+            const uint32_t je = jh > fm.n0 ? (uint32_t)min<uint64_t>(jh - fm.n0, fm.count) : 0u;
             if (je > jt)
-                bf_rows<kBlock>(a, P, gw, n0, jt, je - jt, s0, planes, taps_l + i * kMfmaMaxChan * kW,
+                bf_rows<kBlock>(a, P, fm, jt, je - jt, s0, planes, taps_l + i * kMfmaMaxChan * kW,
                                 steps + i * kMfmaMaxChan * 2, chl + i * kMfmaMaxChan, btl + bi_i * (kBfWordsPerBase / 4),
                                 w_std, sink);
         }
