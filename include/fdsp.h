@@ -251,8 +251,8 @@ int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t 
  * 4096 ... 65536 inputs at 256 threads and of 16384 at 512), reps launches
  * each after a warm-up, and keeps the fastest; then it times that split with
  * fusion on (bit 1 of the trial flags) and keeps it if faster. The launches
- * write scratch
- * outputs and leave the stream position and history as they were. trials
+ * write scratch outputs and leave the stream position and history as they
+ * were. trials
  * (optional, cap entries, up to 17) receives every candidate's average kernel
  * time (us < 0: the split does not fit this channel set). */
 typedef struct fdsp_tiling_trial {
