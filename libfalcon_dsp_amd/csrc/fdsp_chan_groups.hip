@@ -647,13 +647,17 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
         const uint32_t tn = uniform(*slot);
         if (tn < tiles)
             load(tn);
-        // rows of this wave's class in the tile, and their sub-tiles of 16
+        // rows of this wave's class in the tile, and their sub-tiles of 16;
+        // the class's waves take sub-tiles wt, wt + wpc, ... with wt rotating
+        // from tile to tile, so a tile's odd sub-tile does not always fall to
+        // the same wave (and SIMD)
         const uint32_t crows = nt > cls ? (nt - cls + nc - 1) / nc : 0u;
         const uint32_t subtiles = (crows + 15) >> 4;
-        if (wi >= subtiles)
+        const uint32_t wt = (wi + tb) & (wpc - 1);
+        if (wt >= subtiles)
             goto next_tile;  // uniform
         {
-            uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wi * 16 + ro)) * down, (uint32_t)R,
+            uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wt * 16 + ro)) * down, (uint32_t)R,
                                            inv_r);
             float2 ph = nco_phasor((int64_t)k, nu);
             gvec2_ptr out_t = out + j0;
@@ -743,8 +747,8 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                 }
             };
             floatx4 pa, pb;  // the pending sub-tile's chains
-            mma(wi, pa, pb);
-            for (uint32_t st = wi;;) {
+            mma(wt, pa, pb);
+            for (uint32_t st = wt;;) {
                 const uint32_t sn = st + wpc;
                 if (sn >= subtiles) {
                     finish(std::false_type{}, st, pa, pb);
@@ -787,7 +791,7 @@ struct FusedMember {
 
 template <int kBlock>
 __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const FusedMember &fm, uint32_t jt,
-                                        uint32_t nt, int64_t s0, const uint32_t *planes, const float2 *taps_m,
+                                        uint32_t nt, uint32_t rot, int64_t s0, const uint32_t *planes, const float2 *taps_m,
                                         const float2 *steps_m, const ChanStatic *chans_m, const u32x4 *b_m,
                                         uint32_t b_base, gvec2_ptr sink)
 {
@@ -800,7 +804,8 @@ __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const Fu
     const uint32_t cls = wave & (nc - 1), wpc = kWaves >> lnc, wi = wave >> lnc, dn = down << lnc;
     const uint32_t crows = nt > cls ? (nt - cls + nc - 1) >> lnc : 0u;
     const uint32_t subtiles = (crows + 15) >> 4;
-    if (wi >= subtiles)
+    const uint32_t wt = (wi + rot) & (wpc - 1);  // rotating sub-tile start, as in mfma_stripe
+    if (wt >= subtiles)
         return;  // uniform
     const int64_t a0 = (int64_t)((n0 + jt) * down - P);  // chunk-local anchor of row jt
     const uint32_t w0 = (uint32_t)(a0 - (int64_t)(kW - 1) - s0);  // its window's start in the span
@@ -868,7 +873,7 @@ __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const Fu
         }
     };
 
-    uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wi * 16 + ro)) * down, (uint32_t)R, inv_r);
+    uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wt * 16 + ro)) * down, (uint32_t)R, inv_r);
     float2 ph = nco_phasor((int64_t)k, nu);
     // as in mfma_stripe: combine across the lane pair, phasor, store; the
     // next sub-tile's MFMAs interleave with this work when kPipe
@@ -933,8 +938,8 @@ __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const Fu
         }
     };
     floatx4 pa, pb;
-    mma(wi, pa, pb);
-    for (uint32_t st = wi;;) {
+    mma(wt, pa, pb);
+    for (uint32_t st = wt;;) {
         const uint32_t sn = st + wpc;
         if (sn >= subtiles) {
             finish(std::false_type{}, st, pa, pb);
@@ -1054,7 +1059,7 @@ __device__ __forceinline__ void fused_stripe(const GroupsDev &a, const float2 *_
             const uint32_t jt = jl > fm.n0 ? (uint32_t)min<uint64_t>(jl - fm.n0, fm.count) : 0u;
             const uint32_t je = jh > fm.n0 ? (uint32_t)min<uint64_t>(jh - fm.n0, fm.count) : 0u;
             if (je > jt)
-                bf_rows<kBlock>(a, P, fm, jt, je - jt, s0, planes, taps_l + i * kMfmaMaxChan * kW,
+                bf_rows<kBlock>(a, P, fm, jt, je - jt, tb, s0, planes, taps_l + i * kMfmaMaxChan * kW,
                                 steps + i * kMfmaMaxChan * 2, chl + i * kMfmaMaxChan, btl + bi_i * (kBfWordsPerBase / 4),
                                 w_std, sink);
         }
