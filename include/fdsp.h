@@ -36,6 +36,11 @@ extern "C" {
 #define FDSP_EAGAIN   -7  /* serving: not admitted yet, retry later */
 
 typedef struct fdsp_cf32 { float re, im; } fdsp_cf32;
+/* complex int16 sample (std::complex<int16_t>): read exactly as float; results
+ * written as int16 are truncated toward zero, as the reference's float ->
+ * int16_t conversions are, and saturated outside the int16 range (where the
+ * reference's conversion is undefined) */
+typedef struct fdsp_ci16 { int16_t re, im; } fdsp_ci16;
 typedef void *fdsp_stream_t; /* hipStream_t */
 typedef void *fdsp_event_t;  /* hipEvent_t */
 
@@ -95,6 +100,14 @@ int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fd
 /* output of shift c starts at d_out + c * out_stride (in samples) */
 int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t n,
                                  fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream);
+/* ci16 in, ci16 out. Replaces falcon_dsp_freq_shift::apply(vector<complex<int16_t>>&, ...)
+ * (reference src/transform/falcon_dsp_freq_shift.cc:133-162) and
+ * falcon_dsp_freq_shift_cuda::apply(ci16) (falcon_dsp_freq_shift_cuda.cu:308-329):
+ * the shift of the float-converted input, each part truncated back to int16.
+ * Shares the stream position with the cf32 calls. */
+int fdsp_freq_shift_apply_ci16(fdsp_freq_shift_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *const *outs);
+int fdsp_freq_shift_apply_ci16_device(fdsp_freq_shift_t h, const fdsp_ci16 *d_in, size_t n,
+                                      fdsp_ci16 *d_out, size_t out_stride, fdsp_stream_t stream);
 
 /* ---- polyphase rational resampler ----
  * Replaces falcon_dsp_polyphase_resampler::{initialize, apply(cf32),
@@ -131,6 +144,14 @@ int fdsp_fir_reset(fdsp_fir_t h);
 int fdsp_fir_apply(fdsp_fir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out);
 int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
                           fdsp_stream_t stream);
+/* ci16 in, ci16 out. Replaces falcon_dsp_fir_filter::apply(vector<complex<int16_t>>&, ...)
+ * (reference src/transform/falcon_dsp_fir_filter.cc:151-175) and
+ * falcon_dsp_fir_filter_cuda::apply(ci16) (falcon_dsp_fir_filter_cuda.cu:233-254):
+ * the filter over the float-converted input, outputs truncated to int16. The
+ * history stays float and is shared with the cf32 calls. */
+int fdsp_fir_apply_ci16(fdsp_fir_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out);
+int fdsp_fir_apply_ci16_device(fdsp_fir_t h, const fdsp_ci16 *d_in, size_t n, fdsp_ci16 *d_out,
+                               fdsp_stream_t stream);
 
 /* ---- FIR bank: many independent channels, real taps each ----
  * The multi-channel form of falcon_dsp_fir_filter::apply (reference
@@ -157,6 +178,14 @@ int fdsp_discriminator_apply(fdsp_discriminator_t h, const fdsp_cf32 *in, size_t
                              size_t *n_out);
 int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fdsp_cf32 *d_in, size_t n, float *d_out,
                                     size_t *n_out, fdsp_stream_t stream);
+/* ci16 in. Replaces falcon_dsp_polar_discriminator::apply(const vector<complex<int16_t>>&,
+ * vector<float>&) (reference src/transform/falcon_dsp_polar_discriminator.cc:117-132):
+ * the float-converted input through the same discriminator; the carried
+ * sample is shared with the cf32 calls. */
+int fdsp_discriminator_apply_ci16(fdsp_discriminator_t h, const fdsp_ci16 *in, size_t n, float *out,
+                                  size_t *n_out);
+int fdsp_discriminator_apply_ci16_device(fdsp_discriminator_t h, const fdsp_ci16 *d_in, size_t n, float *d_out,
+                                         size_t *n_out, fdsp_stream_t stream);
 
 /* ---- multi-rate channelizer ----
  * Replaces falcon_dsp_multi_rate_channelizer{,_cuda}::{initialize, apply, reset_state}

@@ -134,6 +134,8 @@ _SIGS = [
     ("fdsp_freq_shift_get_state", _i, [_vp, _u32, _P(_u64)]),
     ("fdsp_freq_shift_apply", _i, [_vp, _vp, _sz, _P(_vp)]),
     ("fdsp_freq_shift_apply_device", _i, [_vp, _vp, _sz, _vp, _sz, _vp]),
+    ("fdsp_freq_shift_apply_ci16", _i, [_vp, _vp, _sz, _P(_vp)]),
+    ("fdsp_freq_shift_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _sz, _vp]),
     ("fdsp_resampler_create", _i, [_u32, _u32, _vp, _u32, _P(_vp)]),
     ("fdsp_resampler_destroy", _i, [_vp]),
     ("fdsp_resampler_reset", _i, [_vp]),
@@ -147,6 +149,8 @@ _SIGS = [
     ("fdsp_fir_reset", _i, [_vp]),
     ("fdsp_fir_apply", _i, [_vp, _vp, _sz, _vp]),
     ("fdsp_fir_apply_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_fir_apply_ci16", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_fir_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_fir_bank_create", _i, [_u32, _u32, _vp, _P(_vp)]),
     ("fdsp_fir_bank_destroy", _i, [_vp]),
     ("fdsp_fir_bank_reset", _i, [_vp]),
@@ -156,6 +160,8 @@ _SIGS = [
     ("fdsp_discriminator_reset", _i, [_vp]),
     ("fdsp_discriminator_apply", _i, [_vp, _vp, _sz, _vp, _P(_sz)]),
     ("fdsp_discriminator_apply_device", _i, [_vp, _vp, _sz, _vp, _P(_sz), _vp]),
+    ("fdsp_discriminator_apply_ci16", _i, [_vp, _vp, _sz, _vp, _P(_sz)]),
+    ("fdsp_discriminator_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _P(_sz), _vp]),
     ("fdsp_channelizer_create", _i, [_u32, _P(ChannelDesc), _u32, _i, _P(_vp)]),
     ("fdsp_channelizer_destroy", _i, [_vp]),
     ("fdsp_channelizer_reset", _i, [_vp]),

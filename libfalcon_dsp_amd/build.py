@@ -57,15 +57,20 @@ def build_dropin():
 
 DROPIN_TEST_SRC = os.path.join(ROOT, "tests", "dropin", "test_dropin.cc")
 DROPIN_TEST_BIN = os.path.join(ROOT, "tests", "dropin", "test_dropin")
+FILES_TEST_SRC = os.path.join(ROOT, "tests", "dropin", "test_files.cc")
+FILES_TEST_BIN = os.path.join(ROOT, "tests", "dropin", "test_files")
 
 
 def build_dropin_test():
-    """The C++ program that drives the drop-in classes like the reference's unit tests."""
-    if not os.path.exists(DROPIN_TEST_SRC) or not os.path.exists(DROPIN_LIB):
+    """The C++ programs that drive the drop-in classes like the reference's unit
+    tests (GPU), and its sample-file functions (CPU)."""
+    if not os.path.exists(DROPIN_LIB):
         return
-    _run(["g++", "-O2", "-std=c++14", "-Wall", f"-I{os.path.join(HERE, 'hdr')}", f"-I{os.path.join(ROOT, 'include')}",
-          DROPIN_TEST_SRC, "-o", DROPIN_TEST_BIN, f"-L{HERE}", "-lfalcon_dsp", "-lfdsp_hip",
-          "-Wl,-rpath,$ORIGIN/../../libfalcon_dsp_amd"])
+    for src, out in ((DROPIN_TEST_SRC, DROPIN_TEST_BIN), (FILES_TEST_SRC, FILES_TEST_BIN)):
+        if os.path.exists(src):
+            _run(["g++", "-O2", "-std=c++14", "-Wall", f"-I{os.path.join(HERE, 'hdr')}",
+                  f"-I{os.path.join(ROOT, 'include')}", src, "-o", out, f"-L{HERE}", "-lfalcon_dsp", "-lfdsp_hip",
+                  "-Wl,-rpath,$ORIGIN/../../libfalcon_dsp_amd"])
 
 
 def build_oracle():

@@ -322,8 +322,12 @@ extern "C" int fdsp_freq_shift_get_state(fdsp_freq_shift_t h, uint32_t chan, uin
     return FDSP_OK;
 }
 
-extern "C" int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
-                                            size_t out_stride, fdsp_stream_t stream)
+namespace {
+
+// T: float2 (cf32) or short2 (ci16) samples, in and out alike
+template <typename T>
+int freq_shift_device(fdsp_freq_shift_t h, const T *d_in, size_t n, T *d_out, size_t out_stride,
+                      fdsp_stream_t stream)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
@@ -333,14 +337,14 @@ extern "C" int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32
         return set_err(FDSP_EINVAL, "bad buffers");
     if (int rc = check_chunk(n, 1))
         return rc;
-    HIP_TRY(launch_nco_multi(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, h->start,
-                             h->d_chans.as<NcoChan>(), (uint32_t)h->chans.size(), reinterpret_cast<float2 *>(d_out),
+    HIP_TRY(launch_nco_multi(d_in, (uint32_t)n, h->start, h->d_chans.as<NcoChan>(), (uint32_t)h->chans.size(), d_out,
                              out_stride, pick(h->stream, stream)));
     h->start += n;
     return FDSP_OK;
 }
 
-extern "C" int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *const *outs)
+template <typename T>
+int freq_shift_host(fdsp_freq_shift_t h, const T *in, size_t n, T *const *outs)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
@@ -349,19 +353,43 @@ extern "C" int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, s
     if (!in || !outs)
         return set_err(FDSP_EINVAL, "null buffers");
     const size_t C = h->chans.size();
-    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+    if (int rc = h->d_in.ensure(n * sizeof(T)))
         return rc;
-    if (int rc = h->d_out.ensure(n * C * sizeof(float2)))
+    if (int rc = h->d_out.ensure(n * C * sizeof(T)))
         return rc;
-    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
-    if (int rc = fdsp_freq_shift_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
-                                              h->d_out.as<fdsp_cf32>(), n, h->stream))
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    if (int rc = freq_shift_device(h, h->d_in.as<T>(), n, h->d_out.as<T>(), n, h->stream))
         return rc;
     for (size_t c = 0; c < C; ++c)
-        HIP_TRY(hipMemcpyAsync(outs[c], h->d_out.as<float2>() + c * n, n * sizeof(float2), hipMemcpyDeviceToHost,
-                               h->stream));
+        HIP_TRY(hipMemcpyAsync(outs[c], h->d_out.as<T>() + c * n, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return FDSP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                            size_t out_stride, fdsp_stream_t stream)
+{
+    return freq_shift_device(h, reinterpret_cast<const float2 *>(d_in), n, reinterpret_cast<float2 *>(d_out),
+                             out_stride, stream);
+}
+
+extern "C" int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *const *outs)
+{
+    return freq_shift_host(h, reinterpret_cast<const float2 *>(in), n, reinterpret_cast<float2 *const *>(outs));
+}
+
+extern "C" int fdsp_freq_shift_apply_ci16_device(fdsp_freq_shift_t h, const fdsp_ci16 *d_in, size_t n,
+                                                 fdsp_ci16 *d_out, size_t out_stride, fdsp_stream_t stream)
+{
+    return freq_shift_device(h, reinterpret_cast<const short2 *>(d_in), n, reinterpret_cast<short2 *>(d_out),
+                             out_stride, stream);
+}
+
+extern "C" int fdsp_freq_shift_apply_ci16(fdsp_freq_shift_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *const *outs)
+{
+    return freq_shift_host(h, reinterpret_cast<const short2 *>(in), n, reinterpret_cast<short2 *const *>(outs));
 }
 
 // ===========================================================================
@@ -598,8 +626,12 @@ extern "C" int fdsp_fir_reset(fdsp_fir_t h)
     return FDSP_OK;
 }
 
-extern "C" int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
-                                     fdsp_stream_t stream)
+namespace {
+
+// T: float2 (cf32) or short2 (ci16) samples, in and out alike; the history
+// is float either way
+template <typename T>
+int fir_device(fdsp_fir_t h, const T *d_in, size_t n, T *d_out, fdsp_stream_t stream)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
@@ -609,9 +641,8 @@ extern "C" int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t
         return set_err(FDSP_EINVAL, "null buffers");
     if (int rc = check_chunk(n, 1))
         return rc;
-    hipError_t e = launch_fir(reinterpret_cast<const float2 *>(d_in), (uint32_t)n, h->d_taps.as<float2>(), h->m,
-                              h->d_hist[h->parity].as<float2>(), h->d_hist[h->parity ^ 1].as<float2>(),
-                              reinterpret_cast<float2 *>(d_out), pick(h->stream, stream));
+    hipError_t e = launch_fir(d_in, (uint32_t)n, h->d_taps.as<float2>(), h->m, h->d_hist[h->parity].as<float2>(),
+                              h->d_hist[h->parity ^ 1].as<float2>(), d_out, pick(h->stream, stream));
     if (e == hipErrorInvalidValue)
         return set_err(FDSP_ERANGE, "fir: %u taps exceed the LDS tile", h->m);
     HIP_TRY(e);
@@ -619,7 +650,8 @@ extern "C" int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t
     return FDSP_OK;
 }
 
-extern "C" int fdsp_fir_apply(fdsp_fir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out)
+template <typename T>
+int fir_host(fdsp_fir_t h, const T *in, size_t n, T *out)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
@@ -627,17 +659,40 @@ extern "C" int fdsp_fir_apply(fdsp_fir_t h, const fdsp_cf32 *in, size_t n, fdsp_
         return set_err(FDSP_EINVAL, "empty input");
     if (!in || !out)
         return set_err(FDSP_EINVAL, "null buffers");
-    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+    if (int rc = h->d_in.ensure(n * sizeof(T)))
         return rc;
-    if (int rc = h->d_out.ensure(n * sizeof(float2)))
+    if (int rc = h->d_out.ensure(n * sizeof(T)))
         return rc;
-    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
-    if (int rc = fdsp_fir_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
-                                       h->d_out.as<fdsp_cf32>(), h->stream))
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    if (int rc = fir_device(h, h->d_in.as<T>(), n, h->d_out.as<T>(), h->stream))
         return rc;
-    HIP_TRY(hipMemcpyAsync(out, h->d_out.p, n * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(out, h->d_out.p, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return FDSP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdsp_fir_apply_device(fdsp_fir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                     fdsp_stream_t stream)
+{
+    return fir_device(h, reinterpret_cast<const float2 *>(d_in), n, reinterpret_cast<float2 *>(d_out), stream);
+}
+
+extern "C" int fdsp_fir_apply(fdsp_fir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out)
+{
+    return fir_host(h, reinterpret_cast<const float2 *>(in), n, reinterpret_cast<float2 *>(out));
+}
+
+extern "C" int fdsp_fir_apply_ci16_device(fdsp_fir_t h, const fdsp_ci16 *d_in, size_t n, fdsp_ci16 *d_out,
+                                          fdsp_stream_t stream)
+{
+    return fir_device(h, reinterpret_cast<const short2 *>(d_in), n, reinterpret_cast<short2 *>(d_out), stream);
+}
+
+extern "C" int fdsp_fir_apply_ci16(fdsp_fir_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out)
+{
+    return fir_host(h, reinterpret_cast<const short2 *>(in), n, reinterpret_cast<short2 *>(out));
 }
 
 // ===========================================================================
@@ -780,8 +835,12 @@ extern "C" int fdsp_discriminator_reset(fdsp_discriminator_t h)
     return FDSP_OK;
 }
 
-extern "C" int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fdsp_cf32 *d_in, size_t n, float *d_out,
-                                               size_t *n_out, fdsp_stream_t stream)
+namespace {
+
+// T: float2 (cf32) or short2 (ci16) input samples
+template <typename T>
+int discriminator_device(fdsp_discriminator_t h, const T *d_in, size_t n, float *d_out, size_t *n_out,
+                         fdsp_stream_t stream)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
@@ -794,8 +853,7 @@ extern "C" int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fds
         return set_err(FDSP_EINVAL, "null buffers");
     if (int rc = check_chunk(n, 1))
         return rc;
-    HIP_TRY(launch_discriminator(reinterpret_cast<const float2 *>(d_in), (uint32_t)n,
-                                 h->d_prev[h->parity].as<float2>(), h->has_prev ? 1 : 0,
+    HIP_TRY(launch_discriminator(d_in, (uint32_t)n, h->d_prev[h->parity].as<float2>(), h->has_prev ? 1 : 0,
                                  h->d_prev[h->parity ^ 1].as<float2>(), d_out, pick(h->stream, stream)));
     *n_out = h->has_prev ? n : n - 1;
     h->has_prev = true;
@@ -803,8 +861,8 @@ extern "C" int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fds
     return FDSP_OK;
 }
 
-extern "C" int fdsp_discriminator_apply(fdsp_discriminator_t h, const fdsp_cf32 *in, size_t n, float *out,
-                                        size_t *n_out)
+template <typename T>
+int discriminator_host(fdsp_discriminator_t h, const T *in, size_t n, float *out, size_t *n_out)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
@@ -815,20 +873,45 @@ extern "C" int fdsp_discriminator_apply(fdsp_discriminator_t h, const fdsp_cf32 
         return set_err(FDSP_EINVAL, "empty input");
     if (!in || !out)
         return set_err(FDSP_EINVAL, "null buffers");
-    if (int rc = h->d_in.ensure(n * sizeof(float2)))
+    if (int rc = h->d_in.ensure(n * sizeof(T)))
         return rc;
     if (int rc = h->d_out.ensure(n * sizeof(float)))
         return rc;
-    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
     size_t got = 0;
-    if (int rc = fdsp_discriminator_apply_device(h, reinterpret_cast<const fdsp_cf32 *>(h->d_in.p), n,
-                                                 h->d_out.as<float>(), &got, h->stream))
+    if (int rc = discriminator_device(h, h->d_in.as<T>(), n, h->d_out.as<float>(), &got, h->stream))
         return rc;
     if (got)
         HIP_TRY(hipMemcpyAsync(out, h->d_out.p, got * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     *n_out = got;
     return FDSP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdsp_discriminator_apply_device(fdsp_discriminator_t h, const fdsp_cf32 *d_in, size_t n, float *d_out,
+                                               size_t *n_out, fdsp_stream_t stream)
+{
+    return discriminator_device(h, reinterpret_cast<const float2 *>(d_in), n, d_out, n_out, stream);
+}
+
+extern "C" int fdsp_discriminator_apply(fdsp_discriminator_t h, const fdsp_cf32 *in, size_t n, float *out,
+                                        size_t *n_out)
+{
+    return discriminator_host(h, reinterpret_cast<const float2 *>(in), n, out, n_out);
+}
+
+extern "C" int fdsp_discriminator_apply_ci16_device(fdsp_discriminator_t h, const fdsp_ci16 *d_in, size_t n,
+                                                    float *d_out, size_t *n_out, fdsp_stream_t stream)
+{
+    return discriminator_device(h, reinterpret_cast<const short2 *>(d_in), n, d_out, n_out, stream);
+}
+
+extern "C" int fdsp_discriminator_apply_ci16(fdsp_discriminator_t h, const fdsp_ci16 *in, size_t n, float *out,
+                                             size_t *n_out)
+{
+    return discriminator_host(h, reinterpret_cast<const short2 *>(in), n, out, n_out);
 }
 
 // ===========================================================================

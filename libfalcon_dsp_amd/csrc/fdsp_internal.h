@@ -24,6 +24,27 @@ __device__ __forceinline__ void cmac(float2 &acc, float2 a, float2 b)
     acc.y = fmaf(a.y, b.x, acc.y);
 }
 
+// Complex int16 samples (fdsp_ci16 on the host, short2 here) are read
+// exactly as floats. Results go back to int16 as the reference's
+// float -> int16_t conversions do (std::complex<int16_t> built from float
+// parts, e.g. src/transform/falcon_dsp_fir_filter.cc:168-171): truncation
+// toward zero. Outside the int16 range, where that conversion is undefined,
+// they saturate.
+__device__ __forceinline__ float2 sample_f32(float2 v) { return v; }
+__device__ __forceinline__ float2 sample_f32(short2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ short i16_trunc(float v) { return (short)(int)fminf(fmaxf(v, -32768.f), 32767.f); }
+__device__ __forceinline__ void sample_put(float2 *p, float2 v) { *p = v; }
+__device__ __forceinline__ void sample_put(short2 *p, float2 v) { *p = make_short2(i16_trunc(v.x), i16_trunc(v.y)); }
+
+// a * b rounded as std::complex<float> multiplication rounds it, four
+// products and two sums with no fused multiply-add (the integer-output paths
+// compute as the reference does, so their truncated results agree bit for bit)
+__device__ __forceinline__ float2 cmul_ref(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 // e^{j 2 pi nu k}: the phase is formed and reduced to [-1/2, 1/2] revolutions
 // in FP64 (k < 2^32 and |nu * k| < 2^21, so the reduced phase carries
 // < 1e-9 revolutions of error), then the hardware v_sin_f32 / v_cos_f32,
@@ -61,6 +82,7 @@ struct NcoChan {
     uint32_t r;       // rollover index R (UINT32_MAX for the identity shift)
     uint32_t pad;
     float2 step[4];   // e^{j 2 pi nu i}, i = 0..3
+    double omega;     // 2 pi f / fs rad/sample, formed as the reference forms it
 };
 
 // static description of one channel of the fused channelizer
@@ -86,8 +108,11 @@ struct ChanPos {
 // ---------------------------------------------------------------------------
 // launchers (all asynchronous on `s`)
 // ---------------------------------------------------------------------------
-hipError_t launch_nco_multi(const float2 *in, uint32_t n, uint64_t start, const NcoChan *chans,
-                            uint32_t n_chan, float2 *out, uint64_t out_stride, hipStream_t s);
+// TI, TO: float2 (cf32) or short2 (ci16) samples, in the pairs
+// (float2, float2) and (short2, short2)
+template <typename TI, typename TO>
+hipError_t launch_nco_multi(const TI *in, uint32_t n, uint64_t start, const NcoChan *chans, uint32_t n_chan,
+                            TO *out, uint64_t out_stride, hipStream_t s);
 
 // polyphase resampler over one chunk (global input position p); `taps` are
 // the real parts of the transposed bank, up*cpp floats. hist_in holds cpp-1
@@ -97,8 +122,10 @@ hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t u
                             float2 *out, uint32_t count, hipStream_t s);
 
 // FIR with complex taps over one chunk; hist_in/hist_out hold m-1 samples
-hipError_t launch_fir(const float2 *in, uint32_t n, const float2 *taps, uint32_t m, const float2 *hist_in,
-                      float2 *hist_out, float2 *out, hipStream_t s);
+// (float whatever the sample type); TI, TO as for launch_nco_multi
+template <typename TI, typename TO>
+hipError_t launch_fir(const TI *in, uint32_t n, const float2 *taps, uint32_t m, const float2 *hist_in,
+                      float2 *hist_out, TO *out, hipStream_t s);
 
 // FIR bank with real taps, n_chan channels of n samples each
 hipError_t launch_fir_bank(const float2 *in, uint32_t n, uint64_t in_stride, const float *taps, uint32_t m,
@@ -107,8 +134,10 @@ hipError_t launch_fir_bank(const float2 *in, uint32_t n, uint64_t in_stride, con
 
 // polar discriminator; prev (device, 1 sample) is read when has_prev, and
 // prev_out receives the last input sample
-hipError_t launch_discriminator(const float2 *in, uint32_t n, const float2 *prev, int has_prev,
-                                float2 *prev_out, float *out, hipStream_t s);
+// (float2 or short2 input; the carried sample is float)
+template <typename TI>
+hipError_t launch_discriminator(const TI *in, uint32_t n, const float2 *prev, int has_prev, float2 *prev_out,
+                                float *out, hipStream_t s);
 
 // Work plan of the fused channelizer, built on the host at create time: the
 // channels fall in groups, each computed by its own row tiles (one block per

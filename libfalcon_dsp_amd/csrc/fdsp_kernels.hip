@@ -8,6 +8,8 @@
 // n0 = ceil(p*up/down) ... ceil((p+N)*up/down)-1, and the reference's
 // coeff_phase / xOffset are functions of (p, n0). The NCO index of input i is
 // i mod R (reference src/transform/falcon_dsp_freq_shift.cc:177-189).
+#include <type_traits>
+
 #include "fdsp_internal.h"
 
 namespace fdsp {
@@ -20,13 +22,51 @@ __device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) { retur
 //  src/transform/falcon_dsp_freq_shift_cuda.cu:124-216)
 // Each thread owns 4 consecutive samples: one exact phasor per thread and
 // channel, the other three by a product with e^{j 2 pi nu i} (two roundings).
+// Complex int16 output instead takes the reference's own arithmetic
+// (falcon_dsp_freq_shift.cc:146-151): per sample, angle = omega * s in FP64
+// with s the index modulo R, cos and sin in FP64 rounded to float, and the
+// float complex product, so the truncated outputs equal the reference's.
 // ---------------------------------------------------------------------------
 constexpr int kNcoL = 4;
 
-template <bool kVec>
-__global__ __launch_bounds__(256) void nco_multi_kernel(const float2 *__restrict__ in, uint32_t n, uint64_t start,
+// four consecutive samples as whole 16-byte accesses (p 16-byte aligned)
+__device__ __forceinline__ void load4(const float2 *p, float2 (&x)[4])
+{
+    const float4 a = reinterpret_cast<const float4 *>(p)[0];
+    const float4 b = reinterpret_cast<const float4 *>(p)[1];
+    x[0] = make_float2(a.x, a.y);
+    x[1] = make_float2(a.z, a.w);
+    x[2] = make_float2(b.x, b.y);
+    x[3] = make_float2(b.z, b.w);
+}
+struct alignas(16) Ci16x4 {
+    short2 v[4];
+};
+__device__ __forceinline__ void load4(const short2 *p, float2 (&x)[4])
+{
+    const Ci16x4 q = *reinterpret_cast<const Ci16x4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        x[i] = sample_f32(q.v[i]);
+}
+__device__ __forceinline__ void store4(float2 *p, const float2 (&y)[4])
+{
+    reinterpret_cast<float4 *>(p)[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+    reinterpret_cast<float4 *>(p)[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+}
+__device__ __forceinline__ void store4(short2 *p, const float2 (&y)[4])
+{
+    Ci16x4 q;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        q.v[i] = make_short2(i16_trunc(y[i].x), i16_trunc(y[i].y));
+    *reinterpret_cast<Ci16x4 *>(p) = q;
+}
+
+template <bool kVec, typename TI, typename TO>
+__global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ in, uint32_t n, uint64_t start,
                                                         const NcoChan *__restrict__ chans, uint32_t n_chan,
-                                                        float2 *__restrict__ out, uint64_t out_stride)
+                                                        TO *__restrict__ out, uint64_t out_stride)
 {
     const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kNcoL;
     if (i0 >= n)
@@ -34,64 +74,74 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const float2 *__restrict
     const bool full = (i0 + kNcoL) <= n;
     float2 x[kNcoL];
     if (kVec && full) {
-        const float4 a = *reinterpret_cast<const float4 *>(in + i0);
-        const float4 b = *reinterpret_cast<const float4 *>(in + i0 + 2);
-        x[0] = make_float2(a.x, a.y);
-        x[1] = make_float2(a.z, a.w);
-        x[2] = make_float2(b.x, b.y);
-        x[3] = make_float2(b.z, b.w);
+        load4(in + i0, x);
     } else {
 #pragma unroll
         for (int i = 0; i < kNcoL; ++i)
-            x[i] = (i0 + i < n) ? in[i0 + i] : make_float2(0.f, 0.f);
+            x[i] = (i0 + i < n) ? sample_f32(in[i0 + i]) : make_float2(0.f, 0.f);
     }
     for (uint32_t c = 0; c < n_chan; ++c) {
         const double nu = chans[c].nu;
         const uint32_t r = chans[c].r;
         const uint64_t k0 = mod_u64(start + i0, r, chans[c].inv_r);
-        const float2 base = nco_phasor((int64_t)k0, nu);
         float2 y[kNcoL];
-        if (k0 + (kNcoL - 1) < r) {
+        if constexpr (std::is_same<TO, short2>::value) {
+            const double w = chans[c].omega;
+#pragma unroll
+            for (int i = 0; i < kNcoL; ++i) {
+                const uint64_t k = k0 + i >= r ? k0 + i - r : k0 + i;
+                double sn, cs;
+                sincos(w * (double)k, &sn, &cs);
+                y[i] = cmul_ref(x[i], make_float2((float)cs, (float)sn));
+            }
+        } else if (k0 + (kNcoL - 1) < r) {
+            const float2 base = nco_phasor((int64_t)k0, nu);
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i)
                 y[i] = cmul(x[i], cmul(base, chans[c].step[i]));
         } else {
             // the rollover falls inside this thread's samples: index restarts at 0
+            const float2 base = nco_phasor((int64_t)k0, nu);
             const float2 basew = nco_phasor((int64_t)k0 - (int64_t)r, nu);
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i)
                 y[i] = cmul(x[i], cmul((k0 + i < r) ? base : basew, chans[c].step[i]));
         }
-        float2 *o = out + (uint64_t)c * out_stride;
+        TO *o = out + (uint64_t)c * out_stride;
         if (kVec && full) {
-            *reinterpret_cast<float4 *>(o + i0) = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
-            *reinterpret_cast<float4 *>(o + i0 + 2) = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+            store4(o + i0, y);
         } else {
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i)
                 if (i0 + i < n)
-                    o[i0 + i] = y[i];
+                    sample_put(o + i0 + i, y[i]);
         }
     }
 }
 
-hipError_t launch_nco_multi(const float2 *in, uint32_t n, uint64_t start, const NcoChan *chans, uint32_t n_chan,
-                            float2 *out, uint64_t out_stride, hipStream_t s)
+template <typename TI, typename TO>
+hipError_t launch_nco_multi(const TI *in, uint32_t n, uint64_t start, const NcoChan *chans, uint32_t n_chan, TO *out,
+                            uint64_t out_stride, hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
     const uint32_t threads = (n + kNcoL - 1) / kNcoL;
     const uint32_t blocks = (threads + 255) / 256;
+    // 16-byte accesses of 4 samples: every channel's output must start 16-byte aligned
     const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16 == 0) &&
-                     (out_stride % 2 == 0);
+                     (out_stride * sizeof(TO)) % 16 == 0;
     if (vec)
-        hipLaunchKernelGGL(nco_multi_kernel<true>, dim3(blocks), dim3(256), 0, s, in, n, start, chans, n_chan, out,
-                           out_stride);
+        hipLaunchKernelGGL((nco_multi_kernel<true, TI, TO>), dim3(blocks), dim3(256), 0, s, in, n, start, chans,
+                           n_chan, out, out_stride);
     else
-        hipLaunchKernelGGL(nco_multi_kernel<false>, dim3(blocks), dim3(256), 0, s, in, n, start, chans, n_chan, out,
-                           out_stride);
+        hipLaunchKernelGGL((nco_multi_kernel<false, TI, TO>), dim3(blocks), dim3(256), 0, s, in, n, start, chans,
+                           n_chan, out, out_stride);
     return hipGetLastError();
 }
+template hipError_t launch_nco_multi<float2, float2>(const float2 *, uint32_t, uint64_t, const NcoChan *, uint32_t,
+                                                     float2 *, uint64_t, hipStream_t);
+template hipError_t launch_nco_multi<short2, short2>(const short2 *, uint32_t, uint64_t, const NcoChan *, uint32_t,
+                                                     short2 *, uint64_t, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Polyphase resampler, one stream
@@ -229,11 +279,11 @@ hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t u
 //  _cuda.cu:102-187). y[n] = sum_k h[k] x[n-k], accumulated k = 0..M-1.
 // Input window and taps staged in LDS.
 // ---------------------------------------------------------------------------
-template <int kBlock, bool kTapsLds>
-__global__ __launch_bounds__(kBlock) void fir_kernel(const float2 *__restrict__ in, uint32_t n,
+template <int kBlock, bool kTapsLds, typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void fir_kernel(const TI *__restrict__ in, uint32_t n,
                                                      const float2 *__restrict__ taps, uint32_t m,
                                                      const float2 *__restrict__ hist_in,
-                                                     float2 *__restrict__ hist_out, float2 *__restrict__ out,
+                                                     float2 *__restrict__ hist_out, TO *__restrict__ out,
                                                      uint32_t tile)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -241,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void fir_kernel(const float2 *__restrict__ 
     if (blockIdx.x == 0) {
         for (uint32_t t = threadIdx.x; t < hlen; t += kBlock) {
             const int64_t i = (int64_t)n - hlen + t;
-            hist_out[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+            hist_out[t] = i >= 0 ? sample_f32(in[i]) : hist_in[hlen + i];
         }
     }
     const uint32_t o0 = blockIdx.x * tile;
@@ -251,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void fir_kernel(const float2 *__restrict__ 
     const int64_t lo = (int64_t)o0 - hlen;
     for (uint32_t t = threadIdx.x; t < nt + hlen; t += kBlock) {
         const int64_t i = lo + t;
-        xs[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+        xs[t] = i >= 0 ? sample_f32(in[i]) : hist_in[hlen + i];
     }
     if (kTapsLds)
         for (uint32_t t = threadIdx.x; t < m; t += kBlock)
@@ -261,19 +311,33 @@ __global__ __launch_bounds__(kBlock) void fir_kernel(const float2 *__restrict__ 
     for (uint32_t q = threadIdx.x; q < nt; q += kBlock) {
         const float2 *xw = xs + q + hlen;  // xw[-k] = x[o0 + q - k]
         float2 acc = make_float2(0.f, 0.f);
-        for (uint32_t k = 0; k < m; ++k) {
-            const float2 x = xw[-(int32_t)k];
-            const float2 h = g[k];
-            // complex<float> product added to the accumulator, reference order
-            acc.x += x.x * h.x - x.y * h.y;
-            acc.y += x.x * h.y + x.y * h.x;
+        if constexpr (std::is_same<TO, short2>::value) {
+            // integer output: the reference's rounding, product then sum, no
+            // fused multiply-add (falcon_dsp_fir_filter.cc:208-230)
+            for (uint32_t k = 0; k < m; ++k) {
+                const float2 p = cmul_ref(xw[-(int32_t)k], g[k]);
+                {
+#pragma clang fp contract(off)
+                    acc.x = acc.x + p.x;
+                    acc.y = acc.y + p.y;
+                }
+            }
+        } else {
+            for (uint32_t k = 0; k < m; ++k) {
+                const float2 x = xw[-(int32_t)k];
+                const float2 h = g[k];
+                // complex<float> product added to the accumulator, reference order
+                acc.x += x.x * h.x - x.y * h.y;
+                acc.y += x.x * h.y + x.y * h.x;
+            }
         }
-        out[o0 + q] = acc;
+        sample_put(out + o0 + q, acc);
     }
 }
 
-hipError_t launch_fir(const float2 *in, uint32_t n, const float2 *taps, uint32_t m, const float2 *hist_in,
-                      float2 *hist_out, float2 *out, hipStream_t s)
+template <typename TI, typename TO>
+hipError_t launch_fir(const TI *in, uint32_t n, const float2 *taps, uint32_t m, const float2 *hist_in,
+                      float2 *hist_out, TO *out, hipStream_t s)
 {
     constexpr int kBlock = 256;
     const uint32_t tile = 2048;
@@ -283,13 +347,17 @@ hipError_t launch_fir(const float2 *in, uint32_t n, const float2 *taps, uint32_t
         return hipErrorInvalidValue;
     const uint32_t blocks = (n + tile - 1) / tile;
     if (taps_lds)
-        hipLaunchKernelGGL((fir_kernel<kBlock, true>), dim3(blocks), dim3(kBlock), lds, s, in, n, taps, m, hist_in,
-                           hist_out, out, tile);
+        hipLaunchKernelGGL((fir_kernel<kBlock, true, TI, TO>), dim3(blocks), dim3(kBlock), lds, s, in, n, taps, m,
+                           hist_in, hist_out, out, tile);
     else
-        hipLaunchKernelGGL((fir_kernel<kBlock, false>), dim3(blocks), dim3(kBlock), lds, s, in, n, taps, m, hist_in,
-                           hist_out, out, tile);
+        hipLaunchKernelGGL((fir_kernel<kBlock, false, TI, TO>), dim3(blocks), dim3(kBlock), lds, s, in, n, taps, m,
+                           hist_in, hist_out, out, tile);
     return hipGetLastError();
 }
+template hipError_t launch_fir<float2, float2>(const float2 *, uint32_t, const float2 *, uint32_t, const float2 *,
+                                               float2 *, float2 *, hipStream_t);
+template hipError_t launch_fir<short2, short2>(const short2 *, uint32_t, const float2 *, uint32_t, const float2 *,
+                                               float2 *, short2 *, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // FIR bank, real taps, many channels (compute-bound FP32 VALU kernel).
@@ -405,7 +473,8 @@ hipError_t launch_fir_bank(const float2 *in, uint32_t n, uint64_t in_stride, con
 // Polar discriminator
 // (reference falcon_dsp_polar_discriminator::apply, .cc:134-171)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void discriminator_kernel(const float2 *__restrict__ in, uint32_t n,
+template <typename TI>
+__global__ __launch_bounds__(256) void discriminator_kernel(const TI *__restrict__ in, uint32_t n,
                                                             const float2 *__restrict__ prev, int has_prev,
                                                             float2 *__restrict__ prev_out, float *__restrict__ out)
 {
@@ -413,23 +482,29 @@ __global__ __launch_bounds__(256) void discriminator_kernel(const float2 *__rest
     if (i >= n)
         return;
     if (i == n - 1)
-        *prev_out = in[i];
+        *prev_out = sample_f32(in[i]);
     if (i == 0 && !has_prev)
         return;
-    const float2 a = in[i];
-    const float2 b = (i == 0) ? *prev : in[i - 1];
-    const float2 z = cmul(a, make_float2(b.x, -b.y));
+    const float2 a = sample_f32(in[i]);
+    const float2 b = (i == 0) ? *prev : sample_f32(in[i - 1]);
+    // the reference's float complex product (unfused), whatever the input type
+    const float2 z = cmul_ref(a, make_float2(b.x, -b.y));
     out[has_prev ? i : i - 1] = atan2f(z.y, z.x);
 }
 
-hipError_t launch_discriminator(const float2 *in, uint32_t n, const float2 *prev, int has_prev, float2 *prev_out,
+template <typename TI>
+hipError_t launch_discriminator(const TI *in, uint32_t n, const float2 *prev, int has_prev, float2 *prev_out,
                                 float *out, hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(discriminator_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in, n, prev, has_prev,
+    hipLaunchKernelGGL(discriminator_kernel<TI>, dim3((n + 255) / 256), dim3(256), 0, s, in, n, prev, has_prev,
                        prev_out, out);
     return hipGetLastError();
 }
+template hipError_t launch_discriminator<float2>(const float2 *, uint32_t, const float2 *, int, float2 *, float *,
+                                                 hipStream_t);
+template hipError_t launch_discriminator<short2>(const short2 *, uint32_t, const float2 *, int, float2 *, float *,
+                                                 hipStream_t);
 
 }  // namespace fdsp

@@ -157,6 +157,18 @@ def device_sync():
     check(lib().fdsp_device_sync())
 
 
+# complex int16 samples: an (n, 2) int16 array of (re, im) pairs, the layout
+# of std::complex<int16_t> and fdsp_ci16
+CI16 = np.dtype([("re", np.int16), ("im", np.int16)])
+
+
+def _ci16(x) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int16))
+    if a.ndim != 2 or a.shape[1] != 2:
+        raise ValueError("complex int16 samples are an (n, 2) int16 array of (re, im)")
+    return a
+
+
 # ---------------------------------------------------------------------------
 # rate helpers
 # ---------------------------------------------------------------------------
@@ -248,6 +260,22 @@ class FreqShift(_Handle):
         check(lib().fdsp_freq_shift_apply_device(self.h, d_in.at(in_offset), n, d_out.ptr, out_stride,
                                                  stream.h if stream else None))
 
+    def apply_ci16(self, x):
+        """falcon_dsp_freq_shift::apply(vector<complex<int16_t>>) (reference
+        freq_shift.cc:133-162): (n, 2) int16 in, (n, 2) int16 out per shift."""
+        x = _ci16(x)
+        n = x.shape[0]
+        outs = [np.empty((n, 2), dtype=np.int16) for _ in range(self.n_shifts)]
+        ptrs = (_vp * self.n_shifts)(*[o.ctypes.data for o in outs])
+        check(lib().fdsp_freq_shift_apply_ci16(self.h, _ptr(x), n, ptrs))
+        return outs[0] if self.single else outs
+
+    def apply_ci16_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, out_stride: int,
+                          stream: Optional[Stream] = None):
+        """Device buffers of CI16 samples (DeviceBuffer(count, CI16))."""
+        check(lib().fdsp_freq_shift_apply_ci16_device(self.h, d_in.ptr, n, d_out.ptr, out_stride,
+                                                      stream.h if stream else None))
+
 
 class PolyphaseResampler(_Handle):
     """falcon_dsp_polyphase_resampler: streaming upfirdn with real taps."""
@@ -317,6 +345,18 @@ class FirFilter(_Handle):
     def apply_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None):
         check(lib().fdsp_fir_apply_device(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
 
+    def apply_ci16(self, x) -> np.ndarray:
+        """falcon_dsp_fir_filter::apply(vector<complex<int16_t>>) (reference
+        fir_filter.cc:151-175): (n, 2) int16 in and out; the history is shared
+        with apply()."""
+        x = _ci16(x)
+        out = np.empty_like(x)
+        check(lib().fdsp_fir_apply_ci16(self.h, _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def apply_ci16_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None):
+        check(lib().fdsp_fir_apply_ci16_device(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
+
 
 class FirBank(_Handle):
     """Many independent FIR channels with real taps, one launch for all."""
@@ -366,6 +406,16 @@ class PolarDiscriminator(_Handle):
         out = np.empty(max(x.size, 1), dtype=np.float32)
         n = C.c_size_t()
         check(lib().fdsp_discriminator_apply(self.h, _ptr(x), x.size, _ptr(out), C.byref(n)))
+        return out[: n.value]
+
+    def apply_ci16(self, x) -> np.ndarray:
+        """falcon_dsp_polar_discriminator::apply(vector<complex<int16_t>>)
+        (reference polar_discriminator.cc:117-132); the carried sample is
+        shared with apply()."""
+        x = _ci16(x)
+        out = np.empty(max(x.shape[0], 1), dtype=np.float32)
+        n = C.c_size_t()
+        check(lib().fdsp_discriminator_apply_ci16(self.h, _ptr(x), x.shape[0], _ptr(out), C.byref(n)))
         return out[: n.value]
 
 

@@ -13,6 +13,8 @@ namespace falcon_dsp
 {
     bool fir_filter(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<float>>& in,
                     std::vector<std::complex<float>>& out);
+    bool fir_filter(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<int16_t>>& in,
+                    std::vector<std::complex<int16_t>>& out);
 
     class falcon_dsp_fir_filter
     {
@@ -26,6 +28,10 @@ namespace falcon_dsp
         // y[n] = sum_k h[k] x[n-k] with the history carried between calls;
         // false for an empty input or taps
         virtual bool apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out);
+        // complex int16: filtered as floats (sharing the history above), each
+        // output part truncated toward zero back to int16 (saturated outside
+        // its range); out is cleared first
+        virtual bool apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out);
 
     protected:
         std::mutex  m_mutex;

@@ -8,6 +8,9 @@ namespace falcon_dsp
 {
     bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<float>>& in,
                          std::vector<std::complex<float>>& out);
+    // (appends to out, as the reference's GPU class does)
+    bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<int16_t>>& in,
+                         std::vector<std::complex<int16_t>>& out);
 
     class falcon_dsp_fir_filter_cuda : public falcon_dsp_fir_filter
     {
@@ -17,5 +20,10 @@ namespace falcon_dsp
         ~falcon_dsp_fir_filter_cuda(void) = default;
         falcon_dsp_fir_filter_cuda(void) = delete;
         falcon_dsp_fir_filter_cuda(const falcon_dsp_fir_filter_cuda&) = delete;
+
+        using falcon_dsp_fir_filter::apply;
+        // complex int16; the results are appended to out without clearing it,
+        // as in the reference (falcon_dsp_fir_filter_cuda.cu:233-254)
+        bool apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out) override;
     };
 }

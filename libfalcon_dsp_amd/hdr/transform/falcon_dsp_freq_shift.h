@@ -16,6 +16,8 @@ namespace falcon_dsp
     // y[n] = x[n] * e^{j w s}, s = samples handled modulo the rollover index
     bool freq_shift(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
                     int32_t freq_shift_in_hz, std::vector<std::complex<float>>& out);
+    bool freq_shift(uint32_t in_sample_rate_in_sps, std::vector<std::complex<int16_t>>& in,
+                    int32_t freq_shift_in_hz, std::vector<std::complex<int16_t>>& out);
 
     class falcon_dsp_freq_shift
     {
@@ -34,6 +36,9 @@ namespace falcon_dsp
         void reset_state(void);
         // false when not initialised or the input is empty
         virtual bool apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out);
+        // complex int16: shifted as floats, each output part truncated toward
+        // zero back to int16 (saturated outside its range); out is cleared first
+        virtual bool apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out);
 
     protected:
         bool _create(uint32_t fs, const int32_t *shifts, uint32_t n);

@@ -8,6 +8,9 @@ namespace falcon_dsp
 {
     bool freq_shift_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
                          int32_t freq_shift_in_hz, std::vector<std::complex<float>>& out);
+    // (appends to out, as the reference's GPU class does)
+    bool freq_shift_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<int16_t>>& in,
+                         int32_t freq_shift_in_hz, std::vector<std::complex<int16_t>>& out);
     // one output vector per shift, all from one pass over the input
     bool freq_shift_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>& in,
                          std::vector<int32_t>& freq_shift_in_hz, std::vector<std::vector<std::complex<float>>>& out);
@@ -24,5 +27,8 @@ namespace falcon_dsp
 
         bool apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out) override;
         bool apply(std::vector<std::complex<float>>& in, std::vector<std::vector<std::complex<float>>>& out);
+        // complex int16, single shift; the results are appended to out without
+        // clearing it, as in the reference (falcon_dsp_freq_shift_cuda.cu:308-329)
+        bool apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out) override;
     };
 }

@@ -12,6 +12,7 @@ struct fdsp_discriminator_s;
 namespace falcon_dsp
 {
     bool polar_discriminator(const std::vector<std::complex<float>>& in, std::vector<float>& out);
+    bool polar_discriminator(const std::vector<std::complex<int16_t>>& in, std::vector<float>& out);
 
     class falcon_dsp_polar_discriminator
     {
@@ -24,6 +25,8 @@ namespace falcon_dsp
         // out[n] = arg(x[n] conj(x[n-1])): N-1 outputs on the first call, N
         // afterwards (the last sample is carried); false for an empty input
         virtual bool apply(const std::vector<std::complex<float>>& in, std::vector<float>& out);
+        // complex int16 input, read as floats; shares the carried sample
+        virtual bool apply(const std::vector<std::complex<int16_t>>& in, std::vector<float>& out);
 
     protected:
         std::mutex            m_mutex;

@@ -6,6 +6,7 @@
 // holds the reason.
 #include <cmath>
 #include <cstring>
+#include <fstream>
 
 #include "fdsp.h"
 #include "transform/falcon_dsp_fir_filter_cuda.h"
@@ -21,6 +22,80 @@ namespace
 // std::complex<float> is layout-compatible with float[2], as is fdsp_cf32
 inline const fdsp_cf32 *cf(const std::complex<float> *p) { return reinterpret_cast<const fdsp_cf32 *>(p); }
 inline fdsp_cf32 *cf(std::complex<float> *p) { return reinterpret_cast<fdsp_cf32 *>(p); }
+// and std::complex<int16_t> with int16_t[2] and fdsp_ci16
+inline const fdsp_ci16 *ci(const std::complex<int16_t> *p) { return reinterpret_cast<const fdsp_ci16 *>(p); }
+inline fdsp_ci16 *ci(std::complex<int16_t> *p) { return reinterpret_cast<fdsp_ci16 *>(p); }
+
+// little-endian encoding of the sample files, independent of the host's order
+void put_le(std::ofstream &f, uint32_t bits, int bytes)
+{
+    char b[4];
+    for (int i = 0; i < bytes; ++i)
+        b[i] = (char)((bits >> (8 * i)) & 0xFFu);
+    f.write(b, bytes);
+}
+uint32_t get_le(const unsigned char *b, int bytes)
+{
+    uint32_t v = 0;
+    for (int i = 0; i < bytes; ++i)
+        v |= (uint32_t)b[i] << (8 * i);
+    return v;
+}
+uint32_t f32_bits(float v)
+{
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    return u;
+}
+float bits_f32(uint32_t u)
+{
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+// Write n items of `parts` values each: binary through enc(value) -> bits,
+// ASCII as the values separated by spaces, one item per line
+template <typename T, typename Enc>
+bool write_items(const std::string &name, falcon_dsp::file_type_e type, const T *vals, size_t n, int parts, int bytes,
+                 Enc enc)
+{
+    if (type == falcon_dsp::file_type_e::BINARY) {
+        std::ofstream f(name, std::ios::out | std::ios::binary);
+        if (!f.is_open())
+            return false;
+        for (size_t i = 0; i < n * parts && f.good(); ++i)
+            put_le(f, enc(vals[i]), bytes);
+        return f.good();
+    }
+    if (type == falcon_dsp::file_type_e::ASCII) {
+        std::ofstream f(name, std::ios::out);
+        if (!f.is_open())
+            return false;
+        for (size_t i = 0; i < n && f.good(); ++i) {
+            for (int k = 0; k < parts; ++k)
+                f << (k ? " " : "") << vals[i * parts + k];
+            f << std::endl;
+        }
+        return f.good();
+    }
+    return false;
+}
+
+// Binary file of whole items of `bytes_per_item`, or false
+bool read_binary(const std::string &name, size_t bytes_per_item, std::vector<unsigned char> &raw)
+{
+    std::ifstream f(name, std::ios::in | std::ios::binary);
+    if (!f.is_open())
+        return false;
+    f.seekg(0, std::ios::end);
+    const std::streamoff size = f.tellg();
+    f.seekg(0, std::ios::beg);
+    raw.resize((size_t)(size < 0 ? 0 : size) / bytes_per_item * bytes_per_item);
+    if (!raw.empty())
+        f.read(reinterpret_cast<char *>(raw.data()), (std::streamsize)raw.size());
+    return f.good() || raw.empty();
+}
 }  // namespace
 
 namespace falcon_dsp
@@ -140,6 +215,47 @@ bool freq_shift(uint32_t in_sample_rate_in_sps, std::vector<std::complex<float>>
                 std::vector<std::complex<float>>& out)
 {
     falcon_dsp_freq_shift s;
+    return s.initialize(in_sample_rate_in_sps, freq_shift_in_hz) && s.apply(in, out);
+}
+
+// reference falcon_dsp_freq_shift.cc:133-162: out cleared, false when not
+// initialised or when nothing was produced
+bool falcon_dsp_freq_shift::apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || m_num_shifts != 1 || in.empty())
+        return false;
+    out.resize(in.size());
+    fdsp_ci16 *o = ci(out.data());
+    if (fdsp_freq_shift_apply_ci16(m_handle, ci(in.data()), in.size(), &o) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+// reference falcon_dsp_freq_shift_cuda.cu:308-329: appended to out, which is
+// not cleared first
+bool falcon_dsp_freq_shift_cuda::apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out)
+{
+    std::vector<std::complex<int16_t>> got;
+    const bool ok = falcon_dsp_freq_shift::apply(in, got);
+    out.insert(out.end(), got.begin(), got.end());
+    return ok;
+}
+
+bool freq_shift(uint32_t in_sample_rate_in_sps, std::vector<std::complex<int16_t>>& in, int32_t freq_shift_in_hz,
+                std::vector<std::complex<int16_t>>& out)
+{
+    falcon_dsp_freq_shift s;
+    return s.initialize(in_sample_rate_in_sps, freq_shift_in_hz) && s.apply(in, out);
+}
+
+bool freq_shift_cuda(uint32_t in_sample_rate_in_sps, std::vector<std::complex<int16_t>>& in, int32_t freq_shift_in_hz,
+                     std::vector<std::complex<int16_t>>& out)
+{
+    falcon_dsp_freq_shift_cuda s;
     return s.initialize(in_sample_rate_in_sps, freq_shift_in_hz) && s.apply(in, out);
 }
 
@@ -278,6 +394,45 @@ bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::
     return fir_filter(coeffs, in, out);
 }
 
+// reference falcon_dsp_fir_filter.cc:151-175: out cleared; the float
+// overload's result
+bool falcon_dsp_fir_filter::apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    if (fdsp_fir_apply_ci16(m_handle, ci(in.data()), in.size(), ci(out.data())) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+// reference falcon_dsp_fir_filter_cuda.cu:233-254: appended to out
+bool falcon_dsp_fir_filter_cuda::apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out)
+{
+    std::vector<std::complex<int16_t>> got;
+    const bool ok = falcon_dsp_fir_filter::apply(in, got);
+    out.insert(out.end(), got.begin(), got.end());
+    return ok;
+}
+
+bool fir_filter(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<int16_t>>& in,
+                std::vector<std::complex<int16_t>>& out)
+{
+    falcon_dsp_fir_filter f(coeffs);
+    return f.apply(in, out);
+}
+
+bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<int16_t>>& in,
+                     std::vector<std::complex<int16_t>>& out)
+{
+    falcon_dsp_fir_filter_cuda f(coeffs);
+    return f.apply(in, out);
+}
+
 // ---------------------------------------------------------------------------
 // polar discriminator (reference src/transform/falcon_dsp_polar_discriminator.cc:103-171)
 // ---------------------------------------------------------------------------
@@ -316,6 +471,141 @@ bool polar_discriminator(const std::vector<std::complex<float>>& in, std::vector
 {
     falcon_dsp_polar_discriminator d;
     return d.apply(in, out);
+}
+
+// reference falcon_dsp_polar_discriminator.cc:117-132
+bool falcon_dsp_polar_discriminator::apply(const std::vector<std::complex<int16_t>>& in, std::vector<float>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    size_t got = 0;
+    if (fdsp_discriminator_apply_ci16(m_handle, ci(in.data()), in.size(), out.data(), &got) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    out.resize(got);
+    return true;
+}
+
+bool polar_discriminator(const std::vector<std::complex<int16_t>>& in, std::vector<float>& out)
+{
+    falcon_dsp_polar_discriminator d;
+    return d.apply(in, out);
+}
+
+// ---------------------------------------------------------------------------
+// sample files (reference src/utilities/falcon_dsp_utils.cc:301-811)
+// ---------------------------------------------------------------------------
+namespace
+{
+uint32_t enc16(int16_t v) { return (uint16_t)v; }
+uint32_t encf(float v) { return f32_bits(v); }
+}  // namespace
+
+bool write_data_to_file(std::string file_name, file_type_e type, std::vector<int16_t>& data)
+{
+    return write_items(file_name, type, data.data(), data.size(), 1, 2, enc16);
+}
+
+bool write_data_to_file(std::string file_name, file_type_e type, std::vector<float>& data)
+{
+    return write_items(file_name, type, data.data(), data.size(), 1, 4, encf);
+}
+
+bool write_complex_data_to_file(std::string file_name, file_type_e type, std::vector<std::complex<int16_t>>& data)
+{
+    return write_items(file_name, type, reinterpret_cast<const int16_t *>(data.data()), data.size(), 2, 2, enc16);
+}
+
+bool write_complex_data_to_file(std::string file_name, file_type_e type, std::vector<std::complex<float>>& data)
+{
+    return write_items(file_name, type, reinterpret_cast<const float *>(data.data()), data.size(), 2, 4, encf);
+}
+
+bool read_data_from_file(std::string file_name, file_type_e type, std::vector<int16_t>& data)
+{
+    data.clear();
+    if (type == file_type_e::BINARY) {
+        std::vector<unsigned char> raw;
+        if (!read_binary(file_name, 2, raw))
+            return false;
+        for (size_t i = 0; i < raw.size(); i += 2)
+            data.push_back((int16_t)(uint16_t)get_le(&raw[i], 2));
+        return true;
+    }
+    if (type == file_type_e::ASCII) {
+        std::ifstream f(file_name, std::ios::in);
+        if (!f.is_open())
+            return false;
+        int16_t v;
+        while (f >> v)
+            data.push_back(v);
+        return true;
+    }
+    return false;
+}
+
+bool read_data_from_file(std::string file_name, file_type_e type, std::vector<float>& data)
+{
+    data.clear();
+    if (type != file_type_e::ASCII)
+        return false;  // binary floats: not supported, as in the reference (.cc:498-502)
+    std::ifstream f(file_name, std::ios::in);
+    if (!f.is_open())
+        return false;
+    float v;
+    while (f >> v)
+        data.push_back(v);
+    return true;
+}
+
+bool read_complex_data_from_file(std::string file_name, file_type_e type, std::vector<std::complex<int16_t>>& data)
+{
+    data.clear();
+    if (type == file_type_e::BINARY) {
+        std::vector<unsigned char> raw;
+        if (!read_binary(file_name, 4, raw))
+            return false;
+        for (size_t i = 0; i < raw.size(); i += 4)
+            data.emplace_back((int16_t)(uint16_t)get_le(&raw[i], 2), (int16_t)(uint16_t)get_le(&raw[i + 2], 2));
+        return true;
+    }
+    if (type == file_type_e::ASCII) {
+        std::ifstream f(file_name, std::ios::in);
+        if (!f.is_open())
+            return false;
+        int16_t re, im;
+        while (f >> re >> im)
+            data.emplace_back(re, im);
+        return true;
+    }
+    return false;
+}
+
+bool read_complex_data_from_file(std::string file_name, file_type_e type, std::vector<std::complex<float>>& data)
+{
+    data.clear();
+    if (type == file_type_e::BINARY) {
+        std::vector<unsigned char> raw;
+        if (!read_binary(file_name, 8, raw))
+            return false;
+        for (size_t i = 0; i < raw.size(); i += 8)
+            data.emplace_back(bits_f32(get_le(&raw[i], 4)), bits_f32(get_le(&raw[i + 4], 4)));
+        return true;
+    }
+    if (type == file_type_e::ASCII) {
+        std::ifstream f(file_name, std::ios::in);
+        if (!f.is_open())
+            return false;
+        float re, im;
+        while (f >> re >> im)
+            data.emplace_back(re, im);
+        return true;
+    }
+    return false;
 }
 
 // ---------------------------------------------------------------------------

@@ -496,6 +496,79 @@ int orc_channelizer_state(const orc_channelizer *c, uint32_t chan, double *sampl
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* complex int16 overloads                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* The reference builds its std::complex<int16_t> outputs from float parts, an
+ * implicit float -> int16_t conversion: truncation toward zero (e.g.
+ * fir_filter.cc:168-171). Outside the int16 range that conversion is
+ * undefined; this restatement saturates there (NaN reads as -32768), the
+ * choice the GPU build makes too. */
+int16_t orc_i16(float v)
+{
+    if (v != v || v <= -32768.0f)
+        return -32768;
+    if (v >= 32767.0f)
+        return 32767;
+    return (int16_t)(int32_t)v;
+}
+
+/* freq_shift.cc:133-162: each sample converted to float and shifted exactly
+ * as in the float overload (one angle, one counter step per sample), the
+ * product converted back part by part */
+size_t orc_freq_shift_apply_ci16(orc_freq_shift *s, const int16_t *in, size_t n, int16_t *out)
+{
+    if (!s->initialized)
+        return 0;
+    for (size_t i = 0; i < n; ++i) {
+        const float x[2] = {(float)in[2 * i], (float)in[2 * i + 1]};
+        float y[2];
+        orc_freq_shift_apply(s, x, 1, y);
+        out[2 * i] = orc_i16(y[0]);
+        out[2 * i + 1] = orc_i16(y[1]);
+    }
+    return n;
+}
+
+/* fir_filter.cc:151-175: the input converted to float, filtered by the float
+ * overload (whose history it joins), the outputs converted back */
+size_t orc_fir_apply_ci16(orc_fir *f, const int16_t *in, size_t n, int16_t *out)
+{
+    if (f->m == 0 || n == 0)
+        return 0;
+    float *x = (float *)malloc(2 * n * sizeof(float));
+    float *y = (float *)malloc(2 * n * sizeof(float));
+    size_t k = 0;
+    if (x && y) {
+        for (size_t i = 0; i < 2 * n; ++i)
+            x[i] = (float)in[i];
+        k = orc_fir_apply(f, x, n, y);
+        for (size_t i = 0; i < 2 * k; ++i)
+            out[i] = orc_i16(y[i]);
+    }
+    free(x);
+    free(y);
+    return k;
+}
+
+/* polar_discriminator.cc:117-132: the input converted to float, then the float
+ * overload */
+size_t orc_discrim_apply_ci16(orc_discrim *d, const int16_t *in, size_t n, float *out)
+{
+    if (n == 0)
+        return 0;
+    float *x = (float *)malloc(2 * n * sizeof(float));
+    size_t k = 0;
+    if (x) {
+        for (size_t i = 0; i < 2 * n; ++i)
+            x[i] = (float)in[i];
+        k = orc_discrim_apply(d, x, n, out);
+    }
+    free(x);
+    return k;
+}
+
 size_t orc_sizeof_freq_shift(void) { return sizeof(orc_freq_shift); }
 size_t orc_sizeof_resampler(void) { return sizeof(orc_resampler); }
 size_t orc_sizeof_fir(void) { return sizeof(orc_fir); }

@@ -94,6 +94,12 @@ void orc_discrim_reset(orc_discrim *d);
 /* out holds n floats; returns n-1 on the first call after reset, n otherwise */
 size_t orc_discrim_apply(orc_discrim *d, const float *in, size_t n, float *out);
 
+/* complex int16 overloads (interleaved re, im) */
+int16_t orc_i16(float v);
+size_t orc_freq_shift_apply_ci16(orc_freq_shift *s, const int16_t *in, size_t n, int16_t *out);
+size_t orc_fir_apply_ci16(orc_fir *f, const int16_t *in, size_t n, int16_t *out);
+size_t orc_discrim_apply_ci16(orc_discrim *d, const int16_t *in, size_t n, float *out);
+
 /* ---- multi-rate channelizer: src/transform/falcon_dsp_multi_rate_channelizer.cc ---- */
 typedef struct orc_channelizer {
     uint32_t n_chan;

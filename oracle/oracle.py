@@ -30,6 +30,20 @@ def _fptr(a: np.ndarray):
     return a.ctypes.data_as(_f32p)
 
 
+_i16p = C.POINTER(C.c_int16)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_i16p)
+
+
+def as_ci16(x) -> np.ndarray:
+    """(n, 2) int16 array of (re, im) pairs, the layout of std::complex<int16_t>."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int16))
+    assert a.ndim == 2 and a.shape[1] == 2
+    return a
+
+
 def as_cf32(x) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(x, dtype=np.complex64))
 
@@ -74,6 +88,12 @@ def lib():
         L.orc_discrim_reset.argtypes = [_vp]
         L.orc_discrim_apply.restype = C.c_size_t
         L.orc_discrim_apply.argtypes = [_vp, _f32p, C.c_size_t, _f32p]
+        L.orc_i16.restype = C.c_int16
+        L.orc_i16.argtypes = [C.c_float]
+        for name, outp in (("freq_shift", _i16p), ("fir", _i16p), ("discrim", _f32p)):
+            f = getattr(L, f"orc_{name}_apply_ci16")
+            f.restype = C.c_size_t
+            f.argtypes = [_vp, _i16p, C.c_size_t, outp]
         L.orc_channelizer_init.argtypes = [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_int64),
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                            C.POINTER(_f32p), C.POINTER(C.c_uint32)]
@@ -138,6 +158,13 @@ class FreqShift(_Obj):
         x = as_cf32(x)
         out = np.empty_like(x)
         n = lib().orc_freq_shift_apply(self.p, _fptr(x.view(np.float32)), x.size, _fptr(out.view(np.float32)))
+        return out[:n]
+
+    def apply_ci16(self, x):
+        """freq_shift.cc:133-162 (complex int16 in and out)."""
+        x = as_ci16(x)
+        out = np.empty_like(x)
+        n = lib().orc_freq_shift_apply_ci16(self.p, _iptr(x), x.shape[0], _iptr(out))
         return out[:n]
 
 
@@ -205,6 +232,13 @@ class Fir(_Obj):
         n = lib().orc_fir_apply(self.p, _fptr(x.view(np.float32)), x.size, _fptr(out.view(np.float32)))
         return out[:n]
 
+    def apply_ci16(self, x):
+        """fir_filter.cc:151-175 (complex int16 in and out)."""
+        x = as_ci16(x)
+        out = np.empty_like(x)
+        n = lib().orc_fir_apply_ci16(self.p, _iptr(x), x.shape[0], _iptr(out))
+        return out[:n]
+
 
 class Discriminator(_Obj):
     """falcon_dsp_polar_discriminator (src/transform/falcon_dsp_polar_discriminator.cc:103-171)."""
@@ -221,6 +255,13 @@ class Discriminator(_Obj):
         x = as_cf32(x)
         out = np.empty(x.size, dtype=np.float32)
         n = lib().orc_discrim_apply(self.p, _fptr(x.view(np.float32)), x.size, _fptr(out))
+        return out[:n]
+
+    def apply_ci16(self, x):
+        """polar_discriminator.cc:117-132 (complex int16 in)."""
+        x = as_ci16(x)
+        out = np.empty(x.shape[0], dtype=np.float32)
+        n = lib().orc_discrim_apply_ci16(self.p, _iptr(x), x.shape[0], _fptr(out))
         return out[:n]
 
 
@@ -318,6 +359,14 @@ def ref():
         L.ref_disc_apply.argtypes = [_vp, _f32p, C.c_size_t, _f32p]
         L.ref_disc_reset.argtypes = [_vp]
         L.ref_disc_destroy.argtypes = [_vp]
+        for name, outp in (("fs", _i16p), ("fir", _i16p), ("disc", _f32p)):
+            f = getattr(L, f"ref_{name}_apply_ci16")
+            f.restype = C.c_size_t
+            f.argtypes = [_vp, _i16p, C.c_size_t, outp]
+        L.ref_write_file.restype = C.c_int
+        L.ref_write_file.argtypes = [C.c_int, C.c_int, C.c_char_p, _vp, C.c_size_t]
+        L.ref_read_file.restype = C.c_int64
+        L.ref_read_file.argtypes = [C.c_int, C.c_int, C.c_char_p, _vp, C.c_size_t]
         L.ref_chan_create.restype = _vp
         L.ref_chan_create.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(_f32p),
@@ -347,6 +396,12 @@ class RefFreqShift:
         x = as_cf32(x)
         out = np.empty_like(x)
         n = ref().ref_fs_apply(self.h, _fptr(x.view(np.float32)), x.size, _fptr(out.view(np.float32)))
+        return out[:n]
+
+    def apply_ci16(self, x):
+        x = as_ci16(x)
+        out = np.empty_like(x)
+        n = ref().ref_fs_apply_ci16(self.h, _iptr(x), x.shape[0], _iptr(out))
         return out[:n]
 
 
@@ -393,6 +448,12 @@ class RefFir:
         n = ref().ref_fir_apply(self.h, _fptr(x.view(np.float32)), x.size, _fptr(out.view(np.float32)))
         return out[:n]
 
+    def apply_ci16(self, x):
+        x = as_ci16(x)
+        out = np.empty_like(x)
+        n = ref().ref_fir_apply_ci16(self.h, _iptr(x), x.shape[0], _iptr(out))
+        return out[:n]
+
 
 class RefDiscriminator:
     def __init__(self):
@@ -409,6 +470,12 @@ class RefDiscriminator:
         x = as_cf32(x)
         out = np.empty(x.size, dtype=np.float32)
         n = ref().ref_disc_apply(self.h, _fptr(x.view(np.float32)), x.size, _fptr(out))
+        return out[:n]
+
+    def apply_ci16(self, x):
+        x = as_ci16(x)
+        out = np.empty(x.shape[0], dtype=np.float32)
+        n = ref().ref_disc_apply_ci16(self.h, _iptr(x), x.shape[0], _fptr(out))
         return out[:n]
 
 

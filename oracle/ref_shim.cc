@@ -12,7 +12,9 @@
  *   falcon_dsp::falcon_dsp_multi_rate_channelizer hdr/transform/falcon_dsp_multi_rate_channelizer.h:97-134
  *   calculate_gcd / rat_approx                   hdr/utilities/falcon_dsp_utils.h
  */
+#include <algorithm>
 #include <complex>
+#include <string>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -136,6 +138,115 @@ size_t ref_disc_apply(void *h, const float *in, size_t n, float *out)
 }
 void ref_disc_reset(void *h) { static_cast<falcon_dsp::falcon_dsp_polar_discriminator *>(h)->reset_state(); }
 void ref_disc_destroy(void *h) { delete static_cast<falcon_dsp::falcon_dsp_polar_discriminator *>(h); }
+
+/* ---- complex int16 overloads of the classes above ---- */
+typedef std::complex<int16_t> ci16;
+static std::vector<ci16> to_vec16(const int16_t *p, size_t n)
+{
+    std::vector<ci16> v(n);
+    for (size_t i = 0; i < n; ++i)
+        v[i] = ci16(p[2 * i], p[2 * i + 1]);
+    return v;
+}
+size_t ref_fs_apply_ci16(void *h, const int16_t *in, size_t n, int16_t *out)
+{
+    auto v = to_vec16(in, n);
+    std::vector<ci16> o;
+    static_cast<falcon_dsp::falcon_dsp_freq_shift *>(h)->apply(v, o);
+    for (size_t i = 0; i < o.size(); ++i) {
+        out[2 * i] = o[i].real();
+        out[2 * i + 1] = o[i].imag();
+    }
+    return o.size();
+}
+size_t ref_fir_apply_ci16(void *h, const int16_t *in, size_t n, int16_t *out)
+{
+    auto v = to_vec16(in, n);
+    std::vector<ci16> o;
+    static_cast<falcon_dsp::falcon_dsp_fir_filter *>(h)->apply(v, o);
+    for (size_t i = 0; i < o.size(); ++i) {
+        out[2 * i] = o[i].real();
+        out[2 * i + 1] = o[i].imag();
+    }
+    return o.size();
+}
+size_t ref_disc_apply_ci16(void *h, const int16_t *in, size_t n, float *out)
+{
+    auto v = to_vec16(in, n);
+    std::vector<float> o;
+    static_cast<falcon_dsp::falcon_dsp_polar_discriminator *>(h)->apply(v, o);
+    if (!o.empty())
+        std::memcpy(out, o.data(), o.size() * sizeof(float));
+    return o.size();
+}
+
+/* ---- sample files (falcon_dsp_utils.cc:301-811) ----
+ * kind: 0 ci16, 1 cf32, 2 int16, 3 float; ascii selects file_type_e::ASCII.
+ * data holds n items of interleaved values (2 per complex item). */
+static falcon_dsp::file_type_e ftype(int ascii)
+{
+    return ascii ? falcon_dsp::file_type_e::ASCII : falcon_dsp::file_type_e::BINARY;
+}
+int ref_write_file(int kind, int ascii, const char *path, const void *data, size_t n)
+{
+    const std::string name(path);
+    switch (kind) {
+    case 0: {
+        auto v = to_vec16(static_cast<const int16_t *>(data), n);
+        return falcon_dsp::write_complex_data_to_file(name, ftype(ascii), v) ? 0 : -1;
+    }
+    case 1: {
+        auto v = to_vec(static_cast<const float *>(data), n);
+        return falcon_dsp::write_complex_data_to_file(name, ftype(ascii), v) ? 0 : -1;
+    }
+    case 2: {
+        std::vector<int16_t> v(static_cast<const int16_t *>(data), static_cast<const int16_t *>(data) + n);
+        return falcon_dsp::write_data_to_file(name, ftype(ascii), v) ? 0 : -1;
+    }
+    default: {
+        std::vector<float> v(static_cast<const float *>(data), static_cast<const float *>(data) + n);
+        return falcon_dsp::write_data_to_file(name, ftype(ascii), v) ? 0 : -1;
+    }
+    }
+}
+/* items read (at most cap are copied out), or -1 when the reader returns false */
+int64_t ref_read_file(int kind, int ascii, const char *path, void *out, size_t cap)
+{
+    const std::string name(path);
+    switch (kind) {
+    case 0: {
+        std::vector<ci16> v;
+        if (!falcon_dsp::read_complex_data_from_file(name, ftype(ascii), v))
+            return -1;
+        for (size_t i = 0; i < v.size() && i < cap; ++i) {
+            static_cast<int16_t *>(out)[2 * i] = v[i].real();
+            static_cast<int16_t *>(out)[2 * i + 1] = v[i].imag();
+        }
+        return (int64_t)v.size();
+    }
+    case 1: {
+        std::vector<cf32> v;
+        if (!falcon_dsp::read_complex_data_from_file(name, ftype(ascii), v))
+            return -1;
+        std::memcpy(out, v.data(), std::min(v.size(), cap) * sizeof(cf32));
+        return (int64_t)v.size();
+    }
+    case 2: {
+        std::vector<int16_t> v;
+        if (!falcon_dsp::read_data_from_file(name, ftype(ascii), v))
+            return -1;
+        std::memcpy(out, v.data(), std::min(v.size(), cap) * sizeof(int16_t));
+        return (int64_t)v.size();
+    }
+    default: {
+        std::vector<float> v;
+        if (!falcon_dsp::read_data_from_file(name, ftype(ascii), v))
+            return -1;
+        std::memcpy(out, v.data(), std::min(v.size(), cap) * sizeof(float));
+        return (int64_t)v.size();
+    }
+    }
+}
 
 /* ---- multi-rate channelizer ---- */
 void *ref_chan_create(uint32_t fs, uint32_t n_chan, const int64_t *shift_hz, const uint32_t *out_rate,

@@ -3,7 +3,9 @@
 // classes, with their known answers (reference test/tests/transform/*_unit_tests,
 // test/tests/utilities/falcon_dsp_utils_unit_tests.cc), plus closed forms.
 // Needs a GPU. Prints one line per check; exit status 0 when all pass.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <cstdio>
 #include <cstdint>
@@ -261,12 +263,102 @@ static void test_channelizer()
     check(ok, "channelizer reset_state restarts the stream");
 }
 
+// complex int16 overloads: the float path of the same object type, then
+// truncation toward zero (reference fir_filter.cc:151-175,
+// freq_shift.cc:133-162, polar_discriminator.cc:117-132)
+using ci = std::complex<int16_t>;
+static void test_ci16()
+{
+    std::vector<cf> h = {{0.5f, 0}, {2.0f, 0}, {4.0f, 0}};
+    {
+        // basic_filter_float_003 on int16 input, both signs: 0.5 -> 0, -2.5 -> -2
+        std::vector<ci> x = {{1, -1}, {1, -1}, {1, -1}, {2, -2}, {2, -2}, {2, -2}, {5, -5}, {5, -5}, {5, -5},
+                             {1, -1}, {1, -1}};
+        const int16_t want[] = {0, 2, 6, 7, 9, 13, 14, 20, 32, 30, 22};
+        std::vector<ci> y = {{7, 7}};
+        falcon_dsp_fir_filter f(h);
+        bool ok = f.apply(x, y) && y.size() == 11;  // the CPU class clears out
+        for (int i = 0; ok && i < 11; ++i)
+            ok = y[i] == ci(want[i], (int16_t)-want[i]);
+        falcon_dsp_fir_filter_cuda g(h);
+        std::vector<ci> z = {{7, 7}};
+        ok = ok && g.apply(x, z) && z.size() == 12 && z[0] == ci(7, 7);  // the GPU class appends
+        for (int i = 0; ok && i < 11; ++i)
+            ok = z[i + 1] == y[i];
+        std::vector<ci> w;
+        ok = ok && fir_filter(h, x, w) && w == y && fir_filter_cuda(h, x, w) && w.size() == 22;
+        check(ok, "fir ci16: float_003 truncated toward zero; CPU class clears, GPU class appends");
+    }
+    {
+        // the ci16 shift equals the truncated cf32 shift of the same input
+        std::vector<ci> x(50000);
+        auto n = noise(11, x.size());
+        std::vector<cf> xf(x.size());
+        for (size_t i = 0; i < x.size(); ++i) {
+            // |x| stays well inside int16 (unit-variance noise x 5000), so every
+            // rotated part does too
+            x[i] = ci((int16_t)(n[i].real() * 5000), (int16_t)(n[i].imag() * 5000));
+            xf[i] = cf(x[i].real(), x[i].imag());
+        }
+        falcon_dsp_freq_shift a, b;
+        std::vector<ci> y16, part;
+        std::vector<cf> yf, partf;
+        bool ok = a.initialize(2000000, -170001) && b.initialize(2000000, -170001);
+        size_t at = 0;
+        for (size_t m : {20000, 1, 29999}) {
+            std::vector<ci> in(x.begin() + at, x.begin() + at + m);
+            std::vector<cf> inf(xf.begin() + at, xf.begin() + at + m);
+            ok = ok && a.apply(in, part) && b.apply(inf, partf);
+            y16.insert(y16.end(), part.begin(), part.end());
+            yf.insert(yf.end(), partf.begin(), partf.end());
+            at += m;
+        }
+        // int16 outputs follow the reference's arithmetic exactly
+        // (freq_shift.cc:146-151): angle = w * s in double, cos and sin rounded
+        // to float, the std::complex<float> product, truncated
+        const auto pw = falcon_dsp_freq_shift::get_freq_shift_params(2000000, -170001);
+        size_t differ = 0;
+        int worst = 0;
+        for (size_t i = 0; ok && i < x.size(); ++i) {
+            const double ang = pw.second * (double)(i % pw.first);
+            const cf want = cf(x[i].real(), x[i].imag()) * cf((float)std::cos(ang), (float)std::sin(ang));
+            const int dr = std::abs(y16[i].real() - (int)(int16_t)want.real());
+            const int di = std::abs(y16[i].imag() - (int)(int16_t)want.imag());
+            worst = std::max(worst, std::max(dr, di));
+            differ += (dr != 0) + (di != 0);
+            // the fast cf32 path, truncated, stays within one step of it
+            ok = std::abs(y16[i].real() - (int)yf[i].real()) <= 1 && std::abs(y16[i].imag() - (int)yf[i].imag()) <= 1;
+        }
+        std::printf("  ci16 shift vs the reference's arithmetic: %zu of %zu parts differ (max %d)\n", differ,
+                    2 * x.size(), worst);
+        ok = ok && y16.size() == x.size() && differ == 0;
+        falcon_dsp_freq_shift_cuda c;
+        std::vector<ci> z = {{1, 2}};
+        ok = ok && c.initialize(2000000, -170001) && c.apply(x, z) && z.size() == x.size() + 1 && z[1] == y16[0];
+        std::vector<ci> w;
+        ok = ok && freq_shift(2000000, x, -170001, w) && w == y16;  // one call equals three chunks, bit for bit
+        check(ok, "freq shift ci16 follows the reference's arithmetic bit for bit across chunks; GPU class appends");
+        // the discriminator on int16 equals the one on the same values as floats
+        falcon_dsp_polar_discriminator d16, df;
+        std::vector<float> o16, of;
+        ok = d16.apply(x, o16) && df.apply(xf, of) && o16.size() == of.size() && o16.size() == x.size() - 1;
+        float dmax = 0;
+        for (size_t i = 0; ok && i < o16.size(); ++i)
+            dmax = std::max(dmax, std::fabs(o16[i] - of[i]));
+        std::printf("  ci16 discriminator vs float input: max %.3g rad\n", dmax);
+        std::vector<float> o2;
+        ok = ok && o16 == of && polar_discriminator(x, o2) && o2 == o16;
+        check(ok, "polar discriminator ci16 equals the float input");
+    }
+}
+
 int main()
 {
     test_utils();
     test_resampler_kats();
     test_fir_and_discriminator();
     test_freq_shift();
+    test_ci16();
     test_channelizer();
     std::printf("%s: %d failed\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
