@@ -178,7 +178,27 @@ def bench_other_configs(fd, stream, steps):
                      "unit": "GB/s", "frac": round(b2 / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "note": "2^24 inputs in 64 chunks of 2^18, 16 shifts, 136 B per input sample"}
     log(f"nco_16shift: {ms2:.3f} ms per pass, {out['nco_16shift']['roofline']['achieved']} GB/s")
-    del d2, o2, f, x2
+    del d2, o2, x2
+
+    # the same shifts on complex int16 samples in and out (4 B each way)
+    x3 = w2.input_ci16()
+    d3 = fd.DeviceBuffer.from_array(x3.view(fd.CI16).ravel())
+    o3 = fd.DeviceBuffer(16 * w2.chunk, fd.CI16)
+
+    def pass3():
+        for k in range(w2.n // w2.chunk):
+            f.apply_ci16_device(d3, w2.chunk, o3, w2.chunk, stream, in_offset=k * w2.chunk)
+
+    ms3 = timed_launches(fd, stream, pass3, max(2, steps // 4))
+    b3 = 4 * w2.n + 4 * 16 * w2.n
+    out["nco_16shift_ci16"] = {
+        "value": round(16 * w2.n / (ms3 / 1e3) / 1e6, 1), "unit": "output Msps", "ms_per_pass": round(ms3, 4),
+        "roofline": {"bound": "hbm", "achieved": round(b3 / (ms3 / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(b3 / (ms3 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "note": "as nco_16shift with complex int16 in and out (68 B per input sample); FP64 phasor per "
+                "sample, as the reference computes int16 outputs"}
+    log(f"nco_16shift_ci16: {ms3:.3f} ms per pass, {out['nco_16shift_ci16']['roofline']['achieved']} GB/s")
+    del d3, o3, f, x3
 
     # fir_bank_256: 256 channels x 2^20, 512 real taps
     w4 = FirBank256()

@@ -271,9 +271,9 @@ class FreqShift(_Handle):
         return outs[0] if self.single else outs
 
     def apply_ci16_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, out_stride: int,
-                          stream: Optional[Stream] = None):
+                          stream: Optional[Stream] = None, in_offset: int = 0):
         """Device buffers of CI16 samples (DeviceBuffer(count, CI16))."""
-        check(lib().fdsp_freq_shift_apply_ci16_device(self.h, d_in.ptr, n, d_out.ptr, out_stride,
+        check(lib().fdsp_freq_shift_apply_ci16_device(self.h, d_in.at(in_offset), n, d_out.ptr, out_stride,
                                                       stream.h if stream else None))
 
 

@@ -78,6 +78,13 @@ class Nco16:
     def input(self) -> np.ndarray:
         return cwgn(1, self.n)
 
+    def input_ci16(self, scale: float = 4096.0) -> np.ndarray:
+        """The same noise as complex int16 (an (n, 2) int16 array), rounded from
+        `scale` times the float samples: an SDR front end's sample format."""
+        x = self.input()
+        v = np.stack([np.round(x.real * scale), np.round(x.imag * scale)], axis=1)
+        return v.clip(-32768, 32767).astype(np.int16)
+
 
 @dataclass
 class Chan32:
