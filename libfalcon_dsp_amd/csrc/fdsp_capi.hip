@@ -20,6 +20,8 @@
 #include <string>
 #include <vector>
 
+#include <type_traits>
+
 #include "../../include/fdsp.h"
 #include "fdsp_host.h"
 #include "fdsp_internal.h"
@@ -262,6 +264,8 @@ struct fdsp_freq_shift_s {
     DevBuf d_chans, d_in, d_out;
     uint64_t start = 0;  // samples since reset (the NCO index is start mod R)
     hipStream_t stream = nullptr;
+    DevBuf d_tab;            // int16 output: the shifts' phasor tables, built at the first such call
+    bool tab_built = false;
 };
 
 extern "C" int fdsp_freq_shift_create(uint32_t fs, const int32_t *shifts_hz, uint32_t n_shifts,
@@ -297,6 +301,7 @@ extern "C" int fdsp_freq_shift_destroy(fdsp_freq_shift_t h)
         return FDSP_OK;
     (void)hipStreamSynchronize(h->stream);
     h->d_chans.release();
+    h->d_tab.release();
     h->d_in.release();
     h->d_out.release();
     (void)hipStreamDestroy(h->stream);
@@ -324,6 +329,38 @@ extern "C" int fdsp_freq_shift_get_state(fdsp_freq_shift_t h, uint32_t chan, uin
 
 namespace {
 
+// Phasor tables for int16 output: for each shift with R <= 2^22, the
+// reference's ((float)cos(omega s), (float)sin(omega s)), s = 0 .. R-1, formed
+// on the host exactly as falcon_dsp_freq_shift.cc:146-148 forms them (FP64
+// angle and libm, then float). Up to 32 MiB per shift; larger R keep the
+// device's FP64 sincos.
+constexpr uint32_t kTableMaxR = 1u << 22;
+int build_phasor_tables(fdsp_freq_shift_t h)
+{
+    std::vector<float2> tab;
+    std::vector<NcoChan> chans = h->chans;
+    for (auto &c : chans) {
+        if (c.nu == 0.0 || c.r > kTableMaxR || tab.size() + c.r >= kNoTable)
+            continue;
+        c.tab = (uint32_t)tab.size();
+        for (uint32_t s = 0; s < c.r; ++s) {
+            const double ang = c.omega * (double)s;
+            tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+        }
+    }
+    if (!tab.empty()) {
+        if (int rc = h->d_tab.ensure(tab.size() * sizeof(float2)))
+            return rc;
+        // queued launches read d_chans; their fields other than tab do not change
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipMemcpy(h->d_tab.p, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->d_chans.p, chans.data(), sizeof(NcoChan) * chans.size(), hipMemcpyHostToDevice));
+        h->chans = chans;
+    }
+    h->tab_built = true;
+    return FDSP_OK;
+}
+
 // T: float2 (cf32) or short2 (ci16) samples, in and out alike
 template <typename T>
 int freq_shift_device(fdsp_freq_shift_t h, const T *d_in, size_t n, T *d_out, size_t out_stride,
@@ -337,8 +374,12 @@ int freq_shift_device(fdsp_freq_shift_t h, const T *d_in, size_t n, T *d_out, si
         return set_err(FDSP_EINVAL, "bad buffers");
     if (int rc = check_chunk(n, 1))
         return rc;
+    constexpr bool kInt = std::is_same<T, short2>::value;
+    if (kInt && !h->tab_built)
+        if (int rc = build_phasor_tables(h))
+            return rc;
     HIP_TRY(launch_nco_multi(d_in, (uint32_t)n, h->start, h->d_chans.as<NcoChan>(), (uint32_t)h->chans.size(), d_out,
-                             out_stride, pick(h->stream, stream)));
+                             out_stride, pick(h->stream, stream), kInt ? h->d_tab.as<float2>() : nullptr));
     h->start += n;
     return FDSP_OK;
 }

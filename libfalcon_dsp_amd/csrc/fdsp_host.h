@@ -109,6 +109,7 @@ inline hipStream_t pick(hipStream_t own, fdsp_stream_t s) { return s ? static_ca
 inline NcoChan make_nco(uint32_t fs, int32_t f)
 {
     NcoChan c{};
+    c.tab = kNoTable;
     uint32_t r = 1;
     double w = 0;
     fdsp_freq_shift_params(fs, f, &r, &w);

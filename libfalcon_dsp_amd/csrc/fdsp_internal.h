@@ -80,7 +80,7 @@ struct NcoChan {
     double nu;        // f/fs, signed, revolutions per sample
     double inv_r;     // 1/R
     uint32_t r;       // rollover index R (UINT32_MAX for the identity shift)
-    uint32_t pad;
+    uint32_t tab;     // first entry of this shift's phasor table (kNoTable: none)
     float2 step[4];   // e^{j 2 pi nu i}, i = 0..3
     double omega;     // 2 pi f / fs rad/sample, formed as the reference forms it
 };
@@ -109,10 +109,13 @@ struct ChanPos {
 // launchers (all asynchronous on `s`)
 // ---------------------------------------------------------------------------
 // TI, TO: float2 (cf32) or short2 (ci16) samples, in the pairs
-// (float2, float2) and (short2, short2)
+// (float2, float2) and (short2, short2). tab (int16 output only, may be
+// null): per-shift tables of the reference's phasors, entry NcoChan::tab + s
+// holding ((float)cos(omega s), (float)sin(omega s)) for s < R.
+constexpr uint32_t kNoTable = 0xFFFFFFFFu;
 template <typename TI, typename TO>
 hipError_t launch_nco_multi(const TI *in, uint32_t n, uint64_t start, const NcoChan *chans, uint32_t n_chan,
-                            TO *out, uint64_t out_stride, hipStream_t s);
+                            TO *out, uint64_t out_stride, hipStream_t s, const float2 *tab = nullptr);
 
 // polyphase resampler over one chunk (global input position p); `taps` are
 // the real parts of the transposed bank, up*cpp floats. hist_in holds cpp-1

@@ -24,8 +24,9 @@ __device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) { retur
 // channel, the other three by a product with e^{j 2 pi nu i} (two roundings).
 // Complex int16 output instead takes the reference's own arithmetic
 // (falcon_dsp_freq_shift.cc:146-151): per sample, angle = omega * s in FP64
-// with s the index modulo R, cos and sin in FP64 rounded to float, and the
-// float complex product, so the truncated outputs equal the reference's.
+// with s the index modulo R, cos and sin in FP64 rounded to float (from the
+// host-built table where there is one, else sincos here), and the float
+// complex product, so the truncated outputs equal the reference's.
 // ---------------------------------------------------------------------------
 constexpr int kNcoL = 4;
 
@@ -66,8 +67,12 @@ __device__ __forceinline__ void store4(short2 *p, const float2 (&y)[4])
 template <bool kVec, typename TI, typename TO>
 __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ in, uint32_t n, uint64_t start,
                                                         const NcoChan *__restrict__ chans, uint32_t n_chan,
-                                                        TO *__restrict__ out, uint64_t out_stride)
+                                                        TO *__restrict__ out, uint64_t out_stride,
+                                                        const float2 *__restrict__ tab, uint32_t cpb)
 {
+    // blockIdx.y picks the shifts c0 .. c1-1 (cpb per block): small chunks
+    // spread their shifts over more blocks, re-reading the input from L2
+    const uint32_t c0 = blockIdx.y * cpb, c1 = min(n_chan, c0 + cpb);
     const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kNcoL;
     if (i0 >= n)
         return;
@@ -80,19 +85,26 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ i
         for (int i = 0; i < kNcoL; ++i)
             x[i] = (i0 + i < n) ? sample_f32(in[i0 + i]) : make_float2(0.f, 0.f);
     }
-    for (uint32_t c = 0; c < n_chan; ++c) {
+    for (uint32_t c = c0; c < c1; ++c) {
         const double nu = chans[c].nu;
         const uint32_t r = chans[c].r;
         const uint64_t k0 = mod_u64(start + i0, r, chans[c].inv_r);
         float2 y[kNcoL];
         if constexpr (std::is_same<TO, short2>::value) {
             const double w = chans[c].omega;
+            const uint32_t t0 = chans[c].tab;
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i) {
                 const uint64_t k = k0 + i >= r ? k0 + i - r : k0 + i;
-                double sn, cs;
-                sincos(w * (double)k, &sn, &cs);
-                y[i] = cmul_ref(x[i], make_float2((float)cs, (float)sn));
+                float2 ph;
+                if (tab && t0 != kNoTable) {
+                    ph = tab[t0 + k];
+                } else {
+                    double sn, cs;
+                    sincos(w * (double)k, &sn, &cs);
+                    ph = make_float2((float)cs, (float)sn);
+                }
+                y[i] = cmul_ref(x[i], ph);
             }
         } else if (k0 + (kNcoL - 1) < r) {
             const float2 base = nco_phasor((int64_t)k0, nu);
@@ -121,27 +133,32 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ i
 
 template <typename TI, typename TO>
 hipError_t launch_nco_multi(const TI *in, uint32_t n, uint64_t start, const NcoChan *chans, uint32_t n_chan, TO *out,
-                            uint64_t out_stride, hipStream_t s)
+                            uint64_t out_stride, hipStream_t s, const float2 *tab)
 {
     if (n == 0)
         return hipSuccess;
     const uint32_t threads = (n + kNcoL - 1) / kNcoL;
     const uint32_t blocks = (threads + 255) / 256;
+    // shifts per block: all of them once the chunk alone gives 2048 blocks
+    // (8 per CU), else split so the grid reaches about that
+    const uint32_t groups = min(n_chan, max(1u, 2048u / blocks));
+    const uint32_t cpb = (n_chan + groups - 1) / groups;
+    const dim3 grid(blocks, (n_chan + cpb - 1) / cpb);
     // 16-byte accesses of 4 samples: every channel's output must start 16-byte aligned
     const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16 == 0) &&
                      (out_stride * sizeof(TO)) % 16 == 0;
     if (vec)
-        hipLaunchKernelGGL((nco_multi_kernel<true, TI, TO>), dim3(blocks), dim3(256), 0, s, in, n, start, chans,
-                           n_chan, out, out_stride);
+        hipLaunchKernelGGL((nco_multi_kernel<true, TI, TO>), grid, dim3(256), 0, s, in, n, start, chans, n_chan, out,
+                           out_stride, tab, cpb);
     else
-        hipLaunchKernelGGL((nco_multi_kernel<false, TI, TO>), dim3(blocks), dim3(256), 0, s, in, n, start, chans,
-                           n_chan, out, out_stride);
+        hipLaunchKernelGGL((nco_multi_kernel<false, TI, TO>), grid, dim3(256), 0, s, in, n, start, chans, n_chan,
+                           out, out_stride, tab, cpb);
     return hipGetLastError();
 }
 template hipError_t launch_nco_multi<float2, float2>(const float2 *, uint32_t, uint64_t, const NcoChan *, uint32_t,
-                                                     float2 *, uint64_t, hipStream_t);
+                                                     float2 *, uint64_t, hipStream_t, const float2 *);
 template hipError_t launch_nco_multi<short2, short2>(const short2 *, uint32_t, uint64_t, const NcoChan *, uint32_t,
-                                                     short2 *, uint64_t, hipStream_t);
+                                                     short2 *, uint64_t, hipStream_t, const float2 *);
 
 // ---------------------------------------------------------------------------
 // Polyphase resampler, one stream

@@ -44,7 +44,9 @@ SIZES = [65536, 1, 7, 100003, 4096, 2]
 def test_freq_shift_ci16_chunked_vs_oracle():
     fs = 25_000_000
     x = ci16_noise(1, sum(SIZES), 6000.0)
-    for f in (1_234_567, -9_999_999, 7):
+    # R = 202500, 250000 and 3571429 take the host-built phasor table; 5 Hz
+    # (R = 5e6, above 2^22) takes the FP64 sincos on the device
+    for f in (1_234_567, -9_999_999, 7, 5):
         g, o = fd.FreqShift(fs, f), O.FreqShift(fs, f)
         got, want, i = [], [], 0
         for s in SIZES:
