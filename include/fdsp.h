@@ -187,6 +187,44 @@ int fdsp_discriminator_apply_ci16(fdsp_discriminator_t h, const fdsp_ci16 *in, s
 int fdsp_discriminator_apply_ci16_device(fdsp_discriminator_t h, const fdsp_ci16 *d_in, size_t n, float *d_out,
                                          size_t *n_out, fdsp_stream_t stream);
 
+/* ---- first-order IIR (real samples) ----
+ * Replaces falcon_dsp_iir_filter::{initialize(b = [b0], a = [1, a1]),
+ * apply(vector<float>&, vector<float>&), reset_state} (reference
+ * src/transform/falcon_dsp_iir_filter.cc:104-203,235-396) for the first-order
+ * filters the FM de-emphasis uses: y[n] = b0 x[n] - a1 y[n-1], the output
+ * carried between calls, rounded as the reference's sequential loop rounds it
+ * (bit-identical outputs). |a1| <= 0.94. */
+typedef struct fdsp_iir1_s *fdsp_iir1_t;
+int fdsp_iir1_create(float b0, float a1, fdsp_iir1_t *out);
+int fdsp_iir1_destroy(fdsp_iir1_t h);
+int fdsp_iir1_reset(fdsp_iir1_t h);
+int fdsp_iir1_apply(fdsp_iir1_t h, const float *in, size_t n, float *out);
+int fdsp_iir1_apply_device(fdsp_iir1_t h, const float *d_in, size_t n, float *d_out, fdsp_stream_t stream);
+
+/* ---- FM mono demodulator ----
+ * Replaces falcon_dsp_fm_demodulator::{initialize, demod_mono, reset_state}
+ * (reference src/signals/falcon_dsp_fm_demod.cc:98-271): shift by the signal's
+ * offset, resample to the FM band rate, polar discriminator, single-pole
+ * de-emphasis (time constant tau, coefficients from fdsp_fm_deemphasis),
+ * resample to the audio rate, then scale each chunk to a peak of 10000 and
+ * convert to int16 (truncation; all zero for an all-zero chunk). The
+ * reference reads both resamplers' (up, down, taps) from its predefined
+ * REMEZ-64 table for 1,140,000 -> 228,000 -> 45,600 sps; here the caller
+ * passes them. audio (optional) receives the samples before scaling. */
+int fdsp_fm_deemphasis(double tau, float *b0, float *a1);
+typedef struct fdsp_fm_demod_s *fdsp_fm_demod_t;
+int fdsp_fm_demod_create(uint32_t fs, int32_t offset_hz, double tau, uint32_t iso_up, uint32_t iso_down,
+                         const fdsp_cf32 *iso_taps, uint32_t n_iso, uint32_t mono_up, uint32_t mono_down,
+                         const fdsp_cf32 *mono_taps, uint32_t n_mono, fdsp_fm_demod_t *out);
+int fdsp_fm_demod_destroy(fdsp_fm_demod_t h);
+int fdsp_fm_demod_reset(fdsp_fm_demod_t h);
+/* most outputs the next n-sample chunk can emit */
+int fdsp_fm_demod_out_count(fdsp_fm_demod_t h, size_t n, size_t *count);
+int fdsp_fm_demod_apply(fdsp_fm_demod_t h, const fdsp_cf32 *in, size_t n, int16_t *out, float *audio, size_t cap,
+                        size_t *n_out);
+int fdsp_fm_demod_apply_device(fdsp_fm_demod_t h, const fdsp_cf32 *d_in, size_t n, int16_t *d_out, float *d_audio,
+                               size_t cap, size_t *n_out, fdsp_stream_t stream);
+
 /* ---- multi-rate channelizer ----
  * Replaces falcon_dsp_multi_rate_channelizer{,_cuda}::{initialize, apply, reset_state}
  * (reference hdr/transform/falcon_dsp_multi_rate_channelizer.h:70-134,

@@ -162,6 +162,19 @@ _SIGS = [
     ("fdsp_discriminator_apply_device", _i, [_vp, _vp, _sz, _vp, _P(_sz), _vp]),
     ("fdsp_discriminator_apply_ci16", _i, [_vp, _vp, _sz, _vp, _P(_sz)]),
     ("fdsp_discriminator_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _P(_sz), _vp]),
+    ("fdsp_iir1_create", _i, [C.c_float, C.c_float, _P(_vp)]),
+    ("fdsp_iir1_destroy", _i, [_vp]),
+    ("fdsp_iir1_reset", _i, [_vp]),
+    ("fdsp_iir1_apply", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir1_apply_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_fm_deemphasis", _i, [C.c_double, _P(C.c_float), _P(C.c_float)]),
+    ("fdsp_fm_demod_create", _i, [_u32, C.c_int32, C.c_double, _u32, _u32, _vp, _u32, _u32, _u32, _vp, _u32,
+                                  _P(_vp)]),
+    ("fdsp_fm_demod_destroy", _i, [_vp]),
+    ("fdsp_fm_demod_reset", _i, [_vp]),
+    ("fdsp_fm_demod_out_count", _i, [_vp, _sz, _P(_sz)]),
+    ("fdsp_fm_demod_apply", _i, [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]),
+    ("fdsp_fm_demod_apply_device", _i, [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz), _vp]),
     ("fdsp_channelizer_create", _i, [_u32, _P(ChannelDesc), _u32, _i, _P(_vp)]),
     ("fdsp_channelizer_destroy", _i, [_vp]),
     ("fdsp_channelizer_reset", _i, [_vp]),

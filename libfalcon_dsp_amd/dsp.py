@@ -419,6 +419,73 @@ class PolarDiscriminator(_Handle):
         return out[: n.value]
 
 
+class Iir1(_Handle):
+    """First-order IIR on real samples, y[n] = b0 x[n] - a1 y[n-1]
+    (falcon_dsp_iir_filter with b = [b0], a = [1, a1]; reference
+    iir_filter.cc:104-396); outputs bit-identical to the reference's loop."""
+    _destroy = "fdsp_iir1_destroy"
+
+    def __init__(self, b0: float, a1: float):
+        super().__init__()
+        check(lib().fdsp_iir1_create(float(b0), float(a1), C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_iir1_reset(self.h))
+
+    def apply(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty_like(x)
+        check(lib().fdsp_iir1_apply(self.h, _ptr(x), x.size, _ptr(out)))
+        return out
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None):
+        check(lib().fdsp_iir1_apply_device(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
+
+
+def fm_deemphasis(tau: float):
+    """(b0, a1) of the FM de-emphasis for time constant tau (reference fm_demod.cc:146-165)."""
+    b0, a1 = C.c_float(), C.c_float()
+    check(lib().fdsp_fm_deemphasis(float(tau), C.byref(b0), C.byref(a1)))
+    return b0.value, a1.value
+
+
+class FmDemodulator(_Handle):
+    """falcon_dsp_fm_demodulator::demod_mono (reference src/signals/falcon_dsp_fm_demod.cc:98-271):
+    shift, resample to the FM band, discriminate, de-emphasise, resample to
+    audio, scale to a peak of 10000 and convert to int16. iso and mono are
+    the two resamplers' (up, down, taps), which the reference takes from its
+    predefined table."""
+    _destroy = "fdsp_fm_demod_destroy"
+    AMERICAS_TAU = 75e-6
+    EUROPE_ASIA_TAU = 50e-6
+
+    def __init__(self, fs: int, offset_hz: int, iso, mono, tau: float = AMERICAS_TAU):
+        super().__init__()
+        self._iso_t, self._mono_t = _cf32(iso[2]), _cf32(mono[2])
+        check(lib().fdsp_fm_demod_create(fs, int(offset_hz), float(tau), iso[0], iso[1], _ptr(self._iso_t),
+                                         self._iso_t.size, mono[0], mono[1], _ptr(self._mono_t), self._mono_t.size,
+                                         C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_fm_demod_reset(self.h))
+
+    def out_count(self, n: int) -> int:
+        v = C.c_size_t()
+        check(lib().fdsp_fm_demod_out_count(self.h, n, C.byref(v)))
+        return v.value
+
+    def demod_mono(self, x, audio: bool = False):
+        """-> int16 samples, or (int16, float samples before scaling) with audio=True."""
+        x = _cf32(x)
+        cap = max(self.out_count(x.size), 1)
+        out = np.empty(cap, dtype=np.int16)
+        a = np.empty(cap, dtype=np.float32) if audio else None
+        n = C.c_size_t()
+        check(lib().fdsp_fm_demod_apply(self.h, _ptr(x), x.size, _ptr(out), _ptr(a) if audio else None, cap,
+                                        C.byref(n)))
+        return (out[: n.value], a[: n.value]) if audio else out[: n.value]
+
+
 class MultiRateChannelizer(_Handle):
     """falcon_dsp_multi_rate_channelizer{,_cuda}: per channel, NCO shift then
     polyphase resample.

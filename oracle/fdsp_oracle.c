@@ -391,7 +391,9 @@ void orc_discrim_reset(orc_discrim *d)
 }
 
 /* polar_discriminator.cc:134-171: out = atan2(Im z, Re z), z = x[n]*conj(x[n-1]);
- * the first sample pairs with the saved last sample when one exists. */
+ * the first sample pairs with the saved last sample when one exists. The
+ * reference's unqualified atan2 on float parts is the double overload, its
+ * result rounded to float when stored. */
 size_t orc_discrim_apply(orc_discrim *d, const float *in, size_t n, float *out)
 {
     if (n == 0)
@@ -400,11 +402,11 @@ size_t orc_discrim_apply(orc_discrim *d, const float *in, size_t n, float *out)
     float zr, zi;
     if (d->has_state) {
         cmul(in[0], in[1], d->last[0], -d->last[1], &zr, &zi);
-        out[k++] = atan2f(zi, zr);
+        out[k++] = (float)atan2((double)zi, (double)zr);
     }
     for (size_t i = 1; i < n; ++i) {
         cmul(in[2 * i], in[2 * i + 1], in[2 * i - 2], -in[2 * i - 1], &zr, &zi);
-        out[k++] = atan2f(zi, zr);
+        out[k++] = (float)atan2((double)zi, (double)zr);
     }
     d->last[0] = in[2 * (n - 1)];
     d->last[1] = in[2 * (n - 1) + 1];
@@ -568,6 +570,251 @@ size_t orc_discrim_apply_ci16(orc_discrim *d, const int16_t *in, size_t n, float
     free(x);
     return k;
 }
+
+/* ------------------------------------------------------------------------ */
+/* IIR filter -- src/transform/falcon_dsp_iir_filter.cc                      */
+/* ------------------------------------------------------------------------ */
+
+/* iir_filter.cc:119-155: coefficients copied; when Re a[0] is not 1, b and
+ * a[1..] are divided by a[0] (complex division, textbook form) */
+int orc_iir_init(orc_iir *f, const float *b, size_t nb, const float *a, size_t na)
+{
+    memset(f, 0, sizeof(*f));
+    f->b = (float *)malloc(2 * (nb ? nb : 1) * sizeof(float));
+    f->a = (float *)malloc(2 * (na ? na : 1) * sizeof(float));
+    f->xs = (float *)malloc(2 * (nb ? nb : 1) * sizeof(float));
+    f->ys = (float *)malloc(2 * (na ? na : 1) * sizeof(float));
+    if (!f->b || !f->a || !f->xs || !f->ys)
+        return -1;
+    memcpy(f->b, b, 2 * nb * sizeof(float));
+    memcpy(f->a, a, 2 * na * sizeof(float));
+    f->nb = nb;
+    f->na = na;
+    if (na >= 1 && a[0] != 1.0f) {
+        const float dr = a[0], di = a[1], den = dr * dr + di * di;
+        for (size_t i = 0; i < nb; ++i) {
+            const float xr = f->b[2 * i], xi = f->b[2 * i + 1];
+            f->b[2 * i] = (xr * dr + xi * di) / den;
+            f->b[2 * i + 1] = (xi * dr - xr * di) / den;
+        }
+        for (size_t i = 1; i < na; ++i) {
+            const float xr = f->a[2 * i], xi = f->a[2 * i + 1];
+            f->a[2 * i] = (xr * dr + xi * di) / den;
+            f->a[2 * i + 1] = (xi * dr - xr * di) / den;
+        }
+    }
+    return 0;
+}
+
+void orc_iir_free(orc_iir *f)
+{
+    free(f->b);
+    free(f->a);
+    free(f->xs);
+    free(f->ys);
+    memset(f, 0, sizeof(*f));
+}
+
+/* iir_filter.cc:108-116 */
+void orc_iir_reset(orc_iir *f)
+{
+    f->xs_len = 0;
+    f->ys_len = 0;
+}
+
+/* iir_filter.cc:235-396: y = sum_k b[k] x[n-k] (the newest inputs from the
+ * chunk, older ones from the input state, newest first) minus
+ * sum_j a[j] y[n-j] over the kept outputs (newest first), each product
+ * formed, then added or subtracted, in that order. The state keeps the last
+ * nb inputs and the last na-1 outputs. */
+size_t orc_iir_apply(orc_iir *f, const float *in, size_t n, float *out)
+{
+    if (f->nb == 0 || f->na == 0 || n == 0)
+        return 0;
+    float pr, pi;
+    for (size_t i = 0; i < n; ++i) {
+        float ar = 0.0f, ai = 0.0f;
+        size_t k = 0;
+        if (i < f->nb) {
+            for (int64_t d = (int64_t)i; d >= 0; --d, ++k) {
+                cmul(in[2 * d], in[2 * d + 1], f->b[2 * k], f->b[2 * k + 1], &pr, &pi);
+                ar += pr;
+                ai += pi;
+            }
+            for (int64_t s = (int64_t)f->xs_len - 1; k < f->nb && s >= 0; --s, ++k) {
+                cmul(f->xs[2 * s], f->xs[2 * s + 1], f->b[2 * k], f->b[2 * k + 1], &pr, &pi);
+                ar += pr;
+                ai += pi;
+            }
+        } else {
+            for (; k < f->nb; ++k) {
+                cmul(in[2 * (i - k)], in[2 * (i - k) + 1], f->b[2 * k], f->b[2 * k + 1], &pr, &pi);
+                ar += pr;
+                ai += pi;
+            }
+        }
+        size_t j = 1;
+        for (int64_t s = (int64_t)f->ys_len - 1; j < f->na && s >= 0; --s, ++j) {
+            cmul(f->ys[2 * s], f->ys[2 * s + 1], f->a[2 * j], f->a[2 * j + 1], &pr, &pi);
+            ar -= pr;
+            ai -= pi;
+        }
+        out[2 * i] = ar;
+        out[2 * i + 1] = ai;
+        if (f->na > 1) {
+            if (f->ys_len == f->na - 1) {
+                memmove(f->ys, f->ys + 2, 2 * (f->ys_len - 1) * sizeof(float));
+                f->ys_len--;
+            }
+            f->ys[2 * f->ys_len] = ar;
+            f->ys[2 * f->ys_len + 1] = ai;
+            f->ys_len++;
+        }
+    }
+    if (n >= f->nb) {
+        memcpy(f->xs, in + 2 * (n - f->nb), 2 * f->nb * sizeof(float));
+        f->xs_len = f->nb;
+    } else {
+        /* append, keep the newest nb */
+        for (size_t i = 0; i < n; ++i) {
+            if (f->xs_len == f->nb) {
+                memmove(f->xs, f->xs + 2, 2 * (f->xs_len - 1) * sizeof(float));
+                f->xs_len--;
+            }
+            f->xs[2 * f->xs_len] = in[2 * i];
+            f->xs[2 * f->xs_len + 1] = in[2 * i + 1];
+            f->xs_len++;
+        }
+    }
+    return n;
+}
+
+/* iir_filter.cc:183-203: real input as (x, 0), real parts out */
+size_t orc_iir_apply_f32(orc_iir *f, const float *in, size_t n, float *out)
+{
+    float *x = (float *)calloc(2 * (n ? n : 1), sizeof(float));
+    float *y = (float *)malloc(2 * (n ? n : 1) * sizeof(float));
+    size_t k = 0;
+    if (x && y) {
+        for (size_t i = 0; i < n; ++i)
+            x[2 * i] = in[i];
+        k = orc_iir_apply(f, x, n, y);
+        for (size_t i = 0; i < k; ++i)
+            out[i] = y[2 * i];
+    }
+    free(x);
+    free(y);
+    return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FM mono demodulator -- src/signals/falcon_dsp_fm_demod.cc                 */
+/* ------------------------------------------------------------------------ */
+
+/* fm_demod.cc:146-165: single-pole de-emphasis at the 200 kHz design rate,
+ * x = exp(-1 / (200000 tau)), b = [1 - x], a = [1, -x] (the reference keeps
+ * the two values for 75 and 50 us as float literals of these) */
+void orc_fm_deemphasis(double tau, float *b0, float *a1)
+{
+    const double x = exp(-1.0 / (200000.0 * tau));
+    *b0 = (float)(1.0 - x);
+    *a1 = (float)(-x);
+}
+
+/* fm_demod.cc:98-198, with the two resamplers' taps passed in (the reference
+ * reads them from its predefined table: REMEZ, 64 taps) */
+int orc_fm_init(orc_fm *f, uint32_t fs, int32_t offset_hz, double tau, uint32_t iso_up, uint32_t iso_down,
+                const float *iso_taps, size_t n_iso, uint32_t mono_up, uint32_t mono_down, const float *mono_taps,
+                size_t n_mono)
+{
+    memset(f, 0, sizeof(*f));
+    if (orc_freq_shift_init(&f->shift, fs, offset_hz) != 0)
+        return -1;
+    if (orc_resampler_init(&f->iso, iso_up, iso_down, iso_taps, n_iso) != 0)
+        return -1;
+    if (orc_resampler_init(&f->mono, mono_up, mono_down, mono_taps, n_mono) != 0)
+        return -1;
+    orc_discrim_reset(&f->disc);
+    float b0, a1;
+    orc_fm_deemphasis(tau, &b0, &a1);
+    const float b[2] = {b0, 0.0f}, a[4] = {1.0f, 0.0f, a1, 0.0f};
+    return orc_iir_init(&f->deemph, b, 1, a, 2);
+}
+
+void orc_fm_free(orc_fm *f)
+{
+    orc_resampler_free(&f->iso);
+    orc_resampler_free(&f->mono);
+    orc_iir_free(&f->deemph);
+}
+
+/* fm_demod.cc:198-208 */
+void orc_fm_reset(orc_fm *f)
+{
+    orc_freq_shift_reset(&f->shift);
+    orc_resampler_reset(&f->iso);
+    orc_discrim_reset(&f->disc);
+    orc_iir_reset(&f->deemph);
+    orc_resampler_reset(&f->mono);
+}
+
+/* at most this many outputs for an n-sample chunk */
+size_t orc_fm_out_cap(const orc_fm *f, size_t n)
+{
+    const size_t a = n * f->iso.up / f->iso.down + 2;
+    return a * f->mono.up / f->mono.down + 2;
+}
+
+/* fm_demod.cc:210-271: shift, isolate (resample), discriminate, de-emphasise,
+ * decimate to audio (real samples through the complex resampler), then scale
+ * the chunk so its largest |sample| is 10000: s = (float)(10000.0 / max), each
+ * output the float product converted to int16 (truncation; saturated outside
+ * int16, and all zero when the chunk is all zero, where the reference's
+ * conversion is undefined). audio, when given, receives the samples before
+ * scaling. */
+size_t orc_fm_demod(orc_fm *f, const float *in, size_t n, int16_t *out, float *audio)
+{
+    if (n == 0)
+        return 0;
+    float *shifted = (float *)malloc(2 * n * sizeof(float));
+    const size_t n1cap = orc_resampler_needed_out_count(&f->iso, (uint32_t)n) + 1;
+    float *iso = (float *)malloc(2 * n1cap * sizeof(float));
+    float *disc = (float *)malloc(n1cap * sizeof(float));
+    float *emph = (float *)malloc(2 * n1cap * sizeof(float));
+    float *emph_c = (float *)calloc(2 * n1cap, sizeof(float));
+    const size_t n3cap = orc_resampler_needed_out_count(&f->mono, (uint32_t)n1cap) + 1;
+    float *mono = (float *)malloc(2 * n3cap * sizeof(float));
+    size_t n_out = 0;
+    if (shifted && iso && disc && emph && emph_c && mono) {
+        orc_freq_shift_apply(&f->shift, in, n, shifted);
+        const size_t n1 = orc_resampler_apply(&f->iso, shifted, n, iso);
+        const size_t n2 = n1 ? orc_discrim_apply(&f->disc, iso, n1, disc) : 0;
+        const size_t n3 = n2 ? orc_iir_apply_f32(&f->deemph, disc, n2, emph) : 0;
+        for (size_t i = 0; i < n3; ++i)
+            emph_c[2 * i] = emph[i];
+        const size_t n4 = n3 ? orc_resampler_apply(&f->mono, emph_c, n3, mono) : 0;
+        float amax = 0.0f;
+        for (size_t i = 0; i < n4; ++i)
+            amax = fabsf(mono[2 * i]) > amax ? fabsf(mono[2 * i]) : amax;
+        const float scale = amax > 0.0f ? (float)(10000.0 / (double)amax) : 0.0f;
+        for (size_t i = 0; i < n4; ++i) {
+            out[i] = orc_i16(mono[2 * i] * scale);
+            if (audio)
+                audio[i] = mono[2 * i];
+        }
+        n_out = n4;
+    }
+    free(shifted);
+    free(iso);
+    free(disc);
+    free(emph);
+    free(emph_c);
+    free(mono);
+    return n_out;
+}
+
+size_t orc_sizeof_iir(void) { return sizeof(orc_iir); }
+size_t orc_sizeof_fm(void) { return sizeof(orc_fm); }
 
 size_t orc_sizeof_freq_shift(void) { return sizeof(orc_freq_shift); }
 size_t orc_sizeof_resampler(void) { return sizeof(orc_resampler); }

@@ -94,6 +94,40 @@ void orc_discrim_reset(orc_discrim *d);
 /* out holds n floats; returns n-1 on the first call after reset, n otherwise */
 size_t orc_discrim_apply(orc_discrim *d, const float *in, size_t n, float *out);
 
+/* IIR filter (src/transform/falcon_dsp_iir_filter.cc), complex coefficients */
+typedef struct orc_iir {
+    float *b, *a;   /* nb and na complex coefficients, normalised by a[0] */
+    size_t nb, na;
+    float *xs;      /* input state, oldest first, up to nb samples */
+    size_t xs_len;
+    float *ys;      /* previous outputs, oldest first, up to na-1 */
+    size_t ys_len;
+} orc_iir;
+int orc_iir_init(orc_iir *f, const float *b, size_t nb, const float *a, size_t na);
+void orc_iir_free(orc_iir *f);
+void orc_iir_reset(orc_iir *f);
+size_t orc_iir_apply(orc_iir *f, const float *in, size_t n, float *out);
+size_t orc_iir_apply_f32(orc_iir *f, const float *in, size_t n, float *out);
+
+/* FM mono demodulator (src/signals/falcon_dsp_fm_demod.cc), resampler taps
+ * given by the caller */
+typedef struct orc_fm {
+    orc_freq_shift shift;
+    orc_resampler iso, mono;
+    orc_discrim disc;
+    orc_iir deemph;
+} orc_fm;
+void orc_fm_deemphasis(double tau, float *b0, float *a1);
+int orc_fm_init(orc_fm *f, uint32_t fs, int32_t offset_hz, double tau, uint32_t iso_up, uint32_t iso_down,
+                const float *iso_taps, size_t n_iso, uint32_t mono_up, uint32_t mono_down, const float *mono_taps,
+                size_t n_mono);
+void orc_fm_free(orc_fm *f);
+void orc_fm_reset(orc_fm *f);
+size_t orc_fm_out_cap(const orc_fm *f, size_t n);
+size_t orc_fm_demod(orc_fm *f, const float *in, size_t n, int16_t *out, float *audio);
+size_t orc_sizeof_iir(void);
+size_t orc_sizeof_fm(void);
+
 /* complex int16 overloads (interleaved re, im) */
 int16_t orc_i16(float v);
 size_t orc_freq_shift_apply_ci16(orc_freq_shift *s, const int16_t *in, size_t n, int16_t *out);

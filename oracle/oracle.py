@@ -67,7 +67,7 @@ def lib():
         L.orc_rat_approx.argtypes = [C.c_double, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.orc_freq_shift_params.restype = C.c_int
         L.orc_freq_shift_params.argtypes = [C.c_uint32, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
-        for name in ("freq_shift", "resampler", "fir", "discrim", "channelizer"):
+        for name in ("freq_shift", "resampler", "fir", "discrim", "channelizer", "iir", "fm"):
             getattr(L, f"orc_sizeof_{name}").restype = C.c_size_t
         L.orc_freq_shift_init.argtypes = [_vp, C.c_uint32, C.c_int32]
         L.orc_freq_shift_reset.argtypes = [_vp]
@@ -88,6 +88,21 @@ def lib():
         L.orc_discrim_reset.argtypes = [_vp]
         L.orc_discrim_apply.restype = C.c_size_t
         L.orc_discrim_apply.argtypes = [_vp, _f32p, C.c_size_t, _f32p]
+        L.orc_iir_init.argtypes = [_vp, _f32p, C.c_size_t, _f32p, C.c_size_t]
+        L.orc_iir_free.argtypes = [_vp]
+        L.orc_iir_reset.argtypes = [_vp]
+        for name in ("orc_iir_apply", "orc_iir_apply_f32"):
+            getattr(L, name).restype = C.c_size_t
+            getattr(L, name).argtypes = [_vp, _f32p, C.c_size_t, _f32p]
+        L.orc_fm_deemphasis.argtypes = [C.c_double, _f32p, _f32p]
+        L.orc_fm_init.argtypes = [_vp, C.c_uint32, C.c_int32, C.c_double, C.c_uint32, C.c_uint32, _f32p, C.c_size_t,
+                                  C.c_uint32, C.c_uint32, _f32p, C.c_size_t]
+        L.orc_fm_free.argtypes = [_vp]
+        L.orc_fm_reset.argtypes = [_vp]
+        L.orc_fm_out_cap.restype = C.c_size_t
+        L.orc_fm_out_cap.argtypes = [_vp, C.c_size_t]
+        L.orc_fm_demod.restype = C.c_size_t
+        L.orc_fm_demod.argtypes = [_vp, _f32p, C.c_size_t, _i16p, _f32p]
         L.orc_i16.restype = C.c_int16
         L.orc_i16.argtypes = [C.c_float]
         for name, outp in (("freq_shift", _i16p), ("fir", _i16p), ("discrim", _f32p)):
@@ -265,6 +280,78 @@ class Discriminator(_Obj):
         return out[:n]
 
 
+class Iir(_Obj):
+    """falcon_dsp_iir_filter (src/transform/falcon_dsp_iir_filter.cc:104-396)."""
+    _kind = "iir"
+
+    def __init__(self, b, a):
+        super().__init__()
+        self._b, self._a = as_cf32(b), as_cf32(a)
+        if lib().orc_iir_init(self.p, _fptr(self._b.view(np.float32)), self._b.size,
+                              _fptr(self._a.view(np.float32)), self._a.size) != 0:
+            raise ValueError("init failed")
+
+    def __del__(self):
+        try:
+            lib().orc_iir_free(self.p)
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().orc_iir_reset(self.p)
+
+    def apply(self, x):
+        x = as_cf32(x)
+        out = np.empty_like(x)
+        n = lib().orc_iir_apply(self.p, _fptr(x.view(np.float32)), x.size, _fptr(out.view(np.float32)))
+        return out[:n]
+
+    def apply_f32(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty_like(x)
+        n = lib().orc_iir_apply_f32(self.p, _fptr(x), x.size, _fptr(out))
+        return out[:n]
+
+
+def fm_deemphasis(tau):
+    """(b0, a1) of the single-pole de-emphasis (src/signals/falcon_dsp_fm_demod.cc:146-165)."""
+    b0, a1 = C.c_float(), C.c_float()
+    lib().orc_fm_deemphasis(float(tau), C.byref(b0), C.byref(a1))
+    return b0.value, a1.value
+
+
+class Fm(_Obj):
+    """falcon_dsp_fm_demodulator::demod_mono (src/signals/falcon_dsp_fm_demod.cc:98-271)
+    with the two resamplers' (up, down, taps) given."""
+    _kind = "fm"
+
+    def __init__(self, fs, offset_hz, tau, iso, mono):
+        super().__init__()
+        self._iso_t, self._mono_t = as_cf32(iso[2]), as_cf32(mono[2])
+        if lib().orc_fm_init(self.p, fs, offset_hz, float(tau), iso[0], iso[1], _fptr(self._iso_t.view(np.float32)),
+                             self._iso_t.size, mono[0], mono[1], _fptr(self._mono_t.view(np.float32)),
+                             self._mono_t.size) != 0:
+            raise ValueError("init failed")
+
+    def __del__(self):
+        try:
+            lib().orc_fm_free(self.p)
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().orc_fm_reset(self.p)
+
+    def demod(self, x):
+        """-> (int16 audio, float audio before scaling)."""
+        x = as_cf32(x)
+        cap = int(lib().orc_fm_out_cap(self.p, x.size))
+        out = np.empty(cap, dtype=np.int16)
+        audio = np.empty(cap, dtype=np.float32)
+        n = lib().orc_fm_demod(self.p, _fptr(x.view(np.float32)), x.size, _iptr(out), _fptr(audio))
+        return out[:n], audio[:n]
+
+
 class Channelizer(_Obj):
     """falcon_dsp_multi_rate_channelizer (src/transform/falcon_dsp_multi_rate_channelizer.cc:81-173).
 
@@ -367,6 +454,22 @@ def ref():
         L.ref_write_file.argtypes = [C.c_int, C.c_int, C.c_char_p, _vp, C.c_size_t]
         L.ref_read_file.restype = C.c_int64
         L.ref_read_file.argtypes = [C.c_int, C.c_int, C.c_char_p, _vp, C.c_size_t]
+        L.ref_iir_create.restype = _vp
+        L.ref_iir_create.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t]
+        for name in ("ref_iir_apply", "ref_iir_apply_f32"):
+            getattr(L, name).restype = C.c_size_t
+            getattr(L, name).argtypes = [_vp, _f32p, C.c_size_t, _f32p]
+        L.ref_iir_reset.argtypes = [_vp]
+        L.ref_iir_destroy.argtypes = [_vp]
+        L.ref_resample_fir_params.restype = C.c_int64
+        L.ref_resample_fir_params.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                              _f32p, C.c_size_t]
+        L.ref_fm_create.restype = _vp
+        L.ref_fm_create.argtypes = [C.c_uint32, C.c_int32, C.c_int]
+        L.ref_fm_demod.restype = C.c_int64
+        L.ref_fm_demod.argtypes = [_vp, _f32p, C.c_size_t, _i16p, C.c_size_t]
+        L.ref_fm_reset.argtypes = [_vp]
+        L.ref_fm_destroy.argtypes = [_vp]
         L.ref_chan_create.restype = _vp
         L.ref_chan_create.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(_f32p),
@@ -477,6 +580,64 @@ class RefDiscriminator:
         out = np.empty(x.shape[0], dtype=np.float32)
         n = ref().ref_disc_apply_ci16(self.h, _iptr(x), x.shape[0], _fptr(out))
         return out[:n]
+
+
+class RefIir:
+    def __init__(self, b, a):
+        self._b, self._a = as_cf32(b), as_cf32(a)
+        self.h = ref().ref_iir_create(_fptr(self._b.view(np.float32)), self._b.size,
+                                      _fptr(self._a.view(np.float32)), self._a.size)
+        assert self.h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().ref_iir_destroy(self.h)
+
+    def reset(self):
+        ref().ref_iir_reset(self.h)
+
+    def apply(self, x):
+        x = as_cf32(x)
+        out = np.empty_like(x)
+        n = ref().ref_iir_apply(self.h, _fptr(x.view(np.float32)), x.size, _fptr(out.view(np.float32)))
+        return out[:n]
+
+    def apply_f32(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty_like(x)
+        n = ref().ref_iir_apply_f32(self.h, _fptr(x), x.size, _fptr(out))
+        return out[:n]
+
+
+def ref_resample_fir_params(in_rate, out_rate):
+    """The reference's predefined 64-tap REMEZ resampling filter -> (up, down, taps)."""
+    up, down = C.c_uint32(), C.c_uint32()
+    taps = np.empty(4096, dtype=np.complex64)
+    n = ref().ref_resample_fir_params(in_rate, out_rate, C.byref(up), C.byref(down),
+                                      _fptr(taps.view(np.float32)), taps.size)
+    if n < 0:
+        raise KeyError((in_rate, out_rate))
+    return up.value, down.value, taps[:n].copy()
+
+
+class RefFm:
+    def __init__(self, fs, offset_hz, europe=False):
+        self.h = ref().ref_fm_create(fs, offset_hz, 1 if europe else 0)
+        assert self.h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().ref_fm_destroy(self.h)
+
+    def reset(self):
+        ref().ref_fm_reset(self.h)
+
+    def demod(self, x):
+        x = as_cf32(x)
+        cap = x.size
+        out = np.empty(max(cap, 1), dtype=np.int16)
+        n = ref().ref_fm_demod(self.h, _fptr(x.view(np.float32)), x.size, _iptr(out), cap)
+        return None if n < 0 else out[:n]
 
 
 class RefChannelizer:

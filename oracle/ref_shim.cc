@@ -19,7 +19,10 @@
 #include <cstring>
 #include <vector>
 
+#include "signals/falcon_dsp_fm_demod.h"
 #include "transform/falcon_dsp_fir_filter.h"
+#include "transform/falcon_dsp_iir_filter.h"
+#include "transform/falcon_dsp_predefined_fir_filter.h"
 #include "transform/falcon_dsp_freq_shift.h"
 #include "transform/falcon_dsp_multi_rate_channelizer.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
@@ -247,6 +250,74 @@ int64_t ref_read_file(int kind, int ascii, const char *path, void *out, size_t c
     }
     }
 }
+
+/* ---- IIR filter (complex coefficients), float overload ---- */
+void *ref_iir_create(const float *b, size_t nb, const float *a, size_t na)
+{
+    auto bv = to_vec(b, nb), av = to_vec(a, na);
+    auto *o = new falcon_dsp::falcon_dsp_iir_filter();
+    if (!o->initialize(bv, av)) {
+        delete o;
+        return nullptr;
+    }
+    return o;
+}
+size_t ref_iir_apply_f32(void *h, const float *in, size_t n, float *out)
+{
+    std::vector<float> v(in, in + n), o;
+    static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h)->apply(v, o);
+    if (!o.empty())
+        std::memcpy(out, o.data(), o.size() * sizeof(float));
+    return o.size();
+}
+size_t ref_iir_apply(void *h, const float *in, size_t n, float *out)
+{
+    auto v = to_vec(in, n);
+    std::vector<cf32> o;
+    static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h)->apply(v, o);
+    if (!o.empty())
+        std::memcpy(out, o.data(), o.size() * sizeof(cf32));
+    return o.size();
+}
+void ref_iir_reset(void *h) { static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h)->reset_state(); }
+void ref_iir_destroy(void *h) { delete static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h); }
+
+/* ---- predefined resample filters (64-tap REMEZ) and the FM demodulator ---- */
+/* taps copied out when cap allows; returns the tap count, or -1 */
+int64_t ref_resample_fir_params(uint32_t in_rate, uint32_t out_rate, uint32_t *up, uint32_t *down, float *taps,
+                                size_t cap)
+{
+    std::vector<cf32> c;
+    if (!falcon_dsp::get_resample_fir_params(in_rate, out_rate, filter_taps_e::FILTER_TAPS_64,
+                                             filter_source_type_e::REMEZ, *up, *down, c))
+        return -1;
+    if (c.size() <= cap)
+        std::memcpy(taps, c.data(), c.size() * sizeof(cf32));
+    return (int64_t)c.size();
+}
+void *ref_fm_create(uint32_t fs, int32_t offset_hz, int europe)
+{
+    auto *o = new falcon_dsp::falcon_dsp_fm_demodulator();
+    const float tc = europe ? falcon_dsp::falcon_dsp_fm_demodulator::EUROPE_ASIA_FM_DEEMPHASIS_TIME_CONSTANT
+                            : falcon_dsp::falcon_dsp_fm_demodulator::AMERICAS_FM_DEEMPHASIS_TIME_CONSTANT;
+    if (!o->initialize(fs, offset_hz, tc)) {
+        delete o;
+        return nullptr;
+    }
+    return o;
+}
+/* outputs (at most cap copied), or -1 when demod_mono returns false */
+int64_t ref_fm_demod(void *h, const float *in, size_t n, int16_t *out, size_t cap)
+{
+    auto v = to_vec(in, n);
+    std::vector<int16_t> o;
+    if (!static_cast<falcon_dsp::falcon_dsp_fm_demodulator *>(h)->demod_mono(v, o))
+        return -1;
+    std::memcpy(out, o.data(), std::min(o.size(), cap) * sizeof(int16_t));
+    return (int64_t)o.size();
+}
+void ref_fm_reset(void *h) { static_cast<falcon_dsp::falcon_dsp_fm_demodulator *>(h)->reset_state(); }
+void ref_fm_destroy(void *h) { delete static_cast<falcon_dsp::falcon_dsp_fm_demodulator *>(h); }
 
 /* ---- multi-rate channelizer ---- */
 void *ref_chan_create(uint32_t fs, uint32_t n_chan, const int64_t *shift_hz, const uint32_t *out_rate,
