@@ -131,6 +131,9 @@ def timed_launches(fd, stream, launch, reps):
 def bench_other_configs(fd, stream, steps):
     """The other workloads, each with data resident in HBM: throughput and
     roofline fraction of one pass."""
+    import ctypes as C
+
+    from libfalcon_dsp_amd._native import check
     from libfalcon_dsp_amd.workloads import FirBank256, Nco16, Resampler1ch
     out = {}
 
@@ -179,6 +182,35 @@ def bench_other_configs(fd, stream, steps):
         "note": "2^24 inputs in 64 chunks of 2^18, 16 shifts, 136 B per input sample"}
     log(f"nco_16shift: {ms2:.3f} ms per pass, {out['nco_16shift']['roofline']['achieved']} GB/s")
     del d2, o2, x2
+
+    # FM mono demodulation, 2^24 samples at 1.14 Msps in 2^18-sample chunks
+    from libfalcon_dsp_amd.workloads import FmDemod
+    w5 = FmDemod()
+    iso, mono = w5.taps()
+    fm = fd.FmDemodulator(w5.fs, w5.offset, iso, mono)
+    x5 = w5.input()
+    d5 = fd.DeviceBuffer.from_array(x5)
+    cap5 = fm.out_count(w5.chunk) + 8
+    o5 = fd.DeviceBuffer(cap5, np.int16)
+    got5 = C.c_size_t()
+
+    def pass5():
+        for k in range(w5.n // w5.chunk):
+            check(fd.lib().fdsp_fm_demod_apply_device(fm.h, d5.at(k * w5.chunk), w5.chunk, o5.ptr, None, cap5,
+                                                      C.byref(got5), stream.h))
+
+    ms5 = timed_launches(fd, stream, pass5, max(2, steps // 4))
+    b5 = 8 * w5.n + 2 * (w5.n // 25)
+    out["fm_demod"] = {
+        "value": round(w5.n / (ms5 / 1e3) / 1e6, 1), "unit": "input Msps", "ms_per_pass": round(ms5, 4),
+        "realtime_factor": round((w5.n / w5.fs) / (ms5 / 1e3), 1),
+        "roofline": {"bound": "hbm", "achieved": round(b5 / (ms5 / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(b5 / (ms5 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "note": "2^24 cf32 samples at 1.14 Msps in 64 chunks of 2^18; channelizer pass with a discriminator "
+                "tail, de-emphasis, audio decimator, rescale; 8 B in per input sample, int16 audio out"}
+    log(f"fm_demod: {ms5:.3f} ms per pass, {out['fm_demod']['value']} Msps in, "
+        f"{out['fm_demod']['realtime_factor']}x real time")
+    del d5, o5, fm, x5
 
     # the same shifts on complex int16 samples in and out (4 B each way)
     x3 = w2.input_ci16()

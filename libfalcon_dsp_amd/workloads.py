@@ -87,6 +87,26 @@ class Nco16:
 
 
 @dataclass
+class FmDemod:
+    """FM mono demodulation of an RTL-SDR style stream: 1.14 Msps, the station
+    200 kHz off DC, 1,140,000 -> 228,000 -> 45,600 sps through 64-tap
+    filters (Kaiser-windowed here; the reference uses its REMEZ table)."""
+    n: int = 1 << 24
+    chunk: int = 1 << 18
+    fs: int = 1_140_000
+    offset: int = -200_000
+
+    def taps(self):
+        t = kaiser_sinc(64, 0.5 / 5, 8.0).astype(np.complex64)
+        return (1, 5, t), (1, 5, t)
+
+    def input(self) -> np.ndarray:
+        from .signals import fm_signal
+        sig, _ = fm_signal(self.n, self.fs, [400.0, 1300.0, 2700.0], 75e3, carrier_hz=-self.offset)
+        return (sig + 0.02 * cwgn(7, self.n)).astype(np.complex64)
+
+
+@dataclass
 class Chan32:
     n: int = 1 << 24
     fs: int = 20_000_000
