@@ -187,13 +187,43 @@ int fdsp_discriminator_apply_ci16(fdsp_discriminator_t h, const fdsp_ci16 *in, s
 int fdsp_discriminator_apply_ci16_device(fdsp_discriminator_t h, const fdsp_ci16 *d_in, size_t n, float *d_out,
                                          size_t *n_out, fdsp_stream_t stream);
 
+/* ---- IIR filter ----
+ * Replaces falcon_dsp_iir_filter::{initialize, apply (complex<float>, float,
+ * complex<int16_t>, int16_t), reset_state} (reference
+ * hdr/transform/falcon_dsp_iir_filter.h:97-130, src/transform/
+ * falcon_dsp_iir_filter.cc:104-396). y[n] = sum_k b[k] x[n-k] -
+ * sum_{j>=1} a[j] y[n-j]; when a[0]'s real part is not 1 every b and a[1:] is
+ * first divided by a[0], as the reference does (its complex<float> division).
+ * Inputs and outputs are carried between calls; one handle's calls may mix
+ * the four sample types. Outputs are the reference's bit for bit (up to the
+ * sign of zero) for any coefficients: the GPU recurrence runs in segments and
+ * verifies that each segment's start state equals its predecessor's exact end
+ * state, recomputing any that does not. Filters that decay slowly (or not at
+ * all) stay exact but run slower. int16 outputs are truncated toward zero and
+ * saturated (out of range is undefined in the reference). Limits: 1 <= nb <=
+ * 256, 1 <= na <= 17, a[0] != 0 (the reference would divide by zero). */
+typedef struct fdsp_iir_s *fdsp_iir_t;
+int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 *a, uint32_t na, fdsp_iir_t *out);
+int fdsp_iir_destroy(fdsp_iir_t h);
+int fdsp_iir_reset(fdsp_iir_t h);
+/* warm: samples each segment's recurrence runs before its outputs; order: na - 1 */
+int fdsp_iir_info(fdsp_iir_t h, uint32_t *warm, uint32_t *order);
+int fdsp_iir_apply(fdsp_iir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out);
+int fdsp_iir_apply_f32(fdsp_iir_t h, const float *in, size_t n, float *out);
+int fdsp_iir_apply_ci16(fdsp_iir_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out);
+int fdsp_iir_apply_i16(fdsp_iir_t h, const int16_t *in, size_t n, int16_t *out);
+int fdsp_iir_apply_device(fdsp_iir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out, fdsp_stream_t stream);
+int fdsp_iir_apply_f32_device(fdsp_iir_t h, const float *d_in, size_t n, float *d_out, fdsp_stream_t stream);
+int fdsp_iir_apply_ci16_device(fdsp_iir_t h, const fdsp_ci16 *d_in, size_t n, fdsp_ci16 *d_out,
+                               fdsp_stream_t stream);
+int fdsp_iir_apply_i16_device(fdsp_iir_t h, const int16_t *d_in, size_t n, int16_t *d_out, fdsp_stream_t stream);
+
 /* ---- first-order IIR (real samples) ----
  * Replaces falcon_dsp_iir_filter::{initialize(b = [b0], a = [1, a1]),
  * apply(vector<float>&, vector<float>&), reset_state} (reference
  * src/transform/falcon_dsp_iir_filter.cc:104-203,235-396) for the first-order
- * filters the FM de-emphasis uses: y[n] = b0 x[n] - a1 y[n-1], the output
- * carried between calls, rounded as the reference's sequential loop rounds it
- * (bit-identical outputs). |a1| <= 0.94. */
+ * filters the FM de-emphasis uses: y[n] = b0 x[n] - a1 y[n-1]. A shorthand
+ * for fdsp_iir_create with b = [b0], a = [1, a1]; bit-identical outputs. */
 typedef struct fdsp_iir1_s *fdsp_iir1_t;
 int fdsp_iir1_create(float b0, float a1, fdsp_iir1_t *out);
 int fdsp_iir1_destroy(fdsp_iir1_t h);

@@ -162,6 +162,18 @@ _SIGS = [
     ("fdsp_discriminator_apply_device", _i, [_vp, _vp, _sz, _vp, _P(_sz), _vp]),
     ("fdsp_discriminator_apply_ci16", _i, [_vp, _vp, _sz, _vp, _P(_sz)]),
     ("fdsp_discriminator_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _P(_sz), _vp]),
+    ("fdsp_iir_create", _i, [_vp, _u32, _vp, _u32, _P(_vp)]),
+    ("fdsp_iir_destroy", _i, [_vp]),
+    ("fdsp_iir_reset", _i, [_vp]),
+    ("fdsp_iir_info", _i, [_vp, _P(_u32), _P(_u32)]),
+    ("fdsp_iir_apply", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir_apply_f32", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir_apply_ci16", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir_apply_i16", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir_apply_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_iir_apply_f32_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_iir_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_iir_apply_i16_device", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_iir1_create", _i, [C.c_float, C.c_float, _P(_vp)]),
     ("fdsp_iir1_destroy", _i, [_vp]),
     ("fdsp_iir1_reset", _i, [_vp]),

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = [os.path.join(CSRC, f)
-               for f in ("fdsp_kernels.hip", "fdsp_chan_groups.hip", "fdsp_capi.hip", "fdsp_serve.hip", "fdsp_fm.hip")]
+               for f in ("fdsp_kernels.hip", "fdsp_chan_groups.hip", "fdsp_capi.hip", "fdsp_serve.hip", "fdsp_fm.hip",
+                         "fdsp_iir.hip")]
 HIP_LIB = os.path.join(HERE, "libfdsp_hip.so")
 DROPIN_SOURCES_DIR = os.path.join(HERE, "src")
 DROPIN_LIB = os.path.join(HERE, "libfalcon_dsp.so")

@@ -1,0 +1,818 @@
+// General IIR filter y = (sum_k b[k] x[n-k]) - sum_j a[j] y[n-j], bit-identical
+// to the reference's sequential loop (src/transform/falcon_dsp_iir_filter.cc:
+// 119-155 for the coefficients, 235-396 for the recurrence).
+//
+// The reference forms each output as ((0 + b0 x[n]) + b1 x[n-1] + ...) and
+// then subtracts a1 y[n-1], a2 y[n-2], ... in that order, every product a
+// std::complex<float> product. The b-part v[n] depends on inputs only, so a
+// block computes it for all its positions in parallel; the feedback is a
+// recurrence on v. Each lane owns kSeg consecutive outputs and starts the same
+// recurrence `warm` samples earlier from a zero state (or at the chunk start
+// from the carried state). For a stable filter two float trajectories draw
+// together and, once one step's products round alike, are identical from then
+// on; `warm` is sized on the host from the filter's own decay (2^-44).
+//
+// Nothing rests on that argument: every lane records its state when its
+// segment starts and when it ends. A segment whose start state equals its
+// predecessor's end state bit for bit (up to the sign of zero) computed the
+// sequential outputs exactly, by induction from the chunk's first segment,
+// which starts from the carried state. Lanes compare within their block; the
+// last block to finish compares across blocks and, if any segment failed,
+// recomputes the failed segments in order from their predecessors' exact end
+// states (a slower path that keeps unstable or slowly decaying filters exact).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/fdsp.h"
+#include "fdsp_host.h"
+#include "fdsp_iir.h"
+#include "fdsp_internal.h"
+
+using namespace fdsp;
+
+namespace {
+
+constexpr uint32_t kLanes = 64;                  // one wave per block (lockstep in-place LDS use)
+constexpr uint32_t kSeg = 64;                    // outputs per lane
+constexpr uint32_t kBlockOut = kLanes * kSeg;    // outputs per block
+constexpr uint32_t kMinWarm = 32, kMaxWarm = 2048;
+constexpr uint32_t kMaxB = 256, kMaxFb = 16;
+
+template <bool C> using Val = typename std::conditional<C, float2, float>::type;
+
+template <bool C> __device__ __forceinline__ Val<C> vzero()
+{
+    if constexpr (C)
+        return make_float2(0.f, 0.f);
+    else
+        return 0.f;
+}
+template <bool C> __device__ __forceinline__ Val<C> from2(float2 v)
+{
+    if constexpr (C)
+        return v;
+    else
+        return v.x;
+}
+__device__ __forceinline__ float2 to2(float2 v) { return v; }
+__device__ __forceinline__ float2 to2(float v) { return make_float2(v, 0.f); }
+
+// LDS slot of span position q: 16 bytes of padding after every 64 values, so
+// lanes reading at a stride of kSeg values start on different banks
+template <bool C> __device__ __forceinline__ uint32_t slot(uint32_t q) { return q + (C ? 2u : 4u) * (q >> 6); }
+template <bool C> constexpr uint32_t slots(uint32_t span) { return span + (C ? 2u : 4u) * ((span + 63) >> 6); }
+
+// equal bit patterns, or equal values (+0 and -0 lead to the same outputs)
+__device__ __forceinline__ bool same(float a, float b) { return a == b || __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool same(float2 a, float2 b) { return same(a.x, b.x) && same(a.y, b.y); }
+
+struct IirLaunch {
+    const void *x;            // n inputs of type TIN
+    void *out;                // n outputs of type tout
+    const float2 *b;          // nb taps
+    float2 *xh;               // nb - 1 carried inputs, oldest first
+    float2 *yh;               // kMaxFb carried outputs, newest first
+    float2 *rs, *re;          // per segment: P start-state values, P end-state values
+    unsigned long long *bad;  // per block: bit t set when segment t's start differs from t-1's end
+    uint32_t *ctl;            // [0] blocks finished, [1] blocks with a failed segment
+    uint32_t n, nb, warm, tout;
+};
+template <int P> struct Fb {
+    float2 a[P];  // a[1..P] of the normalised denominator (zero past na - 1)
+};
+
+template <uint32_t TIN, bool C> __device__ __forceinline__ Val<C> load_x(const IirLaunch &g, int32_t i)
+{
+    float2 v;
+    if (i < 0) {
+        v = g.xh[(int32_t)g.nb - 1 + i];
+    } else {
+        if constexpr (TIN == kSampleCf32)
+            v = static_cast<const float2 *>(g.x)[i];
+        else if constexpr (TIN == kSampleF32)
+            v = make_float2(static_cast<const float *>(g.x)[i], 0.f);
+        else if constexpr (TIN == kSampleCi16) {
+            const short2 s = static_cast<const short2 *>(g.x)[i];
+            v = make_float2((float)s.x, (float)s.y);
+        } else
+            v = make_float2((float)static_cast<const short *>(g.x)[i], 0.f);
+    }
+    return from2<C>(v);
+}
+
+// acc + x b, the product as std::complex<float> rounds it (a real-coefficient
+// filter on real samples has zero imaginary parts throughout, so the real
+// form gives the same real parts)
+template <bool C> __device__ __forceinline__ Val<C> mac(Val<C> acc, Val<C> x, float2 b)
+{
+#pragma clang fp contract(off)
+    if constexpr (C) {
+        const float2 p = cmul_ref(x, b);
+        return make_float2(acc.x + p.x, acc.y + p.y);
+    } else {
+        return acc + x * b.x;
+    }
+}
+
+// one output: v - a1 y[n-1] - a2 y[n-2] - ..., subtracted in that order;
+// st[0] is the newest output
+template <int P, bool C> __device__ __forceinline__ Val<C> fb_step(Val<C> v, Val<C> (&st)[P], const Fb<P> &fb)
+{
+#pragma clang fp contract(off)
+    Val<C> y = v;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        if constexpr (C) {
+            const float2 p = cmul_ref(st[k], fb.a[k]);
+            y.x = y.x - p.x;
+            y.y = y.y - p.y;
+        } else {
+            y = y - st[k] * fb.a[k].x;
+        }
+    }
+#pragma unroll
+    for (int k = P - 1; k > 0; --k)
+        st[k] = st[k - 1];
+    st[0] = y;
+    return y;
+}
+
+template <bool C> __device__ __forceinline__ void put_out(const IirLaunch &g, uint32_t i, Val<C> y)
+{
+    const float2 v = to2(y);
+    switch (g.tout) {
+    case kSampleCf32:
+        static_cast<float2 *>(g.out)[i] = v;
+        break;
+    case kSampleF32:
+        static_cast<float *>(g.out)[i] = v.x;
+        break;
+    case kSampleCi16:
+        static_cast<short2 *>(g.out)[i] = make_short2(i16_trunc(v.x), i16_trunc(v.y));
+        break;
+    default:
+        static_cast<short *>(g.out)[i] = i16_trunc(v.x);
+    }
+}
+
+// the b-part of position i >= 0
+template <uint32_t TIN, bool C> __device__ __forceinline__ Val<C> bpart(const IirLaunch &g, int32_t i)
+{
+    Val<C> acc = vzero<C>();
+    for (uint32_t k = 0; k < g.nb; ++k)
+        acc = mac<C>(acc, load_x<TIN, C>(g, i - (int32_t)k), g.b[k]);
+    return acc;
+}
+
+// 32 real or 16 complex values: eight 16-byte LDS accesses
+template <bool C> constexpr int kBatch = C ? 16 : 32;
+
+template <bool C> __device__ __forceinline__ void lds_read(const Val<C> *p, Val<C> (&v)[kBatch<C>])
+{
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float4 w = q[i];
+        if constexpr (C) {
+            v[2 * i] = make_float2(w.x, w.y);
+            v[2 * i + 1] = make_float2(w.z, w.w);
+        } else {
+            v[4 * i] = w.x;
+            v[4 * i + 1] = w.y;
+            v[4 * i + 2] = w.z;
+            v[4 * i + 3] = w.w;
+        }
+    }
+}
+template <bool C> __device__ __forceinline__ void lds_write(Val<C> *p, const Val<C> (&v)[kBatch<C>])
+{
+    float4 *q = reinterpret_cast<float4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if constexpr (C)
+            q[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+        else
+            q[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
+}
+
+__device__ __forceinline__ unsigned long long shfl64(unsigned long long v, int lane)
+{
+    const uint32_t lo = __shfl((int)(uint32_t)v, lane), hi = __shfl((int)(uint32_t)(v >> 32), lane);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Run by the last block to finish, after every other block's outputs,
+// records and failure bits are visible: check the block boundaries, repair
+// failed segments in order, then carry the stream state to the next call.
+template <int P, bool C, uint32_t TIN> __device__ void iir_finish(const IirLaunch &g, const Fb<P> &fb)
+{
+    using V = Val<C>;
+    __shared__ V fixed_end[P];  // end state of the most recently repaired segment
+    __shared__ V stash[kSeg];
+    __shared__ int32_t fixed_seg;
+    const uint32_t lane = threadIdx.x, nblk = gridDim.x;
+    const uint32_t nseg = (g.n + kSeg - 1) / kSeg;
+    if (lane == 0)
+        fixed_seg = -1;
+    auto boundary_bad = [&](uint32_t b) {  // block b's first segment against its predecessor
+        const uint32_t s = b * kLanes;
+        bool d = false;
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            d |= !same(from2<C>(g.rs[(size_t)s * P + k]), from2<C>(g.re[(size_t)(s - 1) * P + k]));
+        return d;
+    };
+    constexpr uint32_t kU = P >= 8 ? 1u : 8u / P;  // boundary checks in flight per lane
+    bool any = false;
+    for (uint32_t b0 = 1; b0 < nblk; b0 += kLanes * kU) {
+        bool d[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t b = b0 + u * kLanes + lane;
+            d[u] = b < nblk && boundary_bad(b);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u)
+            any |= d[u];
+    }
+    any = __ballot(any) != 0ull || atomicAdd(&g.ctl[1], 0u) != 0u;
+    __syncthreads();
+
+    if (any) {
+        // Repair, in segment order. Segments below `next` are final. A set bit
+        // whose predecessor was never repaired is a real failure; after a
+        // repair the successor is checked against the new end state directly.
+        auto repair = [&](uint32_t f) {
+            const uint32_t p0 = f * kSeg, cnt = min(kSeg, g.n - p0);
+            stash[lane] = lane < cnt ? bpart<TIN, C>(g, (int32_t)(p0 + lane)) : vzero<C>();
+            __syncthreads();
+            if (lane == 0) {
+                V st[P];
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                    st[k] = fixed_seg == (int32_t)f - 1 ? fixed_end[k] : from2<C>(g.re[(size_t)(f - 1) * P + k]);
+                for (uint32_t i = 0; i < cnt; ++i)
+                    stash[i] = fb_step<P, C>(stash[i], st, fb);
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    fixed_end[k] = st[k];
+                    g.re[(size_t)f * P + k] = to2(st[k]);
+                }
+                fixed_seg = (int32_t)f;
+            }
+            __syncthreads();
+            if (lane < cnt)
+                put_out<C>(g, p0 + lane, stash[lane]);
+            __syncthreads();
+        };
+        uint32_t next = 1;
+        for (uint32_t w0 = 0; w0 < nblk; w0 += kLanes) {
+            const uint32_t b = w0 + lane;
+            unsigned long long w = 0ull;
+            if (b < nblk) {
+                w = g.bad[b];
+                if (b > 0 && boundary_bad(b))
+                    w |= 1ull;
+            }
+            unsigned long long nz = __ballot(w != 0ull);
+            while (nz) {
+                const int l = __ffsll((long long)nz) - 1;
+                nz &= nz - 1;
+                unsigned long long bits = shfl64(w, l);
+                while (bits) {
+                    const uint32_t f0 = (w0 + (uint32_t)l) * kLanes + (uint32_t)(__ffsll((long long)bits) - 1);
+                    bits &= bits - 1;
+                    if (f0 < next)
+                        continue;
+                    uint32_t f = f0;
+                    for (;;) {
+                        repair(f);
+                        if (++f >= nseg)
+                            break;
+                        bool ok = true;
+#pragma unroll
+                        for (int k = 0; k < P; ++k)
+                            ok &= same(from2<C>(g.rs[(size_t)f * P + k]), fixed_end[k]);
+                        if (ok)
+                            break;
+                    }
+                    next = f + 1;
+                }
+            }
+        }
+    }
+
+    // carried outputs: the last segment's end state
+    const uint32_t last = nseg - 1;
+    if (lane < (uint32_t)P)
+        g.yh[lane] = to2(fixed_seg == (int32_t)last ? fixed_end[lane] : from2<C>(g.re[(size_t)last * P + lane]));
+    // carried inputs: the last nb - 1 of (old history, this chunk)
+    const uint32_t h = g.nb - 1;
+    float2 keep[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = lane + r * kLanes;
+        keep[r] = k < h ? load_x<TIN, true>(g, (int32_t)g.n - (int32_t)h + (int32_t)k) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = lane + r * kLanes;
+        if (k < h)
+            g.xh[k] = keep[r];
+    }
+    if (lane == 0) {
+        atomicExch(&g.ctl[0], 0u);
+        atomicExch(&g.ctl[1], 0u);
+    }
+}
+
+template <int P, bool C, uint32_t TIN>
+__global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
+{
+    using V = Val<C>;
+    constexpr int kB = kBatch<C>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V *vs = reinterpret_cast<V *>(lds_raw);
+    __shared__ V ends[P][kLanes];
+    __shared__ uint32_t s_last;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t first = blockIdx.x * kBlockOut;
+    const uint32_t cnt = min(kBlockOut, g.n - first);
+    const uint32_t span = cnt + g.warm;
+    const int32_t lo = (int32_t)first - (int32_t)g.warm;  // vs slot(q) holds position lo + q
+
+    // 1. b-part of every position the block's recurrences read, 8 per lane in flight
+    constexpr uint32_t kU = 8;
+    for (uint32_t q0 = 0; q0 < span; q0 += kLanes * kU) {
+        V acc[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u)
+            acc[u] = vzero<C>();
+        for (uint32_t k = 0; k < g.nb; ++k) {
+            const float2 bk = g.b[k];
+            V xv[kU];
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u) {
+                const uint32_t q = q0 + u * kLanes + lane;
+                const int32_t i = lo + (int32_t)q;
+                xv[u] = q < span && i >= 0 ? load_x<TIN, C>(g, i - (int32_t)k) : vzero<C>();
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u)
+                acc[u] = mac<C>(acc[u], xv[u], bk);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t q = q0 + u * kLanes + lane;
+            if (q < span)
+                vs[slot<C>(q)] = acc[u];
+        }
+    }
+    __syncthreads();
+
+    // 2. each lane's recurrence. The segment's outputs overwrite its own
+    // b-part slots. Other lanes read those slots only in their warm-up loop,
+    // which the wave finishes before any lane enters the segment loop.
+    const uint32_t s0 = first + lane * kSeg;
+    const bool active = s0 < g.n;
+    V st[P], st0[P];
+    if (active) {
+        const uint32_t s1 = min(g.n, s0 + kSeg);
+        uint32_t j = s0 > g.warm ? s0 - g.warm : 0u;
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            st[k] = j == 0 ? from2<C>(g.yh[k]) : vzero<C>();
+        for (; j < s0; j += kB) {  // s0 - j and j - lo are multiples of kB
+            V v[kB];
+            lds_read<C>(vs + slot<C>((uint32_t)((int32_t)j - lo)), v);
+#pragma unroll
+            for (int u = 0; u < kB; ++u)
+                (void)fb_step<P, C>(v[u], st, fb);
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            st0[k] = st[k];
+        for (; j + kB <= s1; j += kB) {
+            V v[kB];
+            V *p = vs + slot<C>((uint32_t)((int32_t)j - lo));
+            lds_read<C>(p, v);
+#pragma unroll
+            for (int u = 0; u < kB; ++u)
+                v[u] = fb_step<P, C>(v[u], st, fb);
+            lds_write<C>(p, v);
+        }
+        for (; j < s1; ++j) {
+            V *p = vs + slot<C>((uint32_t)((int32_t)j - lo));
+            *p = fb_step<P, C>(*p, st, fb);
+        }
+        const size_t seg = s0 / kSeg;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            g.rs[seg * P + k] = to2(st0[k]);
+            g.re[seg * P + k] = to2(st[k]);
+            ends[k][lane] = st[k];
+        }
+    }
+    __syncthreads();
+
+    // 3. outputs, coalesced
+    for (uint32_t q = lane; q < cnt; q += kLanes)
+        put_out<C>(g, first + q, vs[slot<C>(q + g.warm)]);
+
+    // 4. segments whose start state differs from the previous lane's end state
+    bool bad = false;
+    if (active && lane > 0) {
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            bad |= !same(st0[k], ends[k][lane - 1]);
+    }
+    const unsigned long long m = __ballot(bad);
+    if (lane == 0) {
+        g.bad[blockIdx.x] = m;
+        if (m)
+            atomicAdd(&g.ctl[1], 1u);
+    }
+
+    // 5. the last block to finish checks across blocks and repairs
+    __threadfence();
+    __syncthreads();
+    if (lane == 0)
+        s_last = atomicAdd(&g.ctl[0], 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!s_last)
+        return;
+    __threadfence();
+    iir_finish<P, C, TIN>(g, fb);
+}
+
+template <int P, bool C, uint32_t TIN> hipError_t launch_p(const IirLaunch &g, const float2 *a, hipStream_t s)
+{
+    Fb<P> fb;
+    for (int k = 0; k < P; ++k)
+        fb.a[k] = a[k];
+    const uint32_t blocks = (g.n + kBlockOut - 1) / kBlockOut;
+    const size_t lds = slots<C>(kBlockOut + g.warm) * sizeof(Val<C>);
+    hipLaunchKernelGGL((iir_kernel<P, C, TIN>), dim3(blocks), dim3(kLanes), lds, s, g, fb);
+    return hipGetLastError();
+}
+
+template <bool C, uint32_t TIN> hipError_t launch_c(const IirLaunch &g, uint32_t p, const float2 *a, hipStream_t s)
+{
+    switch (p) {
+    case 1:
+        return launch_p<1, C, TIN>(g, a, s);
+    case 2:
+        return launch_p<2, C, TIN>(g, a, s);
+    case 4:
+        return launch_p<4, C, TIN>(g, a, s);
+    case 8:
+        return launch_p<8, C, TIN>(g, a, s);
+    default:
+        return launch_p<16, C, TIN>(g, a, s);
+    }
+}
+
+// the reference's std::complex<float> division (g++: Smith's method in
+// double, rounded to float once)
+std::complex<float> cdiv(std::complex<float> x, std::complex<float> y)
+{
+    const double a = x.real(), b = x.imag(), c = y.real(), d = y.imag();
+    if (std::fabs(c) < std::fabs(d)) {
+        const double r = c / d, den = c * r + d;
+        return {(float)((a * r + b) / den), (float)((b * r - a) / den)};
+    }
+    const double r = d / c, den = d * r + c;
+    return {(float)((b * r + a) / den), (float)((b - a * r) / den)};
+}
+
+// samples until the homogeneous recurrence shrinks a unit state below 2^-44
+// (two starting states, so a mode missing from one is seen by the other)
+uint32_t warm_for(const std::vector<float2> &a, uint32_t p)
+{
+    bool zero = true;
+    for (uint32_t k = 0; k < p; ++k)
+        zero &= a[k].x == 0.f && a[k].y == 0.f;
+    if (zero)
+        return kMinWarm;
+    uint32_t worst = 0;
+    for (int trial = 0; trial < 2; ++trial) {
+        std::vector<std::complex<double>> y(p);
+        for (uint32_t k = 0; k < p; ++k)
+            y[k] = trial ? std::complex<double>(k & 1 ? -0.5 : 1.0, k & 2 ? 0.75 : -0.25) : 1.0;
+        uint32_t k = 1;
+        for (; k <= kMaxWarm; ++k) {
+            std::complex<double> v = 0.0;
+            for (uint32_t j = 0; j < p; ++j)
+                v -= std::complex<double>(a[j].x, a[j].y) * y[j];
+            for (uint32_t j = p - 1; j > 0; --j)
+                y[j] = y[j - 1];
+            y[0] = v;
+            double mx = 0.0;
+            for (uint32_t j = 0; j < p; ++j)
+                mx = std::max(mx, std::abs(y[j]));
+            if (mx < 0x1p-44)
+                break;
+        }
+        worst = std::max(worst, k);
+    }
+    return std::min(kMaxWarm, std::max(kMinWarm, (worst + 31) / 32 * 32));
+}
+
+}  // namespace
+
+// ===========================================================================
+// handle
+// ===========================================================================
+struct fdsp_iir_s {
+    std::mutex mu;
+    uint32_t nb = 0, nfb = 0, p = 1, warm = kMinWarm;
+    bool real_coeffs = true;
+    std::vector<float2> b, a;  // normalised taps; a[1..p] (zero past nfb)
+    DevBuf d_b, d_xh, d_yh, d_rs, d_re, d_bad, d_ctl, d_in, d_out;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 *a, uint32_t na, fdsp_iir_t *out)
+{
+    if (!out || !b || !a)
+        return set_err(FDSP_EINVAL, "iir_create: null argument");
+    if (nb == 0 || na == 0)
+        return set_err(FDSP_EINVAL, "iir_create: need at least one b and one a coefficient");
+    if (nb > kMaxB || na - 1 > kMaxFb)
+        return set_err(FDSP_EINVAL, "iir_create: at most %u b and %u + 1 a coefficients", kMaxB, kMaxFb);
+    if (a[0].re == 0.f && a[0].im == 0.f)
+        return set_err(FDSP_EINVAL, "iir_create: a[0] must be nonzero");
+    auto *h = new (std::nothrow) fdsp_iir_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->nb = nb;
+    h->nfb = na - 1;
+    h->p = 1;
+    while (h->p < h->nfb)
+        h->p *= 2;
+    // iir_filter.cc:137-150: divide through by a[0] unless its real part is 1
+    const std::complex<float> a0(a[0].re, a[0].im);
+    const bool scale = a0.real() != 1.0f;
+    for (uint32_t k = 0; k < nb; ++k) {
+        std::complex<float> v(b[k].re, b[k].im);
+        if (scale)
+            v = cdiv(v, a0);
+        h->b.push_back(make_float2(v.real(), v.imag()));
+    }
+    h->a.assign(h->p, make_float2(0.f, 0.f));
+    for (uint32_t k = 1; k < na; ++k) {
+        std::complex<float> v(a[k].re, a[k].im);
+        if (scale)
+            v = cdiv(v, a0);
+        h->a[k - 1] = make_float2(v.real(), v.imag());
+    }
+    for (const float2 &v : h->b)
+        h->real_coeffs &= v.y == 0.f;
+    for (const float2 &v : h->a)
+        h->real_coeffs &= v.y == 0.f;
+    h->warm = warm_for(h->a, h->p);
+    int rc = h->d_b.ensure(nb * sizeof(float2));
+    if (!rc)
+        rc = h->d_xh.ensure(std::max<uint32_t>(nb - 1, 1) * sizeof(float2));
+    if (!rc)
+        rc = h->d_yh.ensure(kMaxFb * sizeof(float2));
+    if (!rc)
+        rc = h->d_ctl.ensure(2 * sizeof(uint32_t));
+    if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipMemcpy(h->d_b.p, h->b.data(), nb * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(h->d_xh.p, 0, h->d_xh.bytes) != hipSuccess ||
+                hipMemset(h->d_yh.p, 0, h->d_yh.bytes) != hipSuccess ||
+                hipMemset(h->d_ctl.p, 0, h->d_ctl.bytes) != hipSuccess))
+        rc = set_err(FDSP_EHIP, "iir_create: HIP setup failed");
+    if (rc) {
+        fdsp_iir_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir_destroy(fdsp_iir_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    for (DevBuf *d : {&h->d_b, &h->d_xh, &h->d_yh, &h->d_rs, &h->d_re, &h->d_bad, &h->d_ctl, &h->d_in, &h->d_out})
+        d->release();
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir_reset(fdsp_iir_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemset(h->d_xh.p, 0, h->d_xh.bytes));
+    HIP_TRY(hipMemset(h->d_yh.p, 0, h->d_yh.bytes));
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir_info(fdsp_iir_t h, uint32_t *warm, uint32_t *order)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (warm)
+        *warm = h->warm;
+    if (order)
+        *order = h->nfb;
+    return FDSP_OK;
+}
+
+namespace fdsp {
+
+int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint32_t tout, void *d_out,
+                    hipStream_t s)
+{
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!d_in || !d_out)
+        return set_err(FDSP_EINVAL, "null buffers");
+    if (int rc = check_chunk(n, 1))
+        return rc;
+    const size_t nseg = (n + kSeg - 1) / kSeg, nblk = (n + kBlockOut - 1) / kBlockOut;
+    if (int rc = h->d_rs.ensure(nseg * h->p * sizeof(float2)))
+        return rc;
+    if (int rc = h->d_re.ensure(nseg * h->p * sizeof(float2)))
+        return rc;
+    if (int rc = h->d_bad.ensure(nblk * sizeof(unsigned long long)))
+        return rc;
+    IirLaunch g{};
+    g.x = d_in;
+    g.out = d_out;
+    g.b = h->d_b.as<float2>();
+    g.xh = h->d_xh.as<float2>();
+    g.yh = h->d_yh.as<float2>();
+    g.rs = h->d_rs.as<float2>();
+    g.re = h->d_re.as<float2>();
+    g.bad = h->d_bad.as<unsigned long long>();
+    g.ctl = h->d_ctl.as<uint32_t>();
+    g.n = (uint32_t)n;
+    g.nb = h->nb;
+    g.warm = h->warm;
+    g.tout = tout;
+    const float2 *a = h->a.data();
+    const bool real_in = tin == kSampleF32 || tin == kSampleI16;
+    hipError_t e;
+    if (h->real_coeffs && real_in)
+        e = tin == kSampleF32 ? launch_c<false, kSampleF32>(g, h->p, a, s) : launch_c<false, kSampleI16>(g, h->p, a, s);
+    else if (tin == kSampleCf32)
+        e = launch_c<true, kSampleCf32>(g, h->p, a, s);
+    else if (tin == kSampleCi16)
+        e = launch_c<true, kSampleCi16>(g, h->p, a, s);
+    else if (tin == kSampleF32)
+        e = launch_c<true, kSampleF32>(g, h->p, a, s);
+    else
+        e = launch_c<true, kSampleI16>(g, h->p, a, s);
+    HIP_TRY(e);
+    return FDSP_OK;
+}
+
+}  // namespace fdsp
+
+namespace {
+
+size_t sample_bytes(uint32_t t)
+{
+    return t == kSampleCf32 ? 8 : t == kSampleCi16 || t == kSampleF32 ? 4 : 2;
+}
+
+int apply_device(fdsp_iir_t h, uint32_t t, const void *d_in, size_t n, void *d_out, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    return iir_apply_typed(h, t, d_in, n, t, d_out, pick(h->stream, stream));
+}
+
+int apply_host(fdsp_iir_t h, uint32_t t, const void *in, size_t n, void *out)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return set_err(FDSP_EINVAL, "empty input");
+    if (!in || !out)
+        return set_err(FDSP_EINVAL, "null buffers");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const size_t bytes = n * sample_bytes(t);
+    if (int rc = h->d_in.ensure(bytes))
+        return rc;
+    if (int rc = h->d_out.ensure(bytes))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, bytes, hipMemcpyHostToDevice, h->stream));
+    if (int rc = iir_apply_typed(h, t, h->d_in.p, n, t, h->d_out.p, h->stream))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, h->d_out.p, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdsp_iir_apply(fdsp_iir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out)
+{
+    return apply_host(h, kSampleCf32, in, n, out);
+}
+extern "C" int fdsp_iir_apply_f32(fdsp_iir_t h, const float *in, size_t n, float *out)
+{
+    return apply_host(h, kSampleF32, in, n, out);
+}
+extern "C" int fdsp_iir_apply_ci16(fdsp_iir_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out)
+{
+    return apply_host(h, kSampleCi16, in, n, out);
+}
+extern "C" int fdsp_iir_apply_i16(fdsp_iir_t h, const int16_t *in, size_t n, int16_t *out)
+{
+    return apply_host(h, kSampleI16, in, n, out);
+}
+extern "C" int fdsp_iir_apply_device(fdsp_iir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                     fdsp_stream_t stream)
+{
+    return apply_device(h, kSampleCf32, d_in, n, d_out, stream);
+}
+extern "C" int fdsp_iir_apply_f32_device(fdsp_iir_t h, const float *d_in, size_t n, float *d_out,
+                                         fdsp_stream_t stream)
+{
+    return apply_device(h, kSampleF32, d_in, n, d_out, stream);
+}
+extern "C" int fdsp_iir_apply_ci16_device(fdsp_iir_t h, const fdsp_ci16 *d_in, size_t n, fdsp_ci16 *d_out,
+                                          fdsp_stream_t stream)
+{
+    return apply_device(h, kSampleCi16, d_in, n, d_out, stream);
+}
+extern "C" int fdsp_iir_apply_i16_device(fdsp_iir_t h, const int16_t *d_in, size_t n, int16_t *d_out,
+                                         fdsp_stream_t stream)
+{
+    return apply_device(h, kSampleI16, d_in, n, d_out, stream);
+}
+
+// ===========================================================================
+// first-order IIR (the FM de-emphasis form): a general handle with
+// b = [b0], a = [1, a1]
+// ===========================================================================
+struct fdsp_iir1_s {
+    fdsp_iir_t iir = nullptr;
+};
+
+extern "C" int fdsp_iir1_create(float b0, float a1, fdsp_iir1_t *out)
+{
+    if (!out)
+        return set_err(FDSP_EINVAL, "null output");
+    auto *h = new (std::nothrow) fdsp_iir1_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    const fdsp_cf32 b[1] = {{b0, 0.f}}, a[2] = {{1.f, 0.f}, {a1, 0.f}};
+    if (int rc = fdsp_iir_create(b, 1, a, 2, &h->iir)) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir1_destroy(fdsp_iir1_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    fdsp_iir_destroy(h->iir);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir1_reset(fdsp_iir1_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    return fdsp_iir_reset(h->iir);
+}
+
+extern "C" int fdsp_iir1_apply_device(fdsp_iir1_t h, const float *d_in, size_t n, float *d_out, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    return fdsp_iir_apply_f32_device(h->iir, d_in, n, d_out, stream);
+}
+
+extern "C" int fdsp_iir1_apply(fdsp_iir1_t h, const float *in, size_t n, float *out)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    return fdsp_iir_apply_f32(h->iir, in, n, out);
+}

@@ -419,6 +419,75 @@ class PolarDiscriminator(_Handle):
         return out[: n.value]
 
 
+class IirFilter(_Handle):
+    """IIR filter y[n] = sum_k b[k] x[n-k] - sum_{j>=1} a[j] y[n-j]
+    (falcon_dsp_iir_filter; reference hdr/transform/falcon_dsp_iir_filter.h:
+    97-130, src/transform/falcon_dsp_iir_filter.cc:104-396). When a[0]'s real
+    part is not 1 all coefficients are divided by a[0], as the reference does.
+    The four apply forms share the stream state; outputs are the reference's
+    bit for bit (up to the sign of zero). int16 outputs are truncated toward
+    zero and saturated. Limits: 1 <= len(b) <= 256, 1 <= len(a) <= 17,
+    a[0] != 0."""
+    _destroy = "fdsp_iir_destroy"
+
+    def __init__(self, b, a):
+        super().__init__()
+        b = np.ascontiguousarray(np.atleast_1d(b), dtype=np.complex64)
+        a = np.ascontiguousarray(np.atleast_1d(a), dtype=np.complex64)
+        check(lib().fdsp_iir_create(_ptr(b), b.size, _ptr(a), a.size, C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_iir_reset(self.h))
+
+    @property
+    def warm(self) -> int:
+        """samples each segment's recurrence runs before its outputs"""
+        w = C.c_uint32()
+        check(lib().fdsp_iir_info(self.h, C.byref(w), None))
+        return w.value
+
+    def apply(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex64)
+        out = np.empty_like(x)
+        check(lib().fdsp_iir_apply(self.h, _ptr(x), x.size, _ptr(out)))
+        return out
+
+    def apply_f32(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty_like(x)
+        check(lib().fdsp_iir_apply_f32(self.h, _ptr(x), x.size, _ptr(out)))
+        return out
+
+    def apply_ci16(self, x) -> np.ndarray:
+        """x: (n, 2) int16 (re, im) pairs; returns the same layout"""
+        x = _ci16(x)
+        out = np.empty_like(x)
+        check(lib().fdsp_iir_apply_ci16(self.h, _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def apply_i16(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.int16)
+        out = np.empty_like(x)
+        check(lib().fdsp_iir_apply_i16(self.h, _ptr(x), x.size, _ptr(out)))
+        return out
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None,
+                     kind: str = "cf32"):
+        """kind: "cf32", "f32", "ci16" or "i16" (both buffers in that layout)"""
+        fn = {"cf32": "fdsp_iir_apply_device", "f32": "fdsp_iir_apply_f32_device",
+              "ci16": "fdsp_iir_apply_ci16_device", "i16": "fdsp_iir_apply_i16_device"}[kind]
+        check(getattr(lib(), fn)(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
+
+
+def iir_filter(b, a, x) -> np.ndarray:
+    """One-shot IIR filter (reference iir_filter.cc:85-100): a fresh filter over x."""
+    x = np.asarray(x)
+    f = IirFilter(b, a)
+    if x.dtype == np.int16 and x.ndim == 2:
+        return f.apply_ci16(x)
+    return f.apply(x)
+
+
 class Iir1(_Handle):
     """First-order IIR on real samples, y[n] = b0 x[n] - a1 y[n-1]
     (falcon_dsp_iir_filter with b = [b0], a = [1, a1]; reference

@@ -32,6 +32,27 @@ static inline void cmul(float ar, float ai, float br, float bi, float *cr, float
     *ci = ar * bi + ai * br;
 }
 
+/* (*xr + j *xi) / (c + j d) in place, as the reference's toolchain (g++ 11)
+ * divides std::complex<float> values: Smith's method (R. L. Smith,
+ * "Algorithm 116: Complex division", CACM 5(8), 1962), scaling by the larger
+ * of |c| and |d|, evaluated in double and rounded to float once. (Checked
+ * against g++'s std::complex<float> operator/= on 4e6 random operands.) The
+ * non-finite recovery branch is not restated: the callers never divide by
+ * zero. */
+static void orc_cdiv(float *xr, float *xi, float c_, float d_)
+{
+    const double a = *xr, b = *xi, c = c_, d = d_;
+    if (fabs(c) < fabs(d)) {
+        const double r = c / d, den = c * r + d;
+        *xr = (float)((a * r + b) / den);
+        *xi = (float)((b * r - a) / den);
+    } else {
+        const double r = d / c, den = d * r + c;
+        *xr = (float)((b * r + a) / den);
+        *xi = (float)((b - a * r) / den);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* utilities -- src/utilities/falcon_dsp_utils.cc                            */
 /* ------------------------------------------------------------------------ */
@@ -590,18 +611,13 @@ int orc_iir_init(orc_iir *f, const float *b, size_t nb, const float *a, size_t n
     memcpy(f->a, a, 2 * na * sizeof(float));
     f->nb = nb;
     f->na = na;
+    /* iir_filter.cc:137-150: when a[0]'s real part is not 1, every b and
+     * a[1:] are divided by a[0] with std::complex<float> division */
     if (na >= 1 && a[0] != 1.0f) {
-        const float dr = a[0], di = a[1], den = dr * dr + di * di;
-        for (size_t i = 0; i < nb; ++i) {
-            const float xr = f->b[2 * i], xi = f->b[2 * i + 1];
-            f->b[2 * i] = (xr * dr + xi * di) / den;
-            f->b[2 * i + 1] = (xi * dr - xr * di) / den;
-        }
-        for (size_t i = 1; i < na; ++i) {
-            const float xr = f->a[2 * i], xi = f->a[2 * i + 1];
-            f->a[2 * i] = (xr * dr + xi * di) / den;
-            f->a[2 * i + 1] = (xi * dr - xr * di) / den;
-        }
+        for (size_t i = 0; i < nb; ++i)
+            orc_cdiv(&f->b[2 * i], &f->b[2 * i + 1], a[0], a[1]);
+        for (size_t i = 1; i < na; ++i)
+            orc_cdiv(&f->a[2 * i], &f->a[2 * i + 1], a[0], a[1]);
     }
     return 0;
 }
@@ -701,6 +717,44 @@ size_t orc_iir_apply_f32(orc_iir *f, const float *in, size_t n, float *out)
         k = orc_iir_apply(f, x, n, y);
         for (size_t i = 0; i < k; ++i)
             out[i] = y[2 * i];
+    }
+    free(x);
+    free(y);
+    return k;
+}
+
+/* iir_filter.cc:157-233: int16 samples converted to complex<float> (real ones
+ * with a zero imaginary part), filtered by the complex overload (whose state
+ * they share), the outputs converted back part by part (the real part only
+ * for real samples) */
+size_t orc_iir_apply_ci16(orc_iir *f, const int16_t *in, size_t n, int16_t *out)
+{
+    float *x = (float *)malloc(2 * (n ? n : 1) * sizeof(float));
+    float *y = (float *)malloc(2 * (n ? n : 1) * sizeof(float));
+    size_t k = 0;
+    if (x && y) {
+        for (size_t i = 0; i < 2 * n; ++i)
+            x[i] = (float)in[i];
+        k = orc_iir_apply(f, x, n, y);
+        for (size_t i = 0; i < 2 * k; ++i)
+            out[i] = orc_i16(y[i]);
+    }
+    free(x);
+    free(y);
+    return k;
+}
+
+size_t orc_iir_apply_i16(orc_iir *f, const int16_t *in, size_t n, int16_t *out)
+{
+    float *x = (float *)calloc(2 * (n ? n : 1), sizeof(float));
+    float *y = (float *)malloc(2 * (n ? n : 1) * sizeof(float));
+    size_t k = 0;
+    if (x && y) {
+        for (size_t i = 0; i < n; ++i)
+            x[2 * i] = (float)in[i];
+        k = orc_iir_apply(f, x, n, y);
+        for (size_t i = 0; i < k; ++i)
+            out[i] = orc_i16(y[2 * i]);
     }
     free(x);
     free(y);

@@ -108,6 +108,8 @@ void orc_iir_free(orc_iir *f);
 void orc_iir_reset(orc_iir *f);
 size_t orc_iir_apply(orc_iir *f, const float *in, size_t n, float *out);
 size_t orc_iir_apply_f32(orc_iir *f, const float *in, size_t n, float *out);
+size_t orc_iir_apply_ci16(orc_iir *f, const int16_t *in, size_t n, int16_t *out);
+size_t orc_iir_apply_i16(orc_iir *f, const int16_t *in, size_t n, int16_t *out);
 
 /* FM mono demodulator (src/signals/falcon_dsp_fm_demod.cc), resampler taps
  * given by the caller */

@@ -94,6 +94,9 @@ def lib():
         for name in ("orc_iir_apply", "orc_iir_apply_f32"):
             getattr(L, name).restype = C.c_size_t
             getattr(L, name).argtypes = [_vp, _f32p, C.c_size_t, _f32p]
+        for name in ("orc_iir_apply_ci16", "orc_iir_apply_i16"):
+            getattr(L, name).restype = C.c_size_t
+            getattr(L, name).argtypes = [_vp, _i16p, C.c_size_t, _i16p]
         L.orc_fm_deemphasis.argtypes = [C.c_double, _f32p, _f32p]
         L.orc_fm_init.argtypes = [_vp, C.c_uint32, C.c_int32, C.c_double, C.c_uint32, C.c_uint32, _f32p, C.c_size_t,
                                   C.c_uint32, C.c_uint32, _f32p, C.c_size_t]
@@ -312,6 +315,18 @@ class Iir(_Obj):
         n = lib().orc_iir_apply_f32(self.p, _fptr(x), x.size, _fptr(out))
         return out[:n]
 
+    def apply_ci16(self, x):
+        x = as_ci16(x)
+        out = np.empty_like(x)
+        n = lib().orc_iir_apply_ci16(self.p, _iptr(x), x.shape[0], _iptr(out))
+        return out[:n]
+
+    def apply_i16(self, x):
+        x = np.ascontiguousarray(x, dtype=np.int16)
+        out = np.empty_like(x)
+        n = lib().orc_iir_apply_i16(self.p, _iptr(x), x.size, _iptr(out))
+        return out[:n]
+
 
 def fm_deemphasis(tau):
     """(b0, a1) of the single-pole de-emphasis (src/signals/falcon_dsp_fm_demod.cc:146-165)."""
@@ -459,6 +474,9 @@ def ref():
         for name in ("ref_iir_apply", "ref_iir_apply_f32"):
             getattr(L, name).restype = C.c_size_t
             getattr(L, name).argtypes = [_vp, _f32p, C.c_size_t, _f32p]
+        for name in ("ref_iir_apply_ci16", "ref_iir_apply_i16"):
+            getattr(L, name).restype = C.c_size_t
+            getattr(L, name).argtypes = [_vp, _i16p, C.c_size_t, _i16p]
         L.ref_iir_reset.argtypes = [_vp]
         L.ref_iir_destroy.argtypes = [_vp]
         L.ref_resample_fir_params.restype = C.c_int64
@@ -606,6 +624,18 @@ class RefIir:
         x = np.ascontiguousarray(x, dtype=np.float32)
         out = np.empty_like(x)
         n = ref().ref_iir_apply_f32(self.h, _fptr(x), x.size, _fptr(out))
+        return out[:n]
+
+    def apply_ci16(self, x):
+        x = as_ci16(x)
+        out = np.empty_like(x)
+        n = ref().ref_iir_apply_ci16(self.h, _iptr(x), x.shape[0], _iptr(out))
+        return out[:n]
+
+    def apply_i16(self, x):
+        x = np.ascontiguousarray(x, dtype=np.int16)
+        out = np.empty_like(x)
+        n = ref().ref_iir_apply_i16(self.h, _iptr(x), x.size, _iptr(out))
         return out[:n]
 
 

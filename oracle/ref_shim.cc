@@ -279,6 +279,25 @@ size_t ref_iir_apply(void *h, const float *in, size_t n, float *out)
         std::memcpy(out, o.data(), o.size() * sizeof(cf32));
     return o.size();
 }
+size_t ref_iir_apply_ci16(void *h, const int16_t *in, size_t n, int16_t *out)
+{
+    auto v = to_vec16(in, n);
+    std::vector<ci16> o;
+    static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h)->apply(v, o);
+    for (size_t i = 0; i < o.size(); ++i) {
+        out[2 * i] = o[i].real();
+        out[2 * i + 1] = o[i].imag();
+    }
+    return o.size();
+}
+size_t ref_iir_apply_i16(void *h, const int16_t *in, size_t n, int16_t *out)
+{
+    std::vector<int16_t> v(in, in + n), o;
+    static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h)->apply(v, o);
+    if (!o.empty())
+        std::memcpy(out, o.data(), o.size() * sizeof(int16_t));
+    return o.size();
+}
 void ref_iir_reset(void *h) { static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h)->reset_state(); }
 void ref_iir_destroy(void *h) { delete static_cast<falcon_dsp::falcon_dsp_iir_filter *>(h); }
 
