@@ -128,6 +128,25 @@ def timed_launches(fd, stream, launch, reps):
     return e0.elapsed_ms(e1) / reps
 
 
+def iir_cpu_baseline():
+    """The reference's IIR filter (oracle/_ref; else the oracle restatement)
+    on the iir_order4 filter over the first 2^20 of its samples, one thread
+    (one stream's recurrence is sequential on the CPU too)."""
+    from libfalcon_dsp_amd.workloads import IirOrder4
+    from oracle import oracle as O
+    w = IirOrder4()
+    b, a = w.coeffs()
+    n = 1 << 20
+    x = w.input()[:n]
+    kind = "reference" if O.ref_available() else "port"
+    f = (O.RefIir if kind == "reference" else O.Iir)(b, a)
+    t0 = time.perf_counter()
+    f.apply(x)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt / 1e6, 2), "unit": "output Msps", "cores": 1, "kind": kind,
+            "sample": "first 2^20 samples of iir_order4, one thread"}
+
+
 def bench_other_configs(fd, stream, steps):
     """The other workloads, each with data resident in HBM: throughput and
     roofline fraction of one pass."""
@@ -227,10 +246,42 @@ def bench_other_configs(fd, stream, steps):
         "value": round(16 * w2.n / (ms3 / 1e3) / 1e6, 1), "unit": "output Msps", "ms_per_pass": round(ms3, 4),
         "roofline": {"bound": "hbm", "achieved": round(b3 / (ms3 / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(b3 / (ms3 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "note": "as nco_16shift with complex int16 in and out (68 B per input sample); FP64 phasor per "
-                "sample, as the reference computes int16 outputs"}
+        "note": "as nco_16shift with complex int16 in and out (68 B per input sample); the reference's "
+                "per-sample phasors from a host-built table, as the reference computes int16 outputs"}
     log(f"nco_16shift_ci16: {ms3:.3f} ms per pass, {out['nco_16shift_ci16']['roofline']['achieved']} GB/s")
     del d3, o3, f, x3
+
+    # iir_order4: 2^24 complex samples through a 4th-order Butterworth, 16 chunks of 2^20,
+    # in the default mode (float outputs within rounding: this filter's segments do not
+    # merge bit for bit) and in exact mode (the reference's outputs bit for bit)
+    from libfalcon_dsp_amd.workloads import IirOrder4
+    w6 = IirOrder4()
+    bq, aq = w6.coeffs()
+    d6 = fd.DeviceBuffer.from_array(w6.input())
+    o6 = fd.DeviceBuffer(w6.n)
+    for key, exact in (("iir_order4", None), ("iir_order4_exact", True)):
+        g6 = fd.IirFilter(bq, aq)
+        g6.set_exact(exact)
+
+        def pass6():
+            for k in range(w6.n // w6.chunk):
+                check(fd.lib().fdsp_iir_apply_device(g6.h, d6.at(k * w6.chunk), w6.chunk, o6.at(k * w6.chunk),
+                                                     stream.h))
+
+        ms6 = timed_launches(fd, stream, pass6, 2 if exact else max(2, steps // 4))
+        b6 = 16 * w6.n
+        out[key] = {
+            "value": round(w6.n / (ms6 / 1e3) / 1e6, 1), "unit": "output Msps", "ms_per_pass": round(ms6, 4),
+            "plan": g6.plan, "warm": g6.warm,
+            "roofline": {"bound": "hbm", "achieved": round(b6 / (ms6 / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(b6 / (ms6 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "note": "2^24 cf32 samples in 16 chunks of 2^20, 4th-order Butterworth (5 b, 5 a taps), 16 B per "
+                    "sample; " + ("bit-identical to the reference's loop, which for this filter is one sequential "
+                                  "recurrence per stream" if exact else "parallel segments warmed up 160 samples, "
+                                  "float outputs within rounding of the reference's (tested at 1e-4 x RMS)")}
+        log(f"{key}: {ms6:.3f} ms per pass, {out[key]['value']} Msps (plan {g6.plan}, warm {g6.warm})")
+        del g6
+    del d6, o6
 
     # fir_bank_256: 256 channels x 2^20, 512 real taps
     w4 = FirBank256()
@@ -465,6 +516,9 @@ def main():
         cpu = cpu_baseline(w.fs, chans, x, 1 << 22, n)
         log(f"CPU baseline: {cpu['value']} Msps on {cpu['cores']} threads, "
             f"{cpu['single_thread_value']} Msps single thread ({cpu['kind']})")
+        if others and "iir_order4" in others:
+            others["iir_order4"]["cpu_baseline"] = iir_cpu_baseline()
+            log(f"IIR CPU baseline: {others['iir_order4']['cpu_baseline']['value']} Msps single thread")
 
     rec = {
         "metric": "aggregate output Msps (32-ch multi-rate channelizer) + p99 chunk latency",

@@ -195,19 +195,39 @@ int fdsp_discriminator_apply_ci16_device(fdsp_discriminator_t h, const fdsp_ci16
  * sum_{j>=1} a[j] y[n-j]; when a[0]'s real part is not 1 every b and a[1:] is
  * first divided by a[0], as the reference does (its complex<float> division).
  * Inputs and outputs are carried between calls; one handle's calls may mix
- * the four sample types. Outputs are the reference's bit for bit (up to the
- * sign of zero) for any coefficients: the GPU recurrence runs in segments and
- * verifies that each segment's start state equals its predecessor's exact end
- * state, recomputing any that does not. Filters that decay slowly (or not at
- * all) stay exact but run slower. int16 outputs are truncated toward zero and
+ * the four sample types. int16 outputs are truncated toward zero and
  * saturated (out of range is undefined in the reference). Limits: 1 <= nb <=
- * 256, 1 <= na <= 17, a[0] != 0 (the reference would divide by zero). */
+ * 256, 1 <= na <= 17, a[0] != 0 (the reference would divide by zero).
+ *
+ * The recurrence runs in parallel segments, each started `warm` samples
+ * early from a zero state. When such segments meet the exact trajectory bit
+ * for bit (a trial at create decides; first-order filters such as the FM
+ * de-emphasis do), each is verified against its predecessor's exact end
+ * state and recomputed if it differs: outputs are the reference's bit for
+ * bit (up to the sign of zero). Higher-order filters whose feedback amplifies
+ * rounding differences keep a few-ulp gap for ever; for them the modes are:
+ *   FDSP_IIR_AUTO (default): float outputs from the unverified segments
+ *     (within float rounding of the reference, as the library's other float
+ *     kernels are), int16 outputs from one sequential wave (bit for bit);
+ *     a filter that does not decay within 2048 samples runs sequentially;
+ *   FDSP_IIR_EXACT: always bit for bit (a single stream's exact recurrence is
+ *     inherently serial and runs as one wave);
+ *   FDSP_IIR_FAST: always the unverified segments. */
+#define FDSP_IIR_FAST 0
+#define FDSP_IIR_EXACT 1
+#define FDSP_IIR_AUTO 2
+#define FDSP_IIR_PLAN_FAST 0
+#define FDSP_IIR_PLAN_SEGMENTS 1
+#define FDSP_IIR_PLAN_SERIAL 2
 typedef struct fdsp_iir_s *fdsp_iir_t;
 int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 *a, uint32_t na, fdsp_iir_t *out);
 int fdsp_iir_destroy(fdsp_iir_t h);
 int fdsp_iir_reset(fdsp_iir_t h);
-/* warm: samples each segment's recurrence runs before its outputs; order: na - 1 */
-int fdsp_iir_info(fdsp_iir_t h, uint32_t *warm, uint32_t *order);
+/* FDSP_IIR_AUTO (default), FDSP_IIR_EXACT or FDSP_IIR_FAST for the following calls */
+int fdsp_iir_set_mode(fdsp_iir_t h, int mode);
+/* warm: samples each segment's recurrence runs before its outputs; order:
+ * na - 1; plan: the FDSP_IIR_PLAN_* a call with complex float outputs runs */
+int fdsp_iir_info(fdsp_iir_t h, uint32_t *warm, uint32_t *order, uint32_t *plan);
 int fdsp_iir_apply(fdsp_iir_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out);
 int fdsp_iir_apply_f32(fdsp_iir_t h, const float *in, size_t n, float *out);
 int fdsp_iir_apply_ci16(fdsp_iir_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out);

@@ -165,7 +165,8 @@ _SIGS = [
     ("fdsp_iir_create", _i, [_vp, _u32, _vp, _u32, _P(_vp)]),
     ("fdsp_iir_destroy", _i, [_vp]),
     ("fdsp_iir_reset", _i, [_vp]),
-    ("fdsp_iir_info", _i, [_vp, _P(_u32), _P(_u32)]),
+    ("fdsp_iir_info", _i, [_vp, _P(_u32), _P(_u32), _P(_u32)]),
+    ("fdsp_iir_set_mode", _i, [_vp, C.c_int]),
     ("fdsp_iir_apply", _i, [_vp, _vp, _sz, _vp]),
     ("fdsp_iir_apply_f32", _i, [_vp, _vp, _sz, _vp]),
     ("fdsp_iir_apply_ci16", _i, [_vp, _vp, _sz, _vp]),

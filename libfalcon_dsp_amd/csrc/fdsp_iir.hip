@@ -6,7 +6,7 @@
 // then subtracts a1 y[n-1], a2 y[n-2], ... in that order, every product a
 // std::complex<float> product. The b-part v[n] depends on inputs only, so a
 // block computes it for all its positions in parallel; the feedback is a
-// recurrence on v. Each lane owns kSeg consecutive outputs and starts the same
+// recurrence on v. Each lane owns `seg` consecutive outputs and starts the same
 // recurrence `warm` samples earlier from a zero state (or at the chunk start
 // from the carried state). For a stable filter two float trajectories draw
 // together and, once one step's products round alike, are identical from then
@@ -16,10 +16,12 @@
 // segment starts and when it ends. A segment whose start state equals its
 // predecessor's end state bit for bit (up to the sign of zero) computed the
 // sequential outputs exactly, by induction from the chunk's first segment,
-// which starts from the carried state. Lanes compare within their block; the
-// last block to finish compares across blocks and, if any segment failed,
-// recomputes the failed segments in order from their predecessors' exact end
-// states (a slower path that keeps unstable or slowly decaying filters exact).
+// which starts from the carried state. Lanes compare within their block; a
+// one-wave kernel launched next compares across blocks and, if any segment
+// failed, recomputes the failed segments in order from their predecessors'
+// exact end states. Filters whose segments do not merge run as one
+// sequential wave instead (iir_serial_kernel), or, when the caller accepts
+// float rounding, as unverified segments.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -39,8 +41,7 @@ using namespace fdsp;
 namespace {
 
 constexpr uint32_t kLanes = 64;                  // one wave per block (lockstep in-place LDS use)
-constexpr uint32_t kSeg = 64;                    // outputs per lane
-constexpr uint32_t kBlockOut = kLanes * kSeg;    // outputs per block
+constexpr uint32_t kMaxSeg = 64;                 // outputs per lane: 8 to 64, by chunk length
 constexpr uint32_t kMinWarm = 32, kMaxWarm = 2048;
 constexpr uint32_t kMaxB = 256, kMaxFb = 16;
 
@@ -64,7 +65,7 @@ __device__ __forceinline__ float2 to2(float2 v) { return v; }
 __device__ __forceinline__ float2 to2(float v) { return make_float2(v, 0.f); }
 
 // LDS slot of span position q: 16 bytes of padding after every 64 values, so
-// lanes reading at a stride of kSeg values start on different banks
+// lanes reading at a stride of `seg` values start on different banks
 template <bool C> __device__ __forceinline__ uint32_t slot(uint32_t q) { return q + (C ? 2u : 4u) * (q >> 6); }
 template <bool C> constexpr uint32_t slots(uint32_t span) { return span + (C ? 2u : 4u) * ((span + 63) >> 6); }
 
@@ -80,8 +81,11 @@ struct IirLaunch {
     float2 *yh;               // kMaxFb carried outputs, newest first
     float2 *rs, *re;          // per segment: P start-state values, P end-state values
     unsigned long long *bad;  // per block: bit t set when segment t's start differs from t-1's end
-    uint32_t *ctl;            // [0] blocks finished, [1] blocks with a failed segment
+    uint32_t *ctl;            // [1]: blocks with a failed segment (the finishing wave clears it)
     uint32_t n, nb, warm, tout;
+    uint32_t verify;          // 0: segments are trusted (outputs within float rounding)
+    uint32_t split;           // serial form, complex samples, real a: re and im run on lanes 0 and 1
+    uint32_t seg;             // outputs per lane (8, 16, 32 or 64); a block owns 64 segments
 };
 template <int P> struct Fb {
     float2 a[P];  // a[1..P] of the normalised denominator (zero past na - 1)
@@ -170,14 +174,69 @@ template <uint32_t TIN, bool C> __device__ __forceinline__ Val<C> bpart(const Ii
     return acc;
 }
 
-// 32 real or 16 complex values: eight 16-byte LDS accesses
-template <bool C> constexpr int kBatch = C ? 16 : 32;
+// The b-part of positions lo .. lo + span - 1 into buf[slot(0 .. span - 1)].
+// The inputs lo - (nb - 1) .. lo + span - 1 are first staged at buf[slot(t)],
+// t = position - lo + nb - 1, with coalesced loads, 16 per lane in flight.
+// Then rows of 64 positions, in ascending order, replace them: the b-part of
+// q goes to slot q, whose input (position lo + q - nb + 1) no later row
+// reads, and a row's reads all precede its writes (one wave, in order).
+template <uint32_t TIN, bool C> __device__ void stage_bpart(const IirLaunch &g, Val<C> *buf, int32_t lo, uint32_t span)
+{
+    const uint32_t lane = threadIdx.x, h = g.nb - 1, tot = span + h;
+    const int32_t base = lo - (int32_t)h;
+    constexpr uint32_t kU = 16;
+    for (uint32_t t0 = 0; t0 < tot; t0 += kLanes * kU) {
+        Val<C> xv[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t t = t0 + u * kLanes + lane;
+            const int32_t p = base + (int32_t)t;
+            xv[u] = t < tot && p >= -(int32_t)h && p < (int32_t)g.n ? load_x<TIN, C>(g, p) : vzero<C>();
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t t = t0 + u * kLanes + lane;
+            if (t < tot)
+                buf[slot<C>(t)] = xv[u];
+        }
+    }
+    __syncthreads();
+    // eight rows at a time: their reads (all at or above the group's first
+    // row) before their writes
+    constexpr uint32_t kR = 8;
+    for (uint32_t q0 = 0; q0 < span; q0 += kLanes * kR) {
+        Val<C> acc[kR];
+#pragma unroll
+        for (uint32_t r = 0; r < kR; ++r)
+            acc[r] = vzero<C>();
+        for (uint32_t k = 0; k < g.nb; ++k) {
+            const float2 bk = g.b[k];
+#pragma unroll
+            for (uint32_t r = 0; r < kR; ++r) {
+                const uint32_t q = q0 + r * kLanes + lane;
+                if (q < span)
+                    acc[r] = mac<C>(acc[r], buf[slot<C>(q + h - k)], bk);
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kR; ++r) {
+            const uint32_t q = q0 + r * kLanes + lane;
+            if (q < span)
+                buf[slot<C>(q)] = acc[r];
+        }
+    }
+    __syncthreads();
+}
 
-template <bool C> __device__ __forceinline__ void lds_read(const Val<C> *p, Val<C> (&v)[kBatch<C>])
+// 8 values: two (real) or four (complex) 16-byte LDS accesses; a batch
+// starts at a multiple of 8 and so never crosses a padding gap
+constexpr int kBatch = 8;
+
+template <bool C> __device__ __forceinline__ void lds_read(const Val<C> *p, Val<C> (&v)[kBatch])
 {
     const float4 *q = reinterpret_cast<const float4 *>(p);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < (C ? 4 : 2); ++i) {
         const float4 w = q[i];
         if constexpr (C) {
             v[2 * i] = make_float2(w.x, w.y);
@@ -190,11 +249,11 @@ template <bool C> __device__ __forceinline__ void lds_read(const Val<C> *p, Val<
         }
     }
 }
-template <bool C> __device__ __forceinline__ void lds_write(Val<C> *p, const Val<C> (&v)[kBatch<C>])
+template <bool C> __device__ __forceinline__ void lds_write(Val<C> *p, const Val<C> (&v)[kBatch])
 {
     float4 *q = reinterpret_cast<float4 *>(p);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < (C ? 4 : 2); ++i) {
         if constexpr (C)
             q[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
         else
@@ -208,17 +267,39 @@ __device__ __forceinline__ unsigned long long shfl64(unsigned long long v, int l
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Run by the last block to finish, after every other block's outputs,
-// records and failure bits are visible: check the block boundaries, repair
-// failed segments in order, then carry the stream state to the next call.
-template <int P, bool C, uint32_t TIN> __device__ void iir_finish(const IirLaunch &g, const Fb<P> &fb)
+// carried inputs: the last nb - 1 of (old history, this chunk); every read
+// of the old history is done before the first write
+template <uint32_t TIN> __device__ void carry_inputs(const IirLaunch &g)
+{
+    const uint32_t lane = threadIdx.x, h = g.nb - 1;
+    float2 keep[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = lane + r * kLanes;
+        keep[r] = k < h ? load_x<TIN, true>(g, (int32_t)g.n - (int32_t)h + (int32_t)k) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = lane + r * kLanes;
+        if (k < h)
+            g.xh[k] = keep[r];
+    }
+}
+
+// One wave, launched after iir_kernel on the same stream (so every block's
+// outputs, records and failure bits are visible): check the block
+// boundaries, repair failed segments in order, then carry the stream state to
+// the next call.
+template <int P, bool C, uint32_t TIN>
+__global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> fb)
 {
     using V = Val<C>;
     __shared__ V fixed_end[P];  // end state of the most recently repaired segment
-    __shared__ V stash[kSeg];
+    __shared__ V stash[kMaxSeg];
     __shared__ int32_t fixed_seg;
-    const uint32_t lane = threadIdx.x, nblk = gridDim.x;
-    const uint32_t nseg = (g.n + kSeg - 1) / kSeg;
+    const uint32_t lane = threadIdx.x, S = g.seg, nblk = (g.n + kLanes * S - 1) / (kLanes * S);
+    const uint32_t nseg = (g.n + S - 1) / S;
     if (lane == 0)
         fixed_seg = -1;
     auto boundary_bad = [&](uint32_t b) {  // block b's first segment against its predecessor
@@ -231,7 +312,7 @@ template <int P, bool C, uint32_t TIN> __device__ void iir_finish(const IirLaunc
     };
     constexpr uint32_t kU = P >= 8 ? 1u : 8u / P;  // boundary checks in flight per lane
     bool any = false;
-    for (uint32_t b0 = 1; b0 < nblk; b0 += kLanes * kU) {
+    for (uint32_t b0 = 1; g.verify && b0 < nblk; b0 += kLanes * kU) {
         bool d[kU];
 #pragma unroll
         for (uint32_t u = 0; u < kU; ++u) {
@@ -242,7 +323,7 @@ template <int P, bool C, uint32_t TIN> __device__ void iir_finish(const IirLaunc
         for (uint32_t u = 0; u < kU; ++u)
             any |= d[u];
     }
-    any = __ballot(any) != 0ull || atomicAdd(&g.ctl[1], 0u) != 0u;
+    any = g.verify && (__ballot(any) != 0ull || atomicAdd(&g.ctl[1], 0u) != 0u);
     __syncthreads();
 
     if (any) {
@@ -250,7 +331,7 @@ template <int P, bool C, uint32_t TIN> __device__ void iir_finish(const IirLaunc
         // whose predecessor was never repaired is a real failure; after a
         // repair the successor is checked against the new end state directly.
         auto repair = [&](uint32_t f) {
-            const uint32_t p0 = f * kSeg, cnt = min(kSeg, g.n - p0);
+            const uint32_t p0 = f * S, cnt = min(S, g.n - p0);
             stash[lane] = lane < cnt ? bpart<TIN, C>(g, (int32_t)(p0 + lane)) : vzero<C>();
             __syncthreads();
             if (lane == 0) {
@@ -313,79 +394,37 @@ template <int P, bool C, uint32_t TIN> __device__ void iir_finish(const IirLaunc
     const uint32_t last = nseg - 1;
     if (lane < (uint32_t)P)
         g.yh[lane] = to2(fixed_seg == (int32_t)last ? fixed_end[lane] : from2<C>(g.re[(size_t)last * P + lane]));
-    // carried inputs: the last nb - 1 of (old history, this chunk)
-    const uint32_t h = g.nb - 1;
-    float2 keep[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t k = lane + r * kLanes;
-        keep[r] = k < h ? load_x<TIN, true>(g, (int32_t)g.n - (int32_t)h + (int32_t)k) : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t k = lane + r * kLanes;
-        if (k < h)
-            g.xh[k] = keep[r];
-    }
-    if (lane == 0) {
-        atomicExch(&g.ctl[0], 0u);
+    carry_inputs<TIN>(g);
+    if (lane == 0)
         atomicExch(&g.ctl[1], 0u);
-    }
 }
 
 template <int P, bool C, uint32_t TIN>
 __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
 {
     using V = Val<C>;
-    constexpr int kB = kBatch<C>;
+    constexpr int kB = kBatch;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     V *vs = reinterpret_cast<V *>(lds_raw);
     __shared__ V ends[P][kLanes];
-    __shared__ uint32_t s_last;
     const uint32_t lane = threadIdx.x;
-    const uint32_t first = blockIdx.x * kBlockOut;
-    const uint32_t cnt = min(kBlockOut, g.n - first);
+    const uint32_t S = g.seg;
+    const uint32_t first = blockIdx.x * kLanes * S;
+    const uint32_t cnt = min(kLanes * S, g.n - first);
     const uint32_t span = cnt + g.warm;
     const int32_t lo = (int32_t)first - (int32_t)g.warm;  // vs slot(q) holds position lo + q
 
-    // 1. b-part of every position the block's recurrences read, 8 per lane in flight
-    constexpr uint32_t kU = 8;
-    for (uint32_t q0 = 0; q0 < span; q0 += kLanes * kU) {
-        V acc[kU];
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u)
-            acc[u] = vzero<C>();
-        for (uint32_t k = 0; k < g.nb; ++k) {
-            const float2 bk = g.b[k];
-            V xv[kU];
-#pragma unroll
-            for (uint32_t u = 0; u < kU; ++u) {
-                const uint32_t q = q0 + u * kLanes + lane;
-                const int32_t i = lo + (int32_t)q;
-                xv[u] = q < span && i >= 0 ? load_x<TIN, C>(g, i - (int32_t)k) : vzero<C>();
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kU; ++u)
-                acc[u] = mac<C>(acc[u], xv[u], bk);
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            const uint32_t q = q0 + u * kLanes + lane;
-            if (q < span)
-                vs[slot<C>(q)] = acc[u];
-        }
-    }
-    __syncthreads();
+    // 1. b-part of every position the block's recurrences read
+    stage_bpart<TIN, C>(g, vs, lo, span);
 
     // 2. each lane's recurrence. The segment's outputs overwrite its own
     // b-part slots. Other lanes read those slots only in their warm-up loop,
     // which the wave finishes before any lane enters the segment loop.
-    const uint32_t s0 = first + lane * kSeg;
+    const uint32_t s0 = first + lane * S;
     const bool active = s0 < g.n;
     V st[P], st0[P];
     if (active) {
-        const uint32_t s1 = min(g.n, s0 + kSeg);
+        const uint32_t s1 = min(g.n, s0 + S);
         uint32_t j = s0 > g.warm ? s0 - g.warm : 0u;
 #pragma unroll
         for (int k = 0; k < P; ++k)
@@ -413,11 +452,15 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
             V *p = vs + slot<C>((uint32_t)((int32_t)j - lo));
             *p = fb_step<P, C>(*p, st, fb);
         }
-        const size_t seg = s0 / kSeg;
+        // records: every segment's when verifying, else the last one's (the carried state)
+        const size_t seg = s0 / S;
+        const bool keep = g.verify || s1 == g.n;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            g.rs[seg * P + k] = to2(st0[k]);
-            g.re[seg * P + k] = to2(st[k]);
+            if (keep) {
+                g.rs[seg * P + k] = to2(st0[k]);
+                g.re[seg * P + k] = to2(st[k]);
+            }
             ends[k][lane] = st[k];
         }
     }
@@ -429,7 +472,7 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
 
     // 4. segments whose start state differs from the previous lane's end state
     bool bad = false;
-    if (active && lane > 0) {
+    if (g.verify && active && lane > 0) {
 #pragma unroll
         for (int k = 0; k < P; ++k)
             bad |= !same(st0[k], ends[k][lane - 1]);
@@ -440,43 +483,124 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
         if (m)
             atomicAdd(&g.ctl[1], 1u);
     }
-
-    // 5. the last block to finish checks across blocks and repairs
-    __threadfence();
-    __syncthreads();
-    if (lane == 0)
-        s_last = atomicAdd(&g.ctl[0], 1u) == gridDim.x - 1 ? 1u : 0u;
-    __syncthreads();
-    if (!s_last)
-        return;
-    __threadfence();
-    iir_finish<P, C, TIN>(g, fb);
+    // iir_finish_kernel, next on the stream, checks across blocks and repairs
 }
 
-template <int P, bool C, uint32_t TIN> hipError_t launch_p(const IirLaunch &g, const float2 *a, hipStream_t s)
+// The sequential form, for filters whose segments do not merge: one wave
+// walks the chunk in tiles. All lanes form a tile's b-part in LDS, lane 0 runs
+// the recurrence over it in place, all lanes write the outputs.
+constexpr uint32_t kTile = 4096;
+
+template <int P, bool C, uint32_t TIN>
+__global__ __launch_bounds__(kLanes) void iir_serial_kernel(IirLaunch g, Fb<P> fb)
+{
+    using V = Val<C>;
+    constexpr int kB = kBatch;
+    __shared__ __align__(16) V tile[slots<C>(kTile + kMaxB - 1)];
+    const uint32_t lane = threadIdx.x;
+    V st[P];
+    float sr[P];  // the split form's real state: lane 0 the real parts, lane 1 the imaginary
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        st[k] = from2<C>(g.yh[k]);
+        sr[k] = lane == 1 ? g.yh[k].y : g.yh[k].x;
+    }
+    for (uint32_t t0 = 0; t0 < g.n; t0 += kTile) {
+        const uint32_t cnt = min(kTile, g.n - t0);
+        stage_bpart<TIN, C>(g, tile, (int32_t)t0, cnt);
+        __syncthreads();
+        if constexpr (C) {
+            if (g.split) {
+                // real feedback on complex samples: the real and imaginary
+                // parts are two independent real recurrences (the products
+                // with the zero imaginary parts of a only add signed zeros)
+                if (lane < 2) {
+                    uint32_t q = 0;
+                    for (; q + 16 <= cnt; q += 16) {  // 16 values never cross a padding gap
+                        float *part = reinterpret_cast<float *>(tile + slot<C>(q)) + lane;
+                        float v[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u)
+                            v[u] = part[2 * u];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u)
+                            part[2 * u] = fb_step<P, false>(v[u], sr, fb);
+                    }
+                    for (; q < cnt; ++q) {
+                        float *part = reinterpret_cast<float *>(tile + slot<C>(q)) + lane;
+                        *part = fb_step<P, false>(*part, sr, fb);
+                    }
+                }
+                __syncthreads();
+                for (uint32_t q = lane; q < cnt; q += kLanes)
+                    put_out<C>(g, t0 + q, tile[slot<C>(q)]);
+                __syncthreads();
+                continue;
+            }
+        }
+        if (lane == 0) {
+            uint32_t q = 0;
+            for (; q + kB <= cnt; q += kB) {
+                V v[kB];
+                lds_read<C>(tile + slot<C>(q), v);
+#pragma unroll
+                for (int u = 0; u < kB; ++u)
+                    v[u] = fb_step<P, C>(v[u], st, fb);
+                lds_write<C>(tile + slot<C>(q), v);
+            }
+            for (; q < cnt; ++q)
+                tile[slot<C>(q)] = fb_step<P, C>(tile[slot<C>(q)], st, fb);
+        }
+        __syncthreads();
+        for (uint32_t q = lane; q < cnt; q += kLanes)
+            put_out<C>(g, t0 + q, tile[slot<C>(q)]);
+        __syncthreads();
+    }
+    if (C && g.split) {
+        if (lane < 2) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                reinterpret_cast<float *>(g.yh + k)[lane] = sr[k];
+        }
+    } else if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            g.yh[k] = to2(st[k]);
+    }
+    carry_inputs<TIN>(g);
+}
+
+template <int P, bool C, uint32_t TIN>
+hipError_t launch_p(const IirLaunch &g, const float2 *a, bool serial, hipStream_t s)
 {
     Fb<P> fb;
     for (int k = 0; k < P; ++k)
         fb.a[k] = a[k];
-    const uint32_t blocks = (g.n + kBlockOut - 1) / kBlockOut;
-    const size_t lds = slots<C>(kBlockOut + g.warm) * sizeof(Val<C>);
+    if (serial) {
+        hipLaunchKernelGGL((iir_serial_kernel<P, C, TIN>), dim3(1), dim3(kLanes), 0, s, g, fb);
+        return hipGetLastError();
+    }
+    const uint32_t blocks = (g.n + kLanes * g.seg - 1) / (kLanes * g.seg);
+    const size_t lds = slots<C>(kLanes * g.seg + g.warm + g.nb - 1) * sizeof(Val<C>);
     hipLaunchKernelGGL((iir_kernel<P, C, TIN>), dim3(blocks), dim3(kLanes), lds, s, g, fb);
+    hipLaunchKernelGGL((iir_finish_kernel<P, C, TIN>), dim3(1), dim3(kLanes), 0, s, g, fb);
     return hipGetLastError();
 }
 
-template <bool C, uint32_t TIN> hipError_t launch_c(const IirLaunch &g, uint32_t p, const float2 *a, hipStream_t s)
+template <bool C, uint32_t TIN>
+hipError_t launch_c(const IirLaunch &g, uint32_t p, const float2 *a, bool serial, hipStream_t s)
 {
     switch (p) {
     case 1:
-        return launch_p<1, C, TIN>(g, a, s);
+        return launch_p<1, C, TIN>(g, a, serial, s);
     case 2:
-        return launch_p<2, C, TIN>(g, a, s);
+        return launch_p<2, C, TIN>(g, a, serial, s);
     case 4:
-        return launch_p<4, C, TIN>(g, a, s);
+        return launch_p<4, C, TIN>(g, a, serial, s);
     case 8:
-        return launch_p<8, C, TIN>(g, a, s);
+        return launch_p<8, C, TIN>(g, a, serial, s);
     default:
-        return launch_p<16, C, TIN>(g, a, s);
+        return launch_p<16, C, TIN>(g, a, serial, s);
     }
 }
 
@@ -493,10 +617,13 @@ std::complex<float> cdiv(std::complex<float> x, std::complex<float> y)
     return {(float)((b * r + a) / den), (float)((b - a * r) / den)};
 }
 
-// samples until the homogeneous recurrence shrinks a unit state below 2^-44
-// (two starting states, so a mode missing from one is seen by the other)
-uint32_t warm_for(const std::vector<float2> &a, uint32_t p)
+// samples until the homogeneous recurrence shrinks a unit state below
+// `floor` (two starting states, so a mode missing from one is seen by the
+// other): 2^-44 for segments that must meet the exact trajectory, 2^-26 for
+// unverified ones (a start error that far below float rounding)
+uint32_t warm_for(const std::vector<float2> &a, uint32_t p, bool *decays, double floor = 0x1p-44)
 {
+    *decays = true;
     bool zero = true;
     for (uint32_t k = 0; k < p; ++k)
         zero &= a[k].x == 0.f && a[k].y == 0.f;
@@ -518,12 +645,77 @@ uint32_t warm_for(const std::vector<float2> &a, uint32_t p)
             double mx = 0.0;
             for (uint32_t j = 0; j < p; ++j)
                 mx = std::max(mx, std::abs(y[j]));
-            if (mx < 0x1p-44)
+            if (mx < floor)
                 break;
         }
         worst = std::max(worst, k);
     }
+    *decays = worst <= kMaxWarm;
     return std::min(kMaxWarm, std::max(kMinWarm, (worst + 31) / 32 * 32));
+}
+
+// Whether zero-started segments meet the true trajectory bit for bit within
+// `warm` samples, tried on 16 starts in a deterministic noise input with the
+// GPU's rounding (complex products and sums in float, no fused multiply-add).
+// Filters whose rounding differences are amplified by their feedback keep a
+// few-ulp gap for ever; those run sequentially.
+bool segments_merge(const std::vector<float2> &b, const std::vector<float2> &a, uint32_t p, uint32_t warm)
+{
+#pragma clang fp contract(off)
+    using cx = std::complex<float>;
+    auto mul = [](cx x, cx y) {
+        return cx(x.real() * y.real() - x.imag() * y.imag(), x.real() * y.imag() + x.imag() * y.real());
+    };
+    auto step = [&](cx v, std::vector<cx> &st) {
+        cx y = v;
+        for (uint32_t k = 0; k < p; ++k) {
+            const cx q = mul(st[k], cx(a[k].x, a[k].y));
+            y = cx(y.real() - q.real(), y.imag() - q.imag());
+        }
+        st.insert(st.begin(), y);
+        st.pop_back();
+    };
+    const uint32_t kTrials = 16, first = 256, gap = 96, n = first + kTrials * gap + warm;
+    std::vector<cx> v(n);
+    uint64_t r = 0x9e3779b97f4a7c15ull;
+    std::vector<cx> x(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        auto u = [&]() {
+            r = r * 6364136223846793005ull + 1442695040888963407ull;
+            return (float)((double)(r >> 11) * 0x1p-53 * 2.0 - 1.0);
+        };
+        const float re = u(), im = u();
+        x[i] = cx(re, im);
+        cx acc(0.f, 0.f);
+        for (uint32_t k = 0; k < b.size() && k <= i; ++k) {
+            const cx q = mul(x[i - k], cx(b[k].x, b[k].y));
+            acc = cx(acc.real() + q.real(), acc.imag() + q.imag());
+        }
+        v[i] = acc;
+    }
+    std::vector<std::vector<cx>> truth(n);
+    std::vector<cx> st(p, cx(0.f, 0.f));
+    for (uint32_t i = 0; i < n; ++i) {
+        step(v[i], st);
+        truth[i] = st;
+    }
+    auto same = [](cx u, cx w) {
+        return (u.real() == w.real() || std::memcmp(&u, &w, 4) == 0) &&
+               (u.imag() == w.imag() || std::memcmp(reinterpret_cast<const float *>(&u) + 1,
+                                                    reinterpret_cast<const float *>(&w) + 1, 4) == 0);
+    };
+    uint32_t met = 0;
+    for (uint32_t t = 0; t < kTrials; ++t) {
+        const uint32_t j0 = first + t * gap;
+        std::vector<cx> z(p, cx(0.f, 0.f));
+        for (uint32_t i = j0; i < j0 + warm; ++i)
+            step(v[i], z);
+        bool eq = true;
+        for (uint32_t k = 0; k < p; ++k)
+            eq &= same(z[k], truth[j0 + warm - 1][k]);
+        met += eq;
+    }
+    return met + 1 >= kTrials;  // at most one miss in 16; the device verifies every segment
 }
 
 }  // namespace
@@ -535,6 +727,10 @@ struct fdsp_iir_s {
     std::mutex mu;
     uint32_t nb = 0, nfb = 0, p = 1, warm = kMinWarm;
     bool real_coeffs = true;
+    int mode = FDSP_IIR_AUTO;  // FDSP_IIR_AUTO, FDSP_IIR_EXACT or FDSP_IIR_FAST
+    bool merges = true;        // whether verified segments are expected to hold
+    bool decays = true;        // whether `warm` reaches the 2^-44 decay (else it is capped)
+    uint32_t warm_fast = kMinWarm;  // the unverified segments' warm-up (2^-26 decay)
     std::vector<float2> b, a;  // normalised taps; a[1..p] (zero past nfb)
     DevBuf d_b, d_xh, d_yh, d_rs, d_re, d_bad, d_ctl, d_in, d_out;
     hipStream_t stream = nullptr;
@@ -578,7 +774,10 @@ extern "C" int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 
         h->real_coeffs &= v.y == 0.f;
     for (const float2 &v : h->a)
         h->real_coeffs &= v.y == 0.f;
-    h->warm = warm_for(h->a, h->p);
+    h->warm = warm_for(h->a, h->p, &h->decays);
+    bool unused;
+    h->warm_fast = warm_for(h->a, h->p, &unused, 0x1p-26);
+    h->merges = segments_merge(h->b, h->a, h->p, h->warm);
     int rc = h->d_b.ensure(nb * sizeof(float2));
     if (!rc)
         rc = h->d_xh.ensure(std::max<uint32_t>(nb - 1, 1) * sizeof(float2));
@@ -625,14 +824,46 @@ extern "C" int fdsp_iir_reset(fdsp_iir_t h)
     return FDSP_OK;
 }
 
-extern "C" int fdsp_iir_info(fdsp_iir_t h, uint32_t *warm, uint32_t *order)
+// The plan a call with outputs of layout tout runs. Exact: verified segments
+// when they merge, else one sequential wave. Fast: segments unverified.
+// Auto: verified segments when they merge; otherwise float outputs take the
+// unverified segments (float rounding, like the library's other float
+// kernels) if the warm-up reaches its decay target, and int16 outputs (or a
+// filter that does not decay) the sequential wave.
+static uint32_t plan_of(const fdsp_iir_s *h, uint32_t tout)
+{
+    if (h->mode == FDSP_IIR_FAST)
+        return FDSP_IIR_PLAN_FAST;
+    if (h->merges)
+        return FDSP_IIR_PLAN_SEGMENTS;
+    const bool float_out = tout == kSampleCf32 || tout == kSampleF32;
+    if (h->mode == FDSP_IIR_AUTO && float_out && h->decays)
+        return FDSP_IIR_PLAN_FAST;
+    return FDSP_IIR_PLAN_SERIAL;
+}
+
+extern "C" int fdsp_iir_info(fdsp_iir_t h, uint32_t *warm, uint32_t *order, uint32_t *plan)
 {
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
     if (warm)
         *warm = h->warm;
     if (order)
         *order = h->nfb;
+    if (plan)
+        *plan = plan_of(h, kSampleCf32);
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir_set_mode(fdsp_iir_t h, int mode)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (mode != FDSP_IIR_FAST && mode != FDSP_IIR_EXACT && mode != FDSP_IIR_AUTO)
+        return set_err(FDSP_EINVAL, "iir_set_mode: FDSP_IIR_AUTO, FDSP_IIR_EXACT or FDSP_IIR_FAST");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->mode = mode;
     return FDSP_OK;
 }
 
@@ -647,7 +878,12 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
         return set_err(FDSP_EINVAL, "null buffers");
     if (int rc = check_chunk(n, 1))
         return rc;
-    const size_t nseg = (n + kSeg - 1) / kSeg, nblk = (n + kBlockOut - 1) / kBlockOut;
+    // outputs per lane: short segments give short chunks enough waves, long
+    // ones keep the warm-up overhead down on long chunks
+    uint32_t seg = 8;
+    while (seg < kMaxSeg && (size_t)seg * 2 * 65536 <= n)
+        seg *= 2;
+    const size_t nseg = (n + seg - 1) / seg, nblk = (n + kLanes * seg - 1) / (kLanes * seg);
     if (int rc = h->d_rs.ensure(nseg * h->p * sizeof(float2)))
         return rc;
     if (int rc = h->d_re.ensure(nseg * h->p * sizeof(float2)))
@@ -667,20 +903,28 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     g.n = (uint32_t)n;
     g.nb = h->nb;
     g.warm = h->warm;
+    g.seg = seg;
     g.tout = tout;
+    const uint32_t plan = plan_of(h, tout);
+    g.verify = plan == FDSP_IIR_PLAN_FAST ? 0u : 1u;
+    if (plan == FDSP_IIR_PLAN_FAST)
+        g.warm = h->warm_fast;
+    const bool serial = plan == FDSP_IIR_PLAN_SERIAL;
+    g.split = h->real_coeffs ? 1u : 0u;
     const float2 *a = h->a.data();
     const bool real_in = tin == kSampleF32 || tin == kSampleI16;
     hipError_t e;
     if (h->real_coeffs && real_in)
-        e = tin == kSampleF32 ? launch_c<false, kSampleF32>(g, h->p, a, s) : launch_c<false, kSampleI16>(g, h->p, a, s);
+        e = tin == kSampleF32 ? launch_c<false, kSampleF32>(g, h->p, a, serial, s)
+                              : launch_c<false, kSampleI16>(g, h->p, a, serial, s);
     else if (tin == kSampleCf32)
-        e = launch_c<true, kSampleCf32>(g, h->p, a, s);
+        e = launch_c<true, kSampleCf32>(g, h->p, a, serial, s);
     else if (tin == kSampleCi16)
-        e = launch_c<true, kSampleCi16>(g, h->p, a, s);
+        e = launch_c<true, kSampleCi16>(g, h->p, a, serial, s);
     else if (tin == kSampleF32)
-        e = launch_c<true, kSampleF32>(g, h->p, a, s);
+        e = launch_c<true, kSampleF32>(g, h->p, a, serial, s);
     else
-        e = launch_c<true, kSampleI16>(g, h->p, a, s);
+        e = launch_c<true, kSampleI16>(g, h->p, a, serial, s);
     HIP_TRY(e);
     return FDSP_OK;
 }

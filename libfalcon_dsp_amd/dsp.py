@@ -424,10 +424,12 @@ class IirFilter(_Handle):
     (falcon_dsp_iir_filter; reference hdr/transform/falcon_dsp_iir_filter.h:
     97-130, src/transform/falcon_dsp_iir_filter.cc:104-396). When a[0]'s real
     part is not 1 all coefficients are divided by a[0], as the reference does.
-    The four apply forms share the stream state; outputs are the reference's
-    bit for bit (up to the sign of zero). int16 outputs are truncated toward
-    zero and saturated. Limits: 1 <= len(b) <= 256, 1 <= len(a) <= 17,
-    a[0] != 0."""
+    The four apply forms share the stream state. int16 outputs are the
+    reference's bit for bit; float outputs are too whenever the filter's
+    parallel segments verify (first-order filters), else within float
+    rounding unless set_exact(True) asks for the sequential exact form.
+    int16 outputs are truncated toward zero and saturated. Limits:
+    1 <= len(b) <= 256, 1 <= len(a) <= 17, a[0] != 0."""
     _destroy = "fdsp_iir_destroy"
 
     def __init__(self, b, a):
@@ -439,12 +441,30 @@ class IirFilter(_Handle):
     def reset_state(self):
         check(lib().fdsp_iir_reset(self.h))
 
+    PLANS = {0: "fast", 1: "segments", 2: "serial"}
+
     @property
     def warm(self) -> int:
         """samples each segment's recurrence runs before its outputs"""
         w = C.c_uint32()
-        check(lib().fdsp_iir_info(self.h, C.byref(w), None))
+        check(lib().fdsp_iir_info(self.h, C.byref(w), None, None))
         return w.value
+
+    @property
+    def plan(self) -> str:
+        """how a call with complex float outputs runs: "segments" (parallel,
+        verified exact), "serial" (one sequential wave, exact) or "fast"
+        (parallel, float rounding)"""
+        p = C.c_uint32()
+        check(lib().fdsp_iir_info(self.h, None, None, C.byref(p)))
+        return self.PLANS[p.value]
+
+    def set_exact(self, exact: Optional[bool] = None):
+        """None (default): exact where parallel segments verify, else float
+        rounding for float outputs and exact for int16 ones; True: always the
+        reference's outputs bit for bit; False: always the unverified parallel
+        segments"""
+        check(lib().fdsp_iir_set_mode(self.h, 2 if exact is None else 1 if exact else 0))
 
     def apply(self, x) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.complex64)

@@ -11,6 +11,7 @@
 #include "fdsp.h"
 #include "transform/falcon_dsp_fir_filter_cuda.h"
 #include "transform/falcon_dsp_freq_shift_cuda.h"
+#include "transform/falcon_dsp_iir_filter.h"
 #include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
 #include "transform/falcon_dsp_polyphase_resampler_cuda.h"
@@ -431,6 +432,105 @@ bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::
 {
     falcon_dsp_fir_filter_cuda f(coeffs);
     return f.apply(in, out);
+}
+
+// ---------------------------------------------------------------------------
+// IIR filter (reference src/transform/falcon_dsp_iir_filter.cc:104-396)
+// ---------------------------------------------------------------------------
+falcon_dsp_iir_filter::falcon_dsp_iir_filter(void) : m_initialized(false), m_handle(nullptr) {}
+
+falcon_dsp_iir_filter::~falcon_dsp_iir_filter(void) { fdsp_iir_destroy(m_handle); }
+
+bool falcon_dsp_iir_filter::initialize(std::vector<std::complex<float>> &b_coeffs,
+                                       std::vector<std::complex<float>> &a_coeffs)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_initialized)
+        return false;
+    if (!b_coeffs.empty() && !a_coeffs.empty() &&
+        fdsp_iir_create(cf(b_coeffs.data()), (uint32_t)b_coeffs.size(), cf(a_coeffs.data()),
+                        (uint32_t)a_coeffs.size(), &m_handle) != FDSP_OK)
+        return false;
+    m_initialized = true;
+    return true;
+}
+
+void falcon_dsp_iir_filter::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_iir_reset(m_handle);
+}
+
+bool falcon_dsp_iir_filter::apply(std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    if (fdsp_iir_apply(m_handle, cf(in.data()), in.size(), cf(out.data())) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+bool falcon_dsp_iir_filter::apply(std::vector<float>& in, std::vector<float>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    if (fdsp_iir_apply_f32(m_handle, in.data(), in.size(), out.data()) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+bool falcon_dsp_iir_filter::apply(std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    if (fdsp_iir_apply_ci16(m_handle, ci(in.data()), in.size(), ci(out.data())) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+bool falcon_dsp_iir_filter::apply(std::vector<int16_t>& in, std::vector<int16_t>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    out.resize(in.size());
+    if (fdsp_iir_apply_i16(m_handle, in.data(), in.size(), out.data()) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    return true;
+}
+
+// reference falcon_dsp_iir_filter.cc:85-100: a fresh filter over in
+bool iir_filter(std::vector<std::complex<float>> &b_coeffs, std::vector<std::complex<float>> &a_coeffs,
+                std::vector<std::complex<int16_t>>& in, std::vector<std::complex<int16_t>>& out)
+{
+    falcon_dsp_iir_filter f;
+    return f.initialize(b_coeffs, a_coeffs) && f.apply(in, out);
+}
+
+bool iir_filter(std::vector<std::complex<float>> &b_coeffs, std::vector<std::complex<float>> &a_coeffs,
+                std::vector<std::complex<float>>& in, std::vector<std::complex<float>>& out)
+{
+    falcon_dsp_iir_filter f;
+    return f.initialize(b_coeffs, a_coeffs) && f.apply(in, out);
 }
 
 // ---------------------------------------------------------------------------

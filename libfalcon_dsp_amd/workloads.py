@@ -13,6 +13,10 @@ chan_32         2^24 samples at 20 Msps: CWGN (seed 2) + 32 unit tones, one
                 of 32*up (beta 8, cutoff 0.5/max(up, down), gain up).
 fir_bank_256    256 channels x 2^20 CWGN (seed 3); 512 real Kaiser-sinc taps
                 per channel, cutoff U[0.05, 0.45].
+iir_order4      2^24 samples CWGN (seed 6), complex float, through a 4th-order
+                Butterworth low-pass at 0.1 cycles/sample (bilinear
+                transform, b and a as scipy.signal.butter gives them); 16
+                chunks of 2^20.
 serving         Poisson arrivals (2000/s) of single-channel streams at
                 25 Msps drawn as in chan_32 with the shift on nco_16shift's
                 +-10 MHz grid; chunks U[4k, 256k]; a share of the streams
@@ -84,6 +88,20 @@ class Nco16:
         x = self.input()
         v = np.stack([np.round(x.real * scale), np.round(x.imag * scale)], axis=1)
         return v.clip(-32768, 32767).astype(np.int16)
+
+
+@dataclass
+class IirOrder4:
+    n: int = 1 << 24
+    chunk: int = 1 << 20
+
+    def coeffs(self):
+        from scipy.signal import butter
+        b, a = butter(4, 0.2)  # cutoff 0.2 of Nyquist = 0.1 cycles/sample
+        return b.astype(np.complex64), a.astype(np.complex64)
+
+    def input(self) -> np.ndarray:
+        return cwgn(6, self.n)
 
 
 @dataclass

@@ -13,6 +13,7 @@
 
 #include "transform/falcon_dsp_fir_filter_cuda.h"
 #include "transform/falcon_dsp_freq_shift_cuda.h"
+#include "transform/falcon_dsp_iir_filter.h"
 #include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
 #include "transform/falcon_dsp_polyphase_resampler_cuda.h"
@@ -352,6 +353,61 @@ static void test_ci16()
     }
 }
 
+// reference test/tests/transform/falcon_dsp_iir_filter_cpp_unit_tests.cc:147-260
+static void test_iir()
+{
+    using ci = std::complex<int16_t>;
+    {
+        std::vector<cf> b = {{1.0f, 0}}, a = {{0.5f, 0}, {1.0f, 0}};
+        std::vector<ci> x = {{0, 0}, {1, 0}, {1, 0}, {1, 0}, {1, 0}}, y = {{9, 9}};
+        const int16_t want[] = {0, 2, -2, 6, -10};
+        falcon_dsp_iir_filter f;
+        bool ok = f.initialize(b, a) && !f.initialize(b, a) && f.apply(x, y) && y.size() == 5;
+        for (int i = 0; ok && i < 5; ++i)
+            ok = y[i] == ci(want[i], 0);
+        check(ok, "iir uint16_t_000: a[0] = 0.5 divides through; int16 out; initialize twice is false");
+    }
+    struct Kat {
+        std::vector<cf> b, a, x;
+        std::vector<float> want;
+    } kats[] = {
+        {{{1.0f, 0}}, {{0.5f, 0}, {1.0f, 0}}, {{0.25f, 0}, {1, 0}, {1, 0}, {1, 0}, {1, 0}}, {0.5f, 1, 0, 2, -2}},
+        {{{1.0f, 0}}, {{1.0f, 0}, {0.9f, 0}, {0.8f, 0}}, {{0, 0}, {1, 0}, {1, 0}, {1, 0}, {1, 0}},
+         {0, 1, 0.1f, 0.11f, 0.821f}},
+        {{{0.3f, 0}, {0.16f, 0}, {0.21f, 0}}, {{1.0f, 0}}, {{0, 0}, {10, 0}, {15, 0}, {11, 0}, {13, 0}},
+         {0, 3, 6.1f, 7.8f, 8.81f}},
+    };
+    bool ok = true;
+    for (auto &k : kats) {
+        std::vector<cf> y;
+        ok = ok && iir_filter(k.b, k.a, k.x, y) && y.size() == 5;
+        for (int i = 0; ok && i < 5; ++i)
+            ok = std::fabs(y[i].real() - k.want[i]) <= 1e-4f && std::fabs(y[i].imag()) <= 1e-4f;
+    }
+    check(ok, "iir float_000/001/002 known answers (free function)");
+    {
+        // chunked real and complex calls share the state; empty coefficients
+        // initialise but never filter
+        std::vector<cf> b = {{0.2f, 0}, {0.1f, 0}}, a = {{1.0f, 0}, {-0.7f, 0}};
+        auto xn = noise(5, 3000);
+        std::vector<cf> whole, part, y;
+        falcon_dsp_iir_filter f, g;
+        f.initialize(b, a);
+        g.initialize(b, a);
+        bool ok2 = f.apply(xn, whole);
+        for (size_t i = 0; ok2 && i < xn.size(); i += 1000) {
+            std::vector<cf> c(xn.begin() + i, xn.begin() + i + 1000);
+            ok2 = g.apply(c, y);
+            part.insert(part.end(), y.begin(), y.end());
+        }
+        ok2 = ok2 && part == whole;
+        std::vector<cf> e;
+        falcon_dsp_iir_filter h;
+        ok2 = ok2 && h.initialize(e, a) && !h.apply(xn, y) && y.empty();
+        check(ok2, "iir chunked equals whole; empty b initialises, apply is false");
+    }
+}
+
 int main()
 {
     test_utils();
@@ -359,6 +415,7 @@ int main()
     test_fir_and_discriminator();
     test_freq_shift();
     test_ci16();
+    test_iir();
     test_channelizer();
     std::printf("%s: %d failed\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
