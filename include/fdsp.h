@@ -251,6 +251,23 @@ int fdsp_iir1_reset(fdsp_iir1_t h);
 int fdsp_iir1_apply(fdsp_iir1_t h, const float *in, size_t n, float *out);
 int fdsp_iir1_apply_device(fdsp_iir1_t h, const float *d_in, size_t n, float *d_out, fdsp_stream_t stream);
 
+/* ---- predefined resampling filters ----
+ * Replaces get_resample_fir_params (reference src/transform/
+ * falcon_dsp_fir_filter.cc:111-130) and its table (src/transform/
+ * falcon_dsp_predefined_fir_filter.cc, keys as in hdr/transform/
+ * falcon_dsp_predefined_fir_filter.h:40-110). The filters are designed at build
+ * time by the reference's documented procedure (firls with a Kaiser window,
+ * or remez) for 1,140,000 -> 228,000 / 200,000, 1,000,000 -> 200,000,
+ * 228,000 -> 45,600 and 200,000 -> 45,000 / 44,000 sps. taps: 32, 64, 128,
+ * 256 or FDSP_FILTER_TAPS_OPTIMAL; source: FDSP_FILTER_REMEZ or
+ * FDSP_FILTER_FIRLS. FDSP_EINVAL when no such filter exists (the reference
+ * returns false); FDSP_ERANGE, with *n_coeffs set, when cap is too small. */
+#define FDSP_FILTER_REMEZ 0
+#define FDSP_FILTER_FIRLS 1
+#define FDSP_FILTER_TAPS_OPTIMAL -1
+int fdsp_resample_fir_params(uint32_t in_rate, uint32_t out_rate, int32_t taps, uint32_t source, uint32_t *up,
+                             uint32_t *down, fdsp_cf32 *coeffs, size_t cap, size_t *n_coeffs);
+
 /* ---- FM mono demodulator ----
  * Replaces falcon_dsp_fm_demodulator::{initialize, demod_mono, reset_state}
  * (reference src/signals/falcon_dsp_fm_demod.cc:98-271): shift by the signal's

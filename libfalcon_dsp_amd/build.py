@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = [os.path.join(CSRC, f)
                for f in ("fdsp_kernels.hip", "fdsp_chan_groups.hip", "fdsp_capi.hip", "fdsp_serve.hip", "fdsp_fm.hip",
-                         "fdsp_iir.hip")]
+                         "fdsp_iir.hip", "fdsp_predefined.hip")]
+PREDEFINED_INC = os.path.join(CSRC, "fdsp_predefined_table.inc")
 HIP_LIB = os.path.join(HERE, "libfdsp_hip.so")
 DROPIN_SOURCES_DIR = os.path.join(HERE, "src")
 DROPIN_LIB = os.path.join(HERE, "libfalcon_dsp.so")
@@ -40,7 +41,15 @@ def _run(cmd):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
+def build_predefined_table():
+    """The predefined resampling filters, designed by libfalcon_dsp_amd/predefined.py."""
+    from . import predefined
+    n = predefined.write_include(PREDEFINED_INC)
+    print(f"+ designed {n} predefined filters -> {os.path.relpath(PREDEFINED_INC, ROOT)}", flush=True)
+
+
 def build_hip():
+    build_predefined_table()
     _run([HIPCC, *HIP_FLAGS, *HIP_SOURCES, "-o", HIP_LIB])
 
 

@@ -538,18 +538,40 @@ def fm_deemphasis(tau: float):
     return b0.value, a1.value
 
 
+FILTER_REMEZ, FILTER_FIRLS, FILTER_TAPS_OPTIMAL = 0, 1, -1
+
+
+def resample_fir_params(in_rate: int, out_rate: int, taps: int = 64, source: int = FILTER_REMEZ):
+    """The predefined anti-alias filter for in_rate -> out_rate sps
+    (get_resample_fir_params; reference src/transform/falcon_dsp_fir_filter.cc:
+    111-130): (up, down, complex64 taps). taps: 32, 64, 128, 256 or
+    FILTER_TAPS_OPTIMAL; source: FILTER_REMEZ or FILTER_FIRLS. Raises
+    FdspError when the table has no such filter."""
+    up, down, n = C.c_uint32(), C.c_uint32(), C.c_size_t()
+    buf = np.empty(2048, np.complex64)
+    check(lib().fdsp_resample_fir_params(int(in_rate), int(out_rate), int(taps), int(source), C.byref(up),
+                                         C.byref(down), _ptr(buf), buf.size, C.byref(n)))
+    return up.value, down.value, buf[:n.value].copy()
+
+
 class FmDemodulator(_Handle):
     """falcon_dsp_fm_demodulator::demod_mono (reference src/signals/falcon_dsp_fm_demod.cc:98-271):
     shift, resample to the FM band, discriminate, de-emphasise, resample to
     audio, scale to a peak of 10000 and convert to int16. iso and mono are
-    the two resamplers' (up, down, taps), which the reference takes from its
-    predefined table."""
+    the two resamplers' (up, down, taps); by default the predefined 64-tap
+    REMEZ filters for fs -> 228,000 and 228,000 -> 45,600 sps, as the
+    reference takes them."""
     _destroy = "fdsp_fm_demod_destroy"
     AMERICAS_TAU = 75e-6
     EUROPE_ASIA_TAU = 50e-6
+    FM_RATE, AUDIO_RATE = 228_000, 45_600
 
-    def __init__(self, fs: int, offset_hz: int, iso, mono, tau: float = AMERICAS_TAU):
+    def __init__(self, fs: int, offset_hz: int, iso=None, mono=None, tau: float = AMERICAS_TAU):
         super().__init__()
+        if iso is None:
+            iso = resample_fir_params(fs, self.FM_RATE)
+        if mono is None:
+            mono = resample_fir_params(self.FM_RATE, self.AUDIO_RATE)
         self._iso_t, self._mono_t = _cf32(iso[2]), _cf32(mono[2])
         check(lib().fdsp_fm_demod_create(fs, int(offset_hz), float(tau), iso[0], iso[1], _ptr(self._iso_t),
                                          self._iso_t.size, mono[0], mono[1], _ptr(self._mono_t), self._mono_t.size,

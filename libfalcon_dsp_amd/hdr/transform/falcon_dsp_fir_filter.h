@@ -1,11 +1,14 @@
 // falcon_dsp_fir_filter.h -- streaming complex FIR, drop-in API.
-// (Same class and signatures as reference hdr/transform/falcon_dsp_fir_filter.h:87-140.)
+// (Same class and signatures as reference hdr/transform/falcon_dsp_fir_filter.h:87-140;
+// get_resample_fir_params as .h:95-108.)
 #pragma once
 
 #include <complex>
 #include <cstdint>
 #include <mutex>
 #include <vector>
+
+#include "transform/falcon_dsp_predefined_fir_filter.h"
 
 struct fdsp_fir_s;
 
@@ -15,6 +18,13 @@ namespace falcon_dsp
                     std::vector<std::complex<float>>& out);
     bool fir_filter(std::vector<std::complex<float>>& coeffs, std::vector<std::complex<int16_t>>& in,
                     std::vector<std::complex<int16_t>>& out);
+
+    // the predefined anti-alias filter for in -> out sps with the given tap
+    // count and design; false (coeffs empty) when there is none
+    bool get_resample_fir_params(uint32_t input_sample_rate, uint32_t output_sample_rate,
+                                 filter_taps_e taps, filter_source_type_e source,
+                                 uint32_t& up_rate, uint32_t& down_rate,
+                                 std::vector<std::complex<float>> &coeffs);
 
     class falcon_dsp_fir_filter
     {

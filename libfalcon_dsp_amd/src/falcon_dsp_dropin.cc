@@ -4,6 +4,7 @@
 // the kernels run, outputs come back before the call returns. Errors surface
 // as the reference's bool / count returns; the C ABI's fdsp_last_error()
 // holds the reason.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -12,6 +13,7 @@
 #include "transform/falcon_dsp_fir_filter_cuda.h"
 #include "transform/falcon_dsp_freq_shift_cuda.h"
 #include "transform/falcon_dsp_iir_filter.h"
+#include "signals/falcon_dsp_fm_demod.h"
 #include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
 #include "transform/falcon_dsp_polyphase_resampler_cuda.h"
@@ -432,6 +434,89 @@ bool fir_filter_cuda(std::vector<std::complex<float>>& coeffs, std::vector<std::
 {
     falcon_dsp_fir_filter_cuda f(coeffs);
     return f.apply(in, out);
+}
+
+// ---------------------------------------------------------------------------
+// predefined resampling filters (reference src/transform/falcon_dsp_fir_filter.cc:111-130)
+// ---------------------------------------------------------------------------
+bool get_resample_fir_params(uint32_t input_sample_rate, uint32_t output_sample_rate, filter_taps_e taps,
+                             filter_source_type_e source, uint32_t& up_rate, uint32_t& down_rate,
+                             std::vector<std::complex<float>> &coeffs)
+{
+    coeffs.clear();
+    uint32_t up = 0, down = 0;
+    size_t n = 0;
+    std::vector<std::complex<float>> c(512);
+    if (fdsp_resample_fir_params(input_sample_rate, output_sample_rate, static_cast<int32_t>(taps),
+                                 static_cast<uint32_t>(source), &up, &down, cf(c.data()), c.size(), &n) != FDSP_OK)
+        return false;
+    c.resize(n);
+    up_rate = up;
+    down_rate = down;
+    coeffs = c;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// FM mono demodulator (reference src/signals/falcon_dsp_fm_demod.cc:98-271)
+// ---------------------------------------------------------------------------
+falcon_dsp_fm_demodulator::falcon_dsp_fm_demodulator(void) : m_initialized(false), m_handle(nullptr) {}
+
+falcon_dsp_fm_demodulator::~falcon_dsp_fm_demodulator(void) { fdsp_fm_demod_destroy(m_handle); }
+
+bool falcon_dsp_fm_demodulator::initialize(uint32_t input_sample_rate_in_sps, int32_t signal_offset_from_dc_in_hz,
+                                           float deemphasis_time_constant_in_usecs)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_initialized)
+        return false;
+    // the two supported time constants, as the exact values the de-emphasis
+    // coefficients are defined from
+    double tau;
+    if (deemphasis_time_constant_in_usecs == AMERICAS_FM_DEEMPHASIS_TIME_CONSTANT)
+        tau = 75e-6;
+    else if (deemphasis_time_constant_in_usecs == EUROPE_ASIA_FM_DEEMPHASIS_TIME_CONSTANT)
+        tau = 50e-6;
+    else
+        return false;
+    uint32_t iso_up, iso_down, mono_up, mono_down;
+    std::vector<std::complex<float>> iso, mono;
+    if (!get_resample_fir_params(input_sample_rate_in_sps, FM_RADIO_SAMPLE_RATE_IN_SPS, filter_taps_e::FILTER_TAPS_64,
+                                 filter_source_type_e::REMEZ, iso_up, iso_down, iso) ||
+        !get_resample_fir_params(FM_RADIO_SAMPLE_RATE_IN_SPS, FM_RADIO_AUDIO_SAMPLE_RATE_IN_SPS,
+                                 filter_taps_e::FILTER_TAPS_64, filter_source_type_e::REMEZ, mono_up, mono_down, mono))
+        return false;
+    if (fdsp_fm_demod_create(input_sample_rate_in_sps, signal_offset_from_dc_in_hz, tau, iso_up, iso_down,
+                             cf(iso.data()), (uint32_t)iso.size(), mono_up, mono_down, cf(mono.data()),
+                             (uint32_t)mono.size(), &m_handle) != FDSP_OK)
+        return false;
+    m_initialized = true;
+    return true;
+}
+
+void falcon_dsp_fm_demodulator::reset_state(void)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    if (m_handle)
+        fdsp_fm_demod_reset(m_handle);
+}
+
+bool falcon_dsp_fm_demodulator::demod_mono(std::vector<std::complex<float>>& in, std::vector<int16_t>& out)
+{
+    std::lock_guard<std::mutex> lock(m_mutex);
+    out.clear();
+    if (!m_handle || in.empty())
+        return false;
+    size_t most = 0, got = 0;
+    if (fdsp_fm_demod_out_count(m_handle, in.size(), &most) != FDSP_OK)
+        return false;
+    out.resize(std::max<size_t>(most, 1));
+    if (fdsp_fm_demod_apply(m_handle, cf(in.data()), in.size(), out.data(), nullptr, out.size(), &got) != FDSP_OK) {
+        out.clear();
+        return false;
+    }
+    out.resize(got);
+    return got > 0;
 }
 
 // ---------------------------------------------------------------------------

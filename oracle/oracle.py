@@ -482,6 +482,9 @@ def ref():
         L.ref_resample_fir_params.restype = C.c_int64
         L.ref_resample_fir_params.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                               _f32p, C.c_size_t]
+        L.ref_resample_fir_params_key.restype = C.c_int64
+        L.ref_resample_fir_params_key.argtypes = [C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
+                                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _f32p, C.c_size_t]
         L.ref_fm_create.restype = _vp
         L.ref_fm_create.argtypes = [C.c_uint32, C.c_int32, C.c_int]
         L.ref_fm_demod.restype = C.c_int64
@@ -637,6 +640,18 @@ class RefIir:
         out = np.empty_like(x)
         n = ref().ref_iir_apply_i16(self.h, _iptr(x), x.size, _iptr(out))
         return out[:n]
+
+
+def ref_predefined_filter(in_rate, out_rate, taps, source):
+    """The reference's predefined-table entry (taps: filter_taps_e value, source:
+    0 REMEZ, 1 FIRLS) -> (up, down, complex64 taps), or None when absent."""
+    up, down = C.c_uint32(), C.c_uint32()
+    c = np.empty(4096, dtype=np.complex64)
+    n = ref().ref_resample_fir_params_key(in_rate, out_rate, taps, source, C.byref(up), C.byref(down),
+                                          _fptr(c.view(np.float32)), c.size)
+    if n < 0:
+        return None
+    return up.value, down.value, c[:n].copy()
 
 
 def ref_resample_fir_params(in_rate, out_rate):

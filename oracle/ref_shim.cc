@@ -314,6 +314,18 @@ int64_t ref_resample_fir_params(uint32_t in_rate, uint32_t out_rate, uint32_t *u
         std::memcpy(taps, c.data(), c.size() * sizeof(cf32));
     return (int64_t)c.size();
 }
+/* any table entry: taps is the filter_taps_e value, source 0 REMEZ / 1 FIRLS */
+int64_t ref_resample_fir_params_key(uint32_t in_rate, uint32_t out_rate, int32_t taps, uint32_t source, uint32_t *up,
+                                    uint32_t *down, float *coeffs, size_t cap)
+{
+    std::vector<cf32> c;
+    if (!falcon_dsp::get_resample_fir_params(in_rate, out_rate, static_cast<filter_taps_e>(taps),
+                                             static_cast<filter_source_type_e>(source), *up, *down, c))
+        return -1;
+    if (c.size() <= cap)
+        std::memcpy(coeffs, c.data(), c.size() * sizeof(cf32));
+    return (int64_t)c.size();
+}
 void *ref_fm_create(uint32_t fs, int32_t offset_hz, int europe)
 {
     auto *o = new falcon_dsp::falcon_dsp_fm_demodulator();

@@ -14,6 +14,7 @@
 #include "transform/falcon_dsp_fir_filter_cuda.h"
 #include "transform/falcon_dsp_freq_shift_cuda.h"
 #include "transform/falcon_dsp_iir_filter.h"
+#include "signals/falcon_dsp_fm_demod.h"
 #include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
 #include "transform/falcon_dsp_polyphase_resampler_cuda.h"
@@ -408,6 +409,40 @@ static void test_iir()
     }
 }
 
+// reference src/transform/falcon_dsp_fir_filter.cc:111-130 and
+// src/signals/falcon_dsp_fm_demod.cc:98-271
+static void test_fm_and_predefined()
+{
+    uint32_t up = 0, down = 0;
+    std::vector<cf> c;
+    bool ok = get_resample_fir_params(1140000, 228000, filter_taps_e::FILTER_TAPS_64, filter_source_type_e::REMEZ,
+                                      up, down, c) &&
+              up == 1 && down == 5 && c.size() == 64;
+    ok = ok && !get_resample_fir_params(1140000, 228000, filter_taps_e::FILTER_TAPS_256, filter_source_type_e::REMEZ,
+                                        up, down, c) && c.empty();
+    check(ok, "get_resample_fir_params: 1140000 -> 228000 REMEZ 64 is 1/5 with 64 taps; absent key is false");
+
+    // a 1 kHz tone on an FM carrier 150 kHz above DC
+    const uint32_t fs = 1140000;
+    std::vector<cf> x(200000);
+    double ph = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        ph += 2 * M_PI * (150000.0 + 75000.0 * std::sin(2 * M_PI * 1000.0 * i / fs)) / fs;
+        x[i] = cf((float)std::cos(ph), (float)std::sin(ph));
+    }
+    falcon_dsp_fm_demodulator bad, f;
+    std::vector<int16_t> y, y2;
+    ok = !bad.initialize(fs, -150000, 60e-6f) && f.initialize(fs, -150000) && !f.initialize(fs, -150000);
+    ok = ok && f.demod_mono(x, y) && y.size() == 200000 / 25;
+    int16_t peak = 0;
+    for (int16_t v : y)
+        peak = std::max<int16_t>(peak, (int16_t)std::abs(v));
+    ok = ok && peak >= 9999 && peak <= 10000;
+    f.reset_state();
+    ok = ok && f.demod_mono(x, y2) && y2 == y;
+    check(ok, "fm demodulator: tau check, initialise once, 8000 samples peaking at 10000, reset repeats");
+}
+
 int main()
 {
     test_utils();
@@ -416,6 +451,7 @@ int main()
     test_freq_shift();
     test_ci16();
     test_iir();
+    test_fm_and_predefined();
     test_channelizer();
     std::printf("%s: %d failed\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;

@@ -107,3 +107,15 @@ def test_fm_demod_device_and_errors():
     assert rc == fd._native.FDSP_ERANGE
     a, _ = o2.demod(first)
     assert np.max(np.abs(h2.demod_mono(first).astype(np.int32) - a)) <= 1
+
+
+def test_fm_default_filters_are_the_predefined_ones():
+    """FmDemodulator without filters takes the predefined 64-tap REMEZ pair,
+    as the reference's initialize does; the result equals the fixture's."""
+    iso, mono = fm_taps()
+    x = fm_input(12, 200_000, 150_000)
+    a = fd.FmDemodulator(FS, -150_000)
+    b = fd.FmDemodulator(FS, -150_000, iso, mono)
+    for part in (slice(0, 80_000), slice(80_000, 200_000)):
+        np.testing.assert_array_equal(a.demod_mono(x[part]), b.demod_mono(x[part]))
+
