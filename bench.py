@@ -200,7 +200,26 @@ def bench_other_configs(fd, stream, steps):
                      "unit": "GB/s", "frac": round(b2 / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "note": "2^24 inputs in 64 chunks of 2^18, 16 shifts, 136 B per input sample"}
     log(f"nco_16shift: {ms2:.3f} ms per pass, {out['nco_16shift']['roofline']['achieved']} GB/s")
-    del d2, o2, x2
+    # error of one fresh chunk against the FP64 closed form x * e^{j w (n mod R)} (the reference's
+    # rollover index), and against continuous phase x * e^{j w n}, relative to the input RMS
+    f0 = fd.FreqShift(w2.fs, w2.shifts())
+    f0.apply_device(d2, w2.chunk, o2, w2.chunk, stream)
+    stream.sync()
+    y = o2.download(16 * w2.chunk).reshape(16, w2.chunk).astype(np.complex128)
+    xc = x2[:w2.chunk].astype(np.complex128)
+    rms = np.sqrt(np.mean(np.abs(xc) ** 2))
+    n_idx = np.arange(w2.chunk, dtype=np.float64)
+    e_roll = e_cont = 0.0
+    for c, f_hz in enumerate(w2.shifts()):
+        r_c, om = fd.get_freq_shift_params(w2.fs, f_hz)
+        e_roll = max(e_roll, float(np.max(np.abs(y[c] - xc * np.exp(1j * om * (n_idx % r_c))))))
+        e_cont = max(e_cont, float(np.max(np.abs(y[c] - xc * np.exp(1j * om * n_idx)))))
+    out["nco_16shift"]["fp64_error"] = {"vs_rollover_index": round(e_roll / rms, 9),
+                                        "vs_continuous_phase": round(e_cont / rms, 6),
+                                        "unit": "max abs error / input RMS, first 2^18-sample chunk, 16 shifts"}
+    log(f"nco_16shift error vs FP64: {e_roll / rms:.2e} (rollover index), {e_cont / rms:.2e} (continuous phase)")
+    del f0, x2, y
+    del d2, o2
 
     # FM mono demodulation, 2^24 samples at 1.14 Msps in 2^18-sample chunks
     from libfalcon_dsp_amd.workloads import FmDemod

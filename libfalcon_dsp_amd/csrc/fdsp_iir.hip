@@ -110,13 +110,17 @@ template <uint32_t TIN, bool C> __device__ __forceinline__ Val<C> load_x(const I
     return from2<C>(v);
 }
 
-// acc + x b, the product as std::complex<float> rounds it (a real-coefficient
-// filter on real samples has zero imaginary parts throughout, so the real
-// form gives the same real parts)
-template <bool C> __device__ __forceinline__ Val<C> mac(Val<C> acc, Val<C> x, float2 b)
+// acc + x b, the product as std::complex<float> rounds it. With a real
+// coefficient (RA) the products with its zero imaginary part only add signed
+// zeros, so each part is one real product (the same values for finite
+// samples); a real-coefficient filter on real samples (C false) likewise
+// keeps zero imaginary parts throughout.
+template <bool C, bool RA = false> __device__ __forceinline__ Val<C> mac(Val<C> acc, Val<C> x, float2 b)
 {
 #pragma clang fp contract(off)
-    if constexpr (C) {
+    if constexpr (C && RA) {
+        return make_float2(acc.x + x.x * b.x, acc.y + x.y * b.x);
+    } else if constexpr (C) {
         const float2 p = cmul_ref(x, b);
         return make_float2(acc.x + p.x, acc.y + p.y);
     } else {
@@ -126,13 +130,17 @@ template <bool C> __device__ __forceinline__ Val<C> mac(Val<C> acc, Val<C> x, fl
 
 // one output: v - a1 y[n-1] - a2 y[n-2] - ..., subtracted in that order;
 // st[0] is the newest output
-template <int P, bool C> __device__ __forceinline__ Val<C> fb_step(Val<C> v, Val<C> (&st)[P], const Fb<P> &fb)
+template <int P, bool C, bool RA = false>
+__device__ __forceinline__ Val<C> fb_step(Val<C> v, Val<C> (&st)[P], const Fb<P> &fb)
 {
 #pragma clang fp contract(off)
     Val<C> y = v;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        if constexpr (C) {
+        if constexpr (C && RA) {
+            y.x = y.x - st[k].x * fb.a[k].x;
+            y.y = y.y - st[k].y * fb.a[k].x;
+        } else if constexpr (C) {
             const float2 p = cmul_ref(st[k], fb.a[k]);
             y.x = y.x - p.x;
             y.y = y.y - p.y;
@@ -166,11 +174,11 @@ template <bool C> __device__ __forceinline__ void put_out(const IirLaunch &g, ui
 }
 
 // the b-part of position i >= 0
-template <uint32_t TIN, bool C> __device__ __forceinline__ Val<C> bpart(const IirLaunch &g, int32_t i)
+template <uint32_t TIN, bool C, bool RA = false> __device__ __forceinline__ Val<C> bpart(const IirLaunch &g, int32_t i)
 {
     Val<C> acc = vzero<C>();
     for (uint32_t k = 0; k < g.nb; ++k)
-        acc = mac<C>(acc, load_x<TIN, C>(g, i - (int32_t)k), g.b[k]);
+        acc = mac<C, RA>(acc, load_x<TIN, C>(g, i - (int32_t)k), g.b[k]);
     return acc;
 }
 
@@ -180,7 +188,8 @@ template <uint32_t TIN, bool C> __device__ __forceinline__ Val<C> bpart(const Ii
 // Then rows of 64 positions, in ascending order, replace them: the b-part of
 // q goes to slot q, whose input (position lo + q - nb + 1) no later row
 // reads, and a row's reads all precede its writes (one wave, in order).
-template <uint32_t TIN, bool C> __device__ void stage_bpart(const IirLaunch &g, Val<C> *buf, int32_t lo, uint32_t span)
+template <uint32_t TIN, bool C, bool RA = false>
+__device__ void stage_bpart(const IirLaunch &g, Val<C> *buf, int32_t lo, uint32_t span)
 {
     const uint32_t lane = threadIdx.x, h = g.nb - 1, tot = span + h;
     const int32_t base = lo - (int32_t)h;
@@ -215,7 +224,7 @@ template <uint32_t TIN, bool C> __device__ void stage_bpart(const IirLaunch &g, 
             for (uint32_t r = 0; r < kR; ++r) {
                 const uint32_t q = q0 + r * kLanes + lane;
                 if (q < span)
-                    acc[r] = mac<C>(acc[r], buf[slot<C>(q + h - k)], bk);
+                    acc[r] = mac<C, RA>(acc[r], buf[slot<C>(q + h - k)], bk);
             }
         }
 #pragma unroll
@@ -291,7 +300,7 @@ template <uint32_t TIN> __device__ void carry_inputs(const IirLaunch &g)
 // outputs, records and failure bits are visible): check the block
 // boundaries, repair failed segments in order, then carry the stream state to
 // the next call.
-template <int P, bool C, uint32_t TIN>
+template <int P, bool C, uint32_t TIN, bool RA>
 __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> fb)
 {
     using V = Val<C>;
@@ -332,7 +341,7 @@ __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> f
         // repair the successor is checked against the new end state directly.
         auto repair = [&](uint32_t f) {
             const uint32_t p0 = f * S, cnt = min(S, g.n - p0);
-            stash[lane] = lane < cnt ? bpart<TIN, C>(g, (int32_t)(p0 + lane)) : vzero<C>();
+            stash[lane] = lane < cnt ? bpart<TIN, C, RA>(g, (int32_t)(p0 + lane)) : vzero<C>();
             __syncthreads();
             if (lane == 0) {
                 V st[P];
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> f
                 for (int k = 0; k < P; ++k)
                     st[k] = fixed_seg == (int32_t)f - 1 ? fixed_end[k] : from2<C>(g.re[(size_t)(f - 1) * P + k]);
                 for (uint32_t i = 0; i < cnt; ++i)
-                    stash[i] = fb_step<P, C>(stash[i], st, fb);
+                    stash[i] = fb_step<P, C, RA>(stash[i], st, fb);
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     fixed_end[k] = st[k];
@@ -399,7 +408,7 @@ __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> f
         atomicExch(&g.ctl[1], 0u);
 }
 
-template <int P, bool C, uint32_t TIN>
+template <int P, bool C, uint32_t TIN, bool RA>
 __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
 {
     using V = Val<C>;
@@ -415,7 +424,7 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
     const int32_t lo = (int32_t)first - (int32_t)g.warm;  // vs slot(q) holds position lo + q
 
     // 1. b-part of every position the block's recurrences read
-    stage_bpart<TIN, C>(g, vs, lo, span);
+    stage_bpart<TIN, C, RA>(g, vs, lo, span);
 
     // 2. each lane's recurrence. The segment's outputs overwrite its own
     // b-part slots. Other lanes read those slots only in their warm-up loop,
@@ -434,7 +443,7 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
             lds_read<C>(vs + slot<C>((uint32_t)((int32_t)j - lo)), v);
 #pragma unroll
             for (int u = 0; u < kB; ++u)
-                (void)fb_step<P, C>(v[u], st, fb);
+                (void)fb_step<P, C, RA>(v[u], st, fb);
         }
 #pragma unroll
         for (int k = 0; k < P; ++k)
@@ -445,12 +454,12 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
             lds_read<C>(p, v);
 #pragma unroll
             for (int u = 0; u < kB; ++u)
-                v[u] = fb_step<P, C>(v[u], st, fb);
+                v[u] = fb_step<P, C, RA>(v[u], st, fb);
             lds_write<C>(p, v);
         }
         for (; j < s1; ++j) {
             V *p = vs + slot<C>((uint32_t)((int32_t)j - lo));
-            *p = fb_step<P, C>(*p, st, fb);
+            *p = fb_step<P, C, RA>(*p, st, fb);
         }
         // records: every segment's when verifying, else the last one's (the carried state)
         const size_t seg = s0 / S;
@@ -570,7 +579,7 @@ __global__ __launch_bounds__(kLanes) void iir_serial_kernel(IirLaunch g, Fb<P> f
     carry_inputs<TIN>(g);
 }
 
-template <int P, bool C, uint32_t TIN>
+template <int P, bool C, uint32_t TIN, bool RA>
 hipError_t launch_p(const IirLaunch &g, const float2 *a, bool serial, hipStream_t s)
 {
     Fb<P> fb;
@@ -582,26 +591,36 @@ hipError_t launch_p(const IirLaunch &g, const float2 *a, bool serial, hipStream_
     }
     const uint32_t blocks = (g.n + kLanes * g.seg - 1) / (kLanes * g.seg);
     const size_t lds = slots<C>(kLanes * g.seg + g.warm + g.nb - 1) * sizeof(Val<C>);
-    hipLaunchKernelGGL((iir_kernel<P, C, TIN>), dim3(blocks), dim3(kLanes), lds, s, g, fb);
-    hipLaunchKernelGGL((iir_finish_kernel<P, C, TIN>), dim3(1), dim3(kLanes), 0, s, g, fb);
+    hipLaunchKernelGGL((iir_kernel<P, C, TIN, RA>), dim3(blocks), dim3(kLanes), lds, s, g, fb);
+    hipLaunchKernelGGL((iir_finish_kernel<P, C, TIN, RA>), dim3(1), dim3(kLanes), 0, s, g, fb);
     return hipGetLastError();
 }
 
-template <bool C, uint32_t TIN>
-hipError_t launch_c(const IirLaunch &g, uint32_t p, const float2 *a, bool serial, hipStream_t s)
+template <bool C, uint32_t TIN, bool RA>
+hipError_t launch_pa(const IirLaunch &g, uint32_t p, const float2 *a, bool serial, hipStream_t s)
 {
     switch (p) {
     case 1:
-        return launch_p<1, C, TIN>(g, a, serial, s);
+        return launch_p<1, C, TIN, RA>(g, a, serial, s);
     case 2:
-        return launch_p<2, C, TIN>(g, a, serial, s);
+        return launch_p<2, C, TIN, RA>(g, a, serial, s);
     case 4:
-        return launch_p<4, C, TIN>(g, a, serial, s);
+        return launch_p<4, C, TIN, RA>(g, a, serial, s);
     case 8:
-        return launch_p<8, C, TIN>(g, a, serial, s);
+        return launch_p<8, C, TIN, RA>(g, a, serial, s);
     default:
-        return launch_p<16, C, TIN>(g, a, serial, s);
+        return launch_p<16, C, TIN, RA>(g, a, serial, s);
     }
+}
+
+// complex samples with real coefficients take the two-real-recurrence form
+template <bool C, uint32_t TIN>
+hipError_t launch_c(const IirLaunch &g, uint32_t p, const float2 *a, bool serial, hipStream_t s)
+{
+    if constexpr (C)
+        if (g.split)
+            return launch_pa<C, TIN, true>(g, p, a, serial, s);
+    return launch_pa<C, TIN, false>(g, p, a, serial, s);
 }
 
 // the reference's std::complex<float> division (g++: Smith's method in
