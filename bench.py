@@ -147,6 +147,64 @@ def iir_cpu_baseline():
             "sample": "first 2^20 samples of iir_order4, one thread"}
 
 
+def other_cpu_baselines():
+    """The reference's own CPU classes (oracle/_ref; else the oracle
+    restatement) on bounded samples of C1, C2 and C4, in each line's unit:
+    one object per thread, as the reference's objects are single-threaded."""
+    from libfalcon_dsp_amd.workloads import FirBank256, Nco16, Resampler1ch
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    res = {}
+
+    w1 = Resampler1ch()
+    x1 = w1.input()
+    r = (O.RefResampler if kind == "reference" else O.Resampler)(w1.up, w1.down, w1.taps())
+    n_out, t0 = 0, time.perf_counter()
+    for k in range(4):
+        n_out += r.apply(x1[k * w1.chunk:(k + 1) * w1.chunk]).size
+    dt = time.perf_counter() - t0
+    res["resampler_1ch"] = {"value": round(n_out / dt / 1e6, 2), "unit": "output Msps", "cores": 1, "kind": kind,
+                            "sample": "first 4 of the 16 chunks of 2^16, one stream"}
+
+    w2 = Nco16()
+    x2 = w2.input()[:1 << 20]
+    make = O.RefFreqShift if kind == "reference" else O.FreqShift
+    objs = [make(w2.fs, f) for f in w2.shifts()]
+    t0 = time.perf_counter()
+    objs[0].apply(x2)
+    single = x2.size / (time.perf_counter() - t0) / 1e6
+    objs = [make(w2.fs, f) for f in w2.shifts()]
+    ths = [threading.Thread(target=o.apply, args=(x2,)) for o in objs]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    res["nco_16shift"] = {"value": round(16 * x2.size / dt / 1e6, 1), "unit": "output Msps",
+                          "cores": len(ths), "kind": kind, "single_thread_value": round(single, 2),
+                          "sample": "first 2^20 inputs, the 16 shifts one object per thread"}
+
+    w4 = FirBank256()
+    taps = w4.taps()
+    n4, nch = 1 << 16, 16
+    make = O.RefFir if kind == "reference" else O.Fir
+    objs = [make(taps[c].astype(np.complex64)) for c in range(nch)]
+    xs = [w4.input(c)[:n4] for c in range(nch)]
+    ths = [threading.Thread(target=objs[c].apply, args=(xs[c],)) for c in range(nch)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    res["fir_bank_256"] = {"value": round(4.0 * w4.n_taps * n4 * nch / dt / 1e12, 4), "unit": "TFLOP/s",
+                           "cores": len(ths), "kind": kind,
+                           "sample": "2^16 samples of 16 of the 256 channels, one object per thread (4 flops per "
+                                     "real tap per output, as the GPU line counts them)"}
+    return res
+
+
 def bench_other_configs(fd, stream, steps):
     """The other workloads, each with data resident in HBM: throughput and
     roofline fraction of one pass."""
@@ -538,6 +596,11 @@ def main():
         if others and "iir_order4" in others:
             others["iir_order4"]["cpu_baseline"] = iir_cpu_baseline()
             log(f"IIR CPU baseline: {others['iir_order4']['cpu_baseline']['value']} Msps single thread")
+        if others:
+            for key, v in other_cpu_baselines().items():
+                if key in others:
+                    others[key]["cpu_baseline"] = v
+                    log(f"{key} CPU baseline: {v['value']} {v['unit']} on {v['cores']} threads ({v['kind']})")
 
     rec = {
         "metric": "aggregate output Msps (32-ch multi-rate channelizer) + p99 chunk latency",
