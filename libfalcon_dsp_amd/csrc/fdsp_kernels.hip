@@ -470,10 +470,128 @@ __global__ __launch_bounds__(kBlock) void fir_bank_kernel(const float2 *__restri
             y[ob + r] = acc[r];
 }
 
+// FIR bank on FP32 matrix cores. For 16 consecutive outputs of a channel,
+// y[16b + r] = sum_t H[r][t] W[t][b] with H[r][t] = h[t - 15 + r] (a banded
+// Toeplitz matrix of the taps, zero outside 0 <= t - 15 + r < m) and
+// W[t][b] = x[16b + 15 - t]: a GEMM per channel with M = 16 (r), N = output
+// blocks b, K = m + 15 rounded up to 4 (kp). v_mfma_f32_16x16x4f32 takes
+// A = H (lane l: row l%16, k l/16) and B = W (lane l: column l%16, k l/16)
+// and leaves lane l outputs 4(l/16) .. 4(l/16)+3 of block l%16. The real and
+// imaginary parts of the samples are two GEMMs sharing A. A block stages
+// its input window and the zero-padded taps in LDS; each wave accumulates
+// kFbG column groups of 16 blocks.
+typedef float fb_floatx4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kFbG = 4, kFbWaves = 4;
+constexpr uint32_t kFbTile = kFbWaves * kFbG * 256;  // outputs per block
+// one padding sample per 16, so the 16 columns of a B read (16 samples apart)
+// fall in distinct banks
+__host__ __device__ __forceinline__ uint32_t fb_pad(uint32_t q) { return q + (q >> 4); }
+
+__global__ __launch_bounds__(kFbWaves * 64) void fir_bank_mfma_kernel(
+    const float2 *__restrict__ in, uint32_t n, uint64_t in_stride, const float *__restrict__ taps, uint32_t m,
+    uint32_t kp, const float2 *__restrict__ hist_in, float2 *__restrict__ hist_out, float2 *__restrict__ out,
+    uint64_t out_stride)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float2 *xs = reinterpret_cast<float2 *>(smem);
+    const uint32_t span = kFbTile + kp - 16;
+    float *hs = reinterpret_cast<float *>(xs + fb_pad(span) + 1);  // kp + 16 taps
+    const uint32_t c = blockIdx.y, tid = threadIdx.x;
+    const uint32_t hlen = m - 1;
+    const float2 *x = in + c * in_stride;
+    const float2 *hi = hist_in + (uint64_t)c * hlen;
+    const float *h = taps + (uint64_t)c * m;
+    float2 *y = out + c * out_stride;
+    if (blockIdx.x == 0) {
+        float2 *ho = hist_out + (uint64_t)c * hlen;
+        for (uint32_t t = tid; t < hlen; t += kFbWaves * 64) {
+            const int64_t i = (int64_t)n - hlen + t;
+            ho[t] = i >= 0 ? x[i] : hi[hlen + i];
+        }
+    }
+    const uint32_t o0 = blockIdx.x * kFbTile;
+    // window: x[lo .. lo + span), lo = o0 + 16 - kp; W[t][b] for the block's
+    // blocks bb = 0 .. kFbTile/16 - 1 sits at window index 16 bb + kp - 1 - t
+    const int64_t lo = (int64_t)o0 + 16 - (int64_t)kp;
+    for (uint32_t u = tid; u < kp + 16; u += kFbWaves * 64)
+        hs[u] = u >= 15 && u - 15 < m ? h[u - 15] : 0.f;
+    if (lo >= 0 && lo + span <= n) {
+        // the whole window inside the chunk: eight independent loads per thread in flight
+        constexpr uint32_t kU = 8, kT = kFbWaves * 64;
+        for (uint32_t t0 = tid; t0 < span; t0 += kT * kU) {
+            float2 v[kU];
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u)
+                v[u] = t0 + u * kT < span ? x[lo + t0 + u * kT] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u)
+                if (t0 + u * kT < span)
+                    xs[fb_pad(t0 + u * kT)] = v[u];
+        }
+    } else {
+        for (uint32_t t = tid; t < span; t += kFbWaves * 64) {
+            const int64_t i = lo + t;
+            float2 v = make_float2(0.f, 0.f);
+            if (i >= 0) {
+                if (i < n)
+                    v = x[i];
+            } else if (i >= -(int64_t)hlen) {
+                v = hi[hlen + i];
+            }
+            xs[fb_pad(t)] = v;
+        }
+    }
+    __syncthreads();
+    const uint32_t wave = tid >> 6, lane = tid & 63, kq = lane >> 4, j = lane & 15;
+    fb_floatx4 dr[kFbG], di[kFbG];
+#pragma unroll
+    for (uint32_t g = 0; g < kFbG; ++g) {
+        dr[g] = fb_floatx4{0.f, 0.f, 0.f, 0.f};
+        di[g] = fb_floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    // window index of W[kq][bb] at t0 = 0, for the lane's block in group 0
+    const uint32_t q0 = 16 * (wave * kFbG * 16 + j) + kp - 1 - kq;
+    for (uint32_t t0 = 0; t0 < kp; t0 += 4) {
+        const float a = hs[t0 + kq + j];  // H[j][t0 + kq]
+#pragma unroll
+        for (uint32_t g = 0; g < kFbG; ++g) {
+            const float2 b = xs[fb_pad(q0 + 256 * g - t0)];  // W[t0 + kq][block 16 g + j]
+            dr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.x, dr[g], 0, 0, 0);
+            di[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.y, di[g], 0, 0, 0);
+        }
+    }
+    const bool vec = (reinterpret_cast<uintptr_t>(y) & 15) == 0;
+#pragma unroll
+    for (uint32_t g = 0; g < kFbG; ++g) {
+        const uint32_t o = o0 + 16 * ((wave * kFbG + g) * 16 + j) + 4 * kq;
+        if (vec && o + 4 <= n) {
+            float4 *p = reinterpret_cast<float4 *>(y + o);
+            p[0] = make_float4(dr[g][0], di[g][0], dr[g][1], di[g][1]);
+            p[1] = make_float4(dr[g][2], di[g][2], dr[g][3], di[g][3]);
+        } else {
+#pragma unroll
+            for (uint32_t v = 0; v < 4; ++v)
+                if (o + v < n)
+                    y[o + v] = make_float2(dr[g][v], di[g][v]);
+        }
+    }
+}
+
 hipError_t launch_fir_bank(const float2 *in, uint32_t n, uint64_t in_stride, const float *taps, uint32_t m,
                            uint32_t n_chan, const float2 *hist_in, float2 *hist_out, float2 *out,
                            uint64_t out_stride, hipStream_t s)
 {
+    {
+        // the matrix-core form whenever its window and taps fit the LDS
+        const uint32_t kp = (m + 15 + 3) / 4 * 4;
+        const size_t lds = (size_t)(fb_pad(kFbTile + kp - 16) + 1) * sizeof(float2) + (kp + 16) * sizeof(float);
+        if (lds <= 64 * 1024) {
+            dim3 grid((n + kFbTile - 1) / kFbTile, n_chan);
+            hipLaunchKernelGGL(fir_bank_mfma_kernel, grid, dim3(kFbWaves * 64), lds, s, in, n, in_stride, taps, m,
+                               kp, hist_in, hist_out, out, out_stride);
+            return hipGetLastError();
+        }
+    }
     constexpr int kBlock = 256;
     constexpr uint32_t tile = kBlock * kBankR;
     const uint32_t mpad = (m + kBankR - 1) / kBankR * kBankR;
