@@ -375,9 +375,10 @@ def bench_other_configs(fd, stream, steps):
     tf = flops / (ms4 / 1e3) / 1e12
     out["fir_bank_256"] = {
         "value": round(tf, 2), "unit": "TFLOP/s", "ms_per_pass": round(ms4, 4),
-        "roofline": {"bound": "fp32 valu", "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s",
                      "frac": round(tf / 157.3, 4)},
-        "note": "4 flops per real tap per complex sample; input is channel 0's samples in all 256 channels"}
+        "note": "4 flops per real tap per complex sample on FP32 matrix cores (Toeplitz form); input is "
+                "channel 0's samples in all 256 channels"}
     log(f"fir_bank_256: {ms4:.3f} ms per pass, {tf:.1f} TFLOP/s")
     del d4, o4, fb
     return out
