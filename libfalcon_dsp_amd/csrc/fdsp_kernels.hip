@@ -8,6 +8,7 @@
 // n0 = ceil(p*up/down) ... ceil((p+N)*up/down)-1, and the reference's
 // coeff_phase / xOffset are functions of (p, n0). The NCO index of input i is
 // i mod R (reference src/transform/falcon_dsp_freq_shift.cc:177-189).
+#include <algorithm>
 #include <type_traits>
 
 #include "fdsp_internal.h"
@@ -269,6 +270,9 @@ hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t u
         tile = span_cap * up / down;
         if (tile > 4096)
             tile = 4096;
+        // short launches: smaller tiles, so the grid reaches about 512 blocks
+        // (a chunk of 52k outputs otherwise ran on 13 CUs)
+        tile = std::min<uint64_t>(tile, std::max<uint64_t>(kBlock, (count + 511) / 512));
     }
     const bool staged = tile >= kBlock;
     if (!staged) {
