@@ -62,6 +62,33 @@ def test_fir_bank_against_oracle():
             assert_close_rms(y[c], refs[c].apply(seg[c]), rms(x[c]), what=f"chan {c} [{lo},{hi})")
 
 
+@pytest.mark.parametrize("m", [1, 7, 100, 1000, 3000])
+def test_fir_bank_tap_counts_and_strides(m):
+    """Tap counts off the matrix-core K step of 4 (1 to 1000 taps run on
+    FP32 matrix cores), 3000 taps (a window past the LDS budget: the VALU
+    form), chunks shorter than a 4096-output tile, channel strides above the
+    chunk length; every channel against the oracle at 1e-4 x RMS."""
+    n_chan, n = 3, 9000
+    rng = np.random.default_rng(m)
+    taps = np.stack([kaiser_sinc(m, 0.05 + 0.4 * rng.random(), 8.0) if m > 1 else np.ones(1) for _ in range(n_chan)])
+    x = np.stack([cwgn(40 + c, n) for c in range(n_chan)])
+    bank = fd.FirBank(taps)
+    refs = [O.Fir(taps[c].astype(np.complex64)) for c in range(n_chan)]
+    stride = 5000
+    for lo, hi in ((0, 300), (300, 4396), (4396, 9000)):
+        k = hi - lo
+        d_in, d_out = fd.DeviceBuffer(n_chan * stride), fd.DeviceBuffer(n_chan * stride)
+        for c in range(n_chan):
+            d_in.upload(x[c, lo:hi], offset=c * stride)
+        bank.apply_device(d_in, k, stride, d_out, stride)
+        fd.device_sync()
+        for c in range(n_chan):
+            got = d_out.download(k, offset=c * stride)
+            want = refs[c].apply(x[c, lo:hi])
+            scale = rms(x[c]) * np.sqrt(np.sum(taps[c] ** 2))
+            assert_close_rms(got, want, scale, what=f"m={m} chan {c} [{lo},{hi})")
+
+
 def test_discriminator_kat_and_streaming():
     c = np.asarray([88 - 85j, 22 + 97j, 54 + 91j, 6 - 13j, -29 + 58j], np.complex64)
     want = [2.11582422, -0.31252633, -2.17362758, -3.11035282]
