@@ -81,7 +81,7 @@ struct IirLaunch {
     float2 *yh;               // kMaxFb carried outputs, newest first
     float2 *rs, *re;          // per segment: P start-state values, P end-state values
     unsigned long long *bad;  // per block: bit t set when segment t's start differs from t-1's end
-    uint32_t *ctl;            // [1]: blocks with a failed segment (the finishing wave clears it)
+    uint32_t *failed;         // blocks with a failed segment (the finishing wave clears it)
     uint32_t n, nb, warm, tout;
     uint32_t verify;          // 0: segments are trusted (outputs within float rounding)
     uint32_t split;           // serial form, complex samples, real a: re and im run on lanes 0 and 1
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> f
         for (uint32_t u = 0; u < kU; ++u)
             any |= d[u];
     }
-    any = g.verify && (__ballot(any) != 0ull || atomicAdd(&g.ctl[1], 0u) != 0u);
+    any = g.verify && (__ballot(any) != 0ull || atomicAdd(g.failed, 0u) != 0u);
     __syncthreads();
 
     if (any) {
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> f
         g.yh[lane] = to2(fixed_seg == (int32_t)last ? fixed_end[lane] : from2<C>(g.re[(size_t)last * P + lane]));
     carry_inputs<TIN>(g);
     if (lane == 0)
-        atomicExch(&g.ctl[1], 0u);
+        atomicExch(g.failed, 0u);
 }
 
 template <int P, bool C, uint32_t TIN, bool RA>
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
     if (lane == 0) {
         g.bad[blockIdx.x] = m;
         if (m)
-            atomicAdd(&g.ctl[1], 1u);
+            atomicAdd(g.failed, 1u);
     }
     // iir_finish_kernel, next on the stream, checks across blocks and repairs
 }
@@ -751,7 +751,7 @@ struct fdsp_iir_s {
     bool decays = true;        // whether `warm` reaches the 2^-44 decay (else it is capped)
     uint32_t warm_fast = kMinWarm;  // the unverified segments' warm-up (2^-26 decay)
     std::vector<float2> b, a;  // normalised taps; a[1..p] (zero past nfb)
-    DevBuf d_b, d_xh, d_yh, d_rs, d_re, d_bad, d_ctl, d_in, d_out;
+    DevBuf d_b, d_xh, d_yh, d_rs, d_re, d_bad, d_failed, d_in, d_out;
     hipStream_t stream = nullptr;
 };
 
@@ -803,12 +803,12 @@ extern "C" int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 
     if (!rc)
         rc = h->d_yh.ensure(kMaxFb * sizeof(float2));
     if (!rc)
-        rc = h->d_ctl.ensure(2 * sizeof(uint32_t));
+        rc = h->d_failed.ensure(sizeof(uint32_t));
     if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
                 hipMemcpy(h->d_b.p, h->b.data(), nb * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemset(h->d_xh.p, 0, h->d_xh.bytes) != hipSuccess ||
                 hipMemset(h->d_yh.p, 0, h->d_yh.bytes) != hipSuccess ||
-                hipMemset(h->d_ctl.p, 0, h->d_ctl.bytes) != hipSuccess))
+                hipMemset(h->d_failed.p, 0, h->d_failed.bytes) != hipSuccess))
         rc = set_err(FDSP_EHIP, "iir_create: HIP setup failed");
     if (rc) {
         fdsp_iir_destroy(h);
@@ -824,7 +824,7 @@ extern "C" int fdsp_iir_destroy(fdsp_iir_t h)
         return FDSP_OK;
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
-    for (DevBuf *d : {&h->d_b, &h->d_xh, &h->d_yh, &h->d_rs, &h->d_re, &h->d_bad, &h->d_ctl, &h->d_in, &h->d_out})
+    for (DevBuf *d : {&h->d_b, &h->d_xh, &h->d_yh, &h->d_rs, &h->d_re, &h->d_bad, &h->d_failed, &h->d_in, &h->d_out})
         d->release();
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
@@ -918,7 +918,7 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     g.rs = h->d_rs.as<float2>();
     g.re = h->d_re.as<float2>();
     g.bad = h->d_bad.as<unsigned long long>();
-    g.ctl = h->d_ctl.as<uint32_t>();
+    g.failed = h->d_failed.as<uint32_t>();
     g.n = (uint32_t)n;
     g.nb = h->nb;
     g.warm = h->warm;
