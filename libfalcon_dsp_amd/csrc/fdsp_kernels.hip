@@ -140,9 +140,12 @@ hipError_t launch_nco_multi(const TI *in, uint32_t n, uint64_t start, const NcoC
         return hipSuccess;
     const uint32_t threads = (n + kNcoL - 1) / kNcoL;
     const uint32_t blocks = (threads + 255) / 256;
-    // shifts per block: all of them once the chunk alone gives 2048 blocks
-    // (8 per CU), else split so the grid reaches about that
-    const uint32_t groups = min(n_chan, max(1u, 2048u / blocks));
+    // shifts per block: split so the grid reaches about `target` blocks.
+    // Measured on 2^18-sample chunks of 16 shifts: complex float outputs run
+    // best at 512 (8 shifts per thread), complex int16 outputs, with their
+    // per-sample reference phasors, at 1024 (4 per thread).
+    const uint32_t target = std::is_same<TO, short2>::value ? 1024u : 512u;
+    const uint32_t groups = min(n_chan, max(1u, target / blocks));
     const uint32_t cpb = (n_chan + groups - 1) / groups;
     const dim3 grid(blocks, (n_chan + cpb - 1) / cpb);
     // 16-byte accesses of 4 samples: every channel's output must start 16-byte aligned
