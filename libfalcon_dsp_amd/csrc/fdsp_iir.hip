@@ -77,8 +77,9 @@ struct IirLaunch {
     const void *x;            // n inputs of type TIN
     void *out;                // n outputs of type tout
     const float2 *b;          // nb taps
-    float2 *xh;               // nb - 1 carried inputs, oldest first
-    float2 *yh;               // kMaxFb carried outputs, newest first
+    const float2 *xh;         // nb - 1 carried inputs, oldest first
+    const float2 *yh;         // kMaxFb carried outputs, newest first
+    float2 *xh_next, *yh_next;  // the same after this chunk (the other copy: no read of xh, yh races)
     float2 *rs, *re;          // per segment: P start-state values, P end-state values
     unsigned long long *bad;  // per block: bit t set when segment t's start differs from t-1's end
     uint32_t *failed;         // blocks with a failed segment (the finishing wave clears it)
@@ -276,23 +277,16 @@ __device__ __forceinline__ unsigned long long shfl64(unsigned long long v, int l
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// carried inputs: the last nb - 1 of (old history, this chunk); every read
-// of the old history is done before the first write
+// carried inputs: the last nb - 1 of (old history, this chunk), into the
+// next copy (one wave; nb - 1 <= 255)
 template <uint32_t TIN> __device__ void carry_inputs(const IirLaunch &g)
 {
     const uint32_t lane = threadIdx.x, h = g.nb - 1;
-    float2 keep[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t k = lane + r * kLanes;
-        keep[r] = k < h ? load_x<TIN, true>(g, (int32_t)g.n - (int32_t)h + (int32_t)k) : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t k = lane + r * kLanes;
         if (k < h)
-            g.xh[k] = keep[r];
+            g.xh_next[k] = load_x<TIN, true>(g, (int32_t)g.n - (int32_t)h + (int32_t)k);
     }
 }
 
@@ -402,7 +396,7 @@ __global__ __launch_bounds__(kLanes) void iir_finish_kernel(IirLaunch g, Fb<P> f
     // carried outputs: the last segment's end state
     const uint32_t last = nseg - 1;
     if (lane < (uint32_t)P)
-        g.yh[lane] = to2(fixed_seg == (int32_t)last ? fixed_end[lane] : from2<C>(g.re[(size_t)last * P + lane]));
+        g.yh_next[lane] = to2(fixed_seg == (int32_t)last ? fixed_end[lane] : from2<C>(g.re[(size_t)last * P + lane]));
     carry_inputs<TIN>(g);
     if (lane == 0)
         atomicExch(g.failed, 0u);
@@ -492,7 +486,17 @@ __global__ __launch_bounds__(kLanes) void iir_kernel(IirLaunch g, Fb<P> fb)
         if (m)
             atomicAdd(g.failed, 1u);
     }
-    // iir_finish_kernel, next on the stream, checks across blocks and repairs
+    // verifying: iir_finish_kernel, next on the stream, checks across blocks,
+    // repairs and carries the state. Unverified: the block holding the
+    // chunk's last segment carries it (its lane's end state is final).
+    if (!g.verify && blockIdx.x == gridDim.x - 1) {
+        if (active && s0 + S >= g.n) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                g.yh_next[k] = to2(st[k]);
+        }
+        carry_inputs<TIN>(g);
+    }
 }
 
 // The sequential form, for filters whose segments do not merge: one wave
@@ -569,12 +573,12 @@ __global__ __launch_bounds__(kLanes) void iir_serial_kernel(IirLaunch g, Fb<P> f
         if (lane < 2) {
 #pragma unroll
             for (int k = 0; k < P; ++k)
-                reinterpret_cast<float *>(g.yh + k)[lane] = sr[k];
+                reinterpret_cast<float *>(g.yh_next + k)[lane] = sr[k];
         }
     } else if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < P; ++k)
-            g.yh[k] = to2(st[k]);
+            g.yh_next[k] = to2(st[k]);
     }
     carry_inputs<TIN>(g);
 }
@@ -592,7 +596,8 @@ hipError_t launch_p(const IirLaunch &g, const float2 *a, bool serial, hipStream_
     const uint32_t blocks = (g.n + kLanes * g.seg - 1) / (kLanes * g.seg);
     const size_t lds = slots<C>(kLanes * g.seg + g.warm + g.nb - 1) * sizeof(Val<C>);
     hipLaunchKernelGGL((iir_kernel<P, C, TIN, RA>), dim3(blocks), dim3(kLanes), lds, s, g, fb);
-    hipLaunchKernelGGL((iir_finish_kernel<P, C, TIN, RA>), dim3(1), dim3(kLanes), 0, s, g, fb);
+    if (g.verify)
+        hipLaunchKernelGGL((iir_finish_kernel<P, C, TIN, RA>), dim3(1), dim3(kLanes), 0, s, g, fb);
     return hipGetLastError();
 }
 
@@ -751,7 +756,8 @@ struct fdsp_iir_s {
     bool decays = true;        // whether `warm` reaches the 2^-44 decay (else it is capped)
     uint32_t warm_fast = kMinWarm;  // the unverified segments' warm-up (2^-26 decay)
     std::vector<float2> b, a;  // normalised taps; a[1..p] (zero past nfb)
-    DevBuf d_b, d_xh, d_yh, d_rs, d_re, d_bad, d_failed, d_in, d_out;
+    DevBuf d_b, d_xh[2], d_yh[2], d_rs, d_re, d_bad, d_failed, d_in, d_out;
+    uint32_t parity = 0;  // which copy of the carried state is current
     hipStream_t stream = nullptr;
 };
 
@@ -799,15 +805,19 @@ extern "C" int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 
     h->merges = segments_merge(h->b, h->a, h->p, h->warm);
     int rc = h->d_b.ensure(nb * sizeof(float2));
     if (!rc)
-        rc = h->d_xh.ensure(std::max<uint32_t>(nb - 1, 1) * sizeof(float2));
+        rc = h->d_xh[0].ensure(std::max<uint32_t>(nb - 1, 1) * sizeof(float2));
     if (!rc)
-        rc = h->d_yh.ensure(kMaxFb * sizeof(float2));
+        rc = h->d_xh[1].ensure(std::max<uint32_t>(nb - 1, 1) * sizeof(float2));
+    if (!rc)
+        rc = h->d_yh[0].ensure(kMaxFb * sizeof(float2));
+    if (!rc)
+        rc = h->d_yh[1].ensure(kMaxFb * sizeof(float2));
     if (!rc)
         rc = h->d_failed.ensure(sizeof(uint32_t));
     if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
                 hipMemcpy(h->d_b.p, h->b.data(), nb * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemset(h->d_xh.p, 0, h->d_xh.bytes) != hipSuccess ||
-                hipMemset(h->d_yh.p, 0, h->d_yh.bytes) != hipSuccess ||
+                hipMemset(h->d_xh[0].p, 0, h->d_xh[0].bytes) != hipSuccess ||
+                hipMemset(h->d_yh[0].p, 0, h->d_yh[0].bytes) != hipSuccess ||
                 hipMemset(h->d_failed.p, 0, h->d_failed.bytes) != hipSuccess))
         rc = set_err(FDSP_EHIP, "iir_create: HIP setup failed");
     if (rc) {
@@ -824,7 +834,8 @@ extern "C" int fdsp_iir_destroy(fdsp_iir_t h)
         return FDSP_OK;
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
-    for (DevBuf *d : {&h->d_b, &h->d_xh, &h->d_yh, &h->d_rs, &h->d_re, &h->d_bad, &h->d_failed, &h->d_in, &h->d_out})
+    for (DevBuf *d : {&h->d_b, &h->d_xh[0], &h->d_xh[1], &h->d_yh[0], &h->d_yh[1], &h->d_rs, &h->d_re, &h->d_bad,
+                      &h->d_failed, &h->d_in, &h->d_out})
         d->release();
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
@@ -838,8 +849,8 @@ extern "C" int fdsp_iir_reset(fdsp_iir_t h)
         return set_err(FDSP_ENOTINIT, "null handle");
     std::lock_guard<std::mutex> lk(h->mu);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMemset(h->d_xh.p, 0, h->d_xh.bytes));
-    HIP_TRY(hipMemset(h->d_yh.p, 0, h->d_yh.bytes));
+    HIP_TRY(hipMemset(h->d_xh[h->parity].p, 0, h->d_xh[h->parity].bytes));
+    HIP_TRY(hipMemset(h->d_yh[h->parity].p, 0, h->d_yh[h->parity].bytes));
     return FDSP_OK;
 }
 
@@ -913,8 +924,10 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     g.x = d_in;
     g.out = d_out;
     g.b = h->d_b.as<float2>();
-    g.xh = h->d_xh.as<float2>();
-    g.yh = h->d_yh.as<float2>();
+    g.xh = h->d_xh[h->parity].as<float2>();
+    g.yh = h->d_yh[h->parity].as<float2>();
+    g.xh_next = h->d_xh[h->parity ^ 1].as<float2>();
+    g.yh_next = h->d_yh[h->parity ^ 1].as<float2>();
     g.rs = h->d_rs.as<float2>();
     g.re = h->d_re.as<float2>();
     g.bad = h->d_bad.as<unsigned long long>();
@@ -945,6 +958,7 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     else
         e = launch_c<true, kSampleI16>(g, h->p, a, serial, s);
     HIP_TRY(e);
+    h->parity ^= 1;  // the launch wrote the next carried state into the other copy
     return FDSP_OK;
 }
 
