@@ -558,11 +558,37 @@ __global__ __launch_bounds__(kFbWaves * 64) void fir_bank_mfma_kernel(
     }
     // window index of W[kq][bb] at t0 = 0, for the lane's block in group 0
     const uint32_t q0 = 16 * (wave * kFbG * 16 + j) + kp - 1 - kq;
-    for (uint32_t t0 = 0; t0 < kp; t0 += 4) {
-        const float a = hs[t0 + kq + j];  // H[j][t0 + kq]
+    // K-steps in rounds of four, t0 = 16u + 4s. W[t0 + kq][block 16g + j]
+    // sits at fb_pad(q0 + 256g - t0) = fb_pad(q0) + 272g - e_s - 17u with
+    // e_s = 4s + (4s > q0 mod 16): one offset per s, the groups at constant
+    // distances, 17 slots back per round; H[j][t0 + kq] at t0 + kq + j.
+    uint32_t bo[4];
+#pragma unroll
+    for (uint32_t st = 0; st < 4; ++st)
+        bo[st] = fb_pad(q0) - (4 * st + (4 * st > (q0 & 15) ? 1u : 0u));
+    uint32_t ao = kq + j;
+    const uint32_t rounds = kp / 16;
+    for (uint32_t u = 0; u < rounds; ++u) {
+#pragma unroll
+        for (uint32_t st = 0; st < 4; ++st) {
+            const float a = hs[ao + 4 * st];
+#pragma unroll
+            for (uint32_t g = 0; g < kFbG; ++g) {
+                const float2 b = xs[bo[st] + 272 * g];
+                dr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.x, dr[g], 0, 0, 0);
+                di[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.y, di[g], 0, 0, 0);
+            }
+        }
+        ao += 16;
+#pragma unroll
+        for (uint32_t st = 0; st < 4; ++st)
+            bo[st] -= 17;
+    }
+    for (uint32_t t0 = rounds * 16; t0 < kp; t0 += 4) {  // kp not a multiple of 16
+        const float a = hs[t0 + kq + j];
 #pragma unroll
         for (uint32_t g = 0; g < kFbG; ++g) {
-            const float2 b = xs[fb_pad(q0 + 256 * g - t0)];  // W[t0 + kq][block 16 g + j]
+            const float2 b = xs[fb_pad(q0 + 256 * g - t0)];
             dr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.x, dr[g], 0, 0, 0);
             di[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.y, di[g], 0, 0, 0);
         }
