@@ -583,8 +583,9 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
     const float2 s_sub = nco_phasor((int64_t)du, nu);
     const uint32_t k_hi = R > (uint64_t)du + dn ? (uint32_t)(R - du - dn) : 0u;
 
-    // A operands and MFMA chains of one sub-tile: two independent chains (the
-    // real-part and the imaginary-part steps), summed by the consumer
+    // A operands and MFMA chains of one sub-tile: FP32, two independent
+    // chains (the real-part and the imaginary-part steps), summed by the
+    // consumer; bf16 planes, one chain in c0
     auto mma = [&](uint32_t st, floatx4 &c0, floatx4 &c1) {
         if constexpr (kBf) {
             const unsigned char *pb = reinterpret_cast<const unsigned char *>(planes) + st * 16 * dn * 2;
@@ -594,14 +595,16 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
                 ah[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q]);
                 al[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q] + 2 * kWords * 4);
             }
+            // one chain (the lo terms first, then hi x hi): the epilogue
+            // needs no sum of two chains
             c0 = floatx4{0.f, 0.f, 0.f, 0.f};
-            c1 = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (uint32_t q = 0; q < 3; ++q) {
+                c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[q], c0, 0, 0, 0);
+                c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[q], c0, 0, 0, 0);
                 c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh[q], c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[q], c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[q], c1, 0, 0, 0);
             }
+            c1 = c0;  // unused: the bf16 epilogue reads c0 alone
             return;
         }
         float2 x[kS];
@@ -666,7 +669,11 @@ __device__ __forceinline__ void mfma_stripe(const GroupsDev &a, const float2 *__
             // kPipe, the next sub-tile's LDS reads and MFMAs were issued just
             // before in the same block of code and interleave with this work.
             auto finish = [&](auto pipe, uint32_t st, const floatx4 &c0, const floatx4 &c1) {
-                const floatx4 acc = c0 + c1;
+                floatx4 acc;
+                if constexpr (kBf)
+                    acc = c0;
+                else
+                    acc = c0 + c1;
                 // partner lane's values (quad_perm 1,0,3,2 swaps lanes 2c and 2c+1)
                 float oth[4];
 #pragma unroll
@@ -863,14 +870,15 @@ __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const Fu
             ah[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q]);
             al[q] = *reinterpret_cast<const bf16x8 *>(pb + aoff[q] + 2 * kWords * 4);
         }
+        // one chain, as in mfma_stripe
         c0 = floatx4{0.f, 0.f, 0.f, 0.f};
-        c1 = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (uint32_t q = 0; q < 3; ++q) {
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[q], c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[q], c0, 0, 0, 0);
             c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh[q], c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[q], c1, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[q], c1, 0, 0, 0);
         }
+        c1 = c0;
     };
 
     uint32_t k = (uint32_t)mod_u64(P + (uint64_t)a0 + (uint64_t)(cls + nc * (wt * 16 + ro)) * down, (uint32_t)R, inv_r);
@@ -878,7 +886,8 @@ __device__ __forceinline__ void bf_rows(const GroupsDev &a, uint64_t P, const Fu
     // as in mfma_stripe: combine across the lane pair, phasor, store; the
     // next sub-tile's MFMAs interleave with this work when kPipe
     auto finish = [&](auto pipe, uint32_t st, const floatx4 &c0, const floatx4 &c1) {
-        const floatx4 acc = c0 + c1;
+        const floatx4 acc = c0;
+        (void)c1;
         float oth[4];
 #pragma unroll
         for (uint32_t e = 0; e < 4; ++e)
