@@ -69,6 +69,26 @@ def test_rollover_crossing_windows(batched):
     _compare(fs, chans, x, [434_780, 5, 1, 262_144], batched)
 
 
+@pytest.mark.parametrize("batched", [True, False])
+def test_empty_chunk_keeps_the_stream(batched):
+    # the reference's apply pushes one (here empty) output per channel and an
+    # empty chunk moves no state (falcon_dsp_multi_rate_channelizer.cc:146-164)
+    fs = 20_000_000
+    chans = _chans_small(fs)
+    x = cwgn(41, 50_000)
+    gpu, ref = fd.MultiRateChannelizer(fs, chans, batched=batched), O.Channelizer(fs, chans)
+    xr = rms(x)
+    for lo, hi in ((0, 20_000), (20_000, 20_000), (20_000, 20_001), (20_001, 20_001), (20_001, 50_000)):
+        print(f"channelizer batched={batched} chunk [{lo},{hi})", flush=True)
+        ys, rs = gpu.apply(x[lo:hi]), ref.apply(x[lo:hi])
+        assert len(ys) == len(rs) == len(chans)
+        for c, (y, r) in enumerate(zip(ys, rs)):
+            assert y.size == r.size, f"chan {c} chunk [{lo},{hi})"
+            if r.size:
+                assert_close_rms(y, r, xr, what=f"chan {c} chunk [{lo},{hi})")
+            assert gpu.state(c)[3] == hi
+
+
 def test_batched_equals_plain_and_state():
     fs = 20_000_000
     chans = _chans_small(fs)
