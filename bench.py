@@ -94,20 +94,56 @@ def cpu_baseline(fs, chans, x, n_single, n_par):
     }
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "chan_groups_kernel_chan32_traffic.json")
-
-
-def measured_traffic():
-    """HBM bytes per launch of the fused kernel on this workload, from the
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 32-channel workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes, each scaled by the
     same counter's reading for a device copy of known size; (None, None)
-    when no such measurement is committed."""
+    when no such measurement is committed for that kernel."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_chan32_traffic.json")
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
-        return int(t["traffic_bytes_per_dispatch"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+        return int(t["traffic_bytes_per_dispatch"]), os.path.relpath(path, ROOT)
     except (OSError, KeyError, ValueError):
         return None, None
+
+
+def chan32_graph_rate(fd, w, chans, x_dev, stream, steps, warmup, arith, kernel=None, tune=True):
+    """Aggregate output Msps of one channelizer form over the 32-channel
+    workload, replayed as a HIP graph, and its kernel's average time from
+    back-to-back eager launches (events on the kernel's stream)."""
+    n = w.n
+    g = fd.MultiRateChannelizer(w.fs, chans, batched=True, arith=arith)
+    if kernel:
+        g.set_kernel(kernel)
+    trials = g.autotune(x_dev, n, reps=5) if tune else None
+    outs = [fd.DeviceBuffer(m) for m in g.max_out(n)]
+    g.graph_create(n, x_dev, outs)
+    for _ in range(warmup):
+        g.graph_launch(stream)
+    stream.sync()
+    e0, e1 = fd.Event(), fd.Event()
+    total = 0
+    e0.record(stream)
+    for _ in range(steps):
+        g.graph_launch(stream)
+        total += sum(g.graph_counts())
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1)
+    g.apply_device(x_dev, n, outs, stream)
+    stream.sync()
+    k0, k1 = fd.Event(), fd.Event()
+    k0.record(stream)
+    for _ in range(steps):
+        g.apply_device(x_dev, n, outs, stream)
+    k1.record(stream)
+    k1.sync()
+    k_us = 1e3 * k0.elapsed_ms(k1) / steps
+    seg, block = g.get_tiling()
+    return {"value": round(total / (ms / 1e3) / 1e6, 1), "unit": "Msps", "ms_per_step": round(ms / steps, 5),
+            "kernel": g.kernel_info()["kernel"], "kernel_avg_us": round(k_us, 2),
+            "tiling": [seg, block], "autotune_trials": trials}
 
 
 def pct(v, q):
@@ -364,10 +400,12 @@ def bench_other_configs(fd, stream, steps):
     w4 = FirBank256()
     taps = w4.taps()
     fb = fd.FirBank(taps)
-    one = w4.input(0)
     d4 = fd.DeviceBuffer(w4.n_chan * w4.n)
-    for c in range(w4.n_chan):
-        d4.upload(one, offset=c * w4.n)  # the same samples in every channel (timing does not depend on them)
+    # each channel's own seeded samples (CWGN seed 3 at offset c * 2^20), made on the host cores
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(cpu_threads()) as pool:
+        for c, xc in enumerate(pool.map(w4.input, range(w4.n_chan))):
+            d4.upload(xc, offset=c * w4.n)
     o4 = fd.DeviceBuffer(w4.n_chan * w4.n)
     ms4 = timed_launches(fd, stream, lambda: fb.apply_device(d4, w4.n, w4.n, o4, w4.n, stream),
                          max(2, steps // 4))
@@ -377,8 +415,8 @@ def bench_other_configs(fd, stream, steps):
         "value": round(tf, 2), "unit": "TFLOP/s", "ms_per_pass": round(ms4, 4),
         "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s",
                      "frac": round(tf / 157.3, 4)},
-        "note": "4 flops per real tap per complex sample on FP32 matrix cores (Toeplitz form); input is "
-                "channel 0's samples in all 256 channels"}
+        "note": "4 flops per real tap per complex sample on FP32 matrix cores (Toeplitz form); each channel "
+                "its own seeded CWGN samples"}
     log(f"fir_bank_256: {ms4:.3f} ms per pass, {tf:.1f} TFLOP/s")
     del d4, o4, fb
     return out
@@ -482,7 +520,8 @@ def main():
     x = w.input()
     n = x.size
 
-    g = fd.MultiRateChannelizer(w.fs, chans, batched=True)
+    g = fd.MultiRateChannelizer(w.fs, chans, batched=True)  # FP32 (the default arithmetic)
+    kinfo = g.kernel_info()
     stream = fd.Stream()
     d_in = fd.DeviceBuffer.from_array(x)
     trials = None
@@ -491,8 +530,9 @@ def main():
         g.set_tiling(args.seg or s0, args.block or b0)
     else:
         trials = g.autotune(d_in, n, reps=5)
-        log("autotune: " + ", ".join(f"{t['segment_len']}/{t['block_threads']}{'m' if t['measured_weights'] else ''}"
-                                     f"{'f' if t['fused'] else ''}: {t['us']} us" for t in trials))
+        log(f"{kinfo['kernel']}: autotune: " + ", ".join(
+            f"{t['segment_len']}/{t['block_threads']}{'m' if t['measured_weights'] else ''}"
+            f"{'f' if t['fused'] else ''}: {t['us']} us" for t in trials))
     seg, block = g.get_tiling()
     outs = [fd.DeviceBuffer(m) for m in g.max_out(n)]
     g.graph_create(n, d_in, outs)
@@ -537,8 +577,8 @@ def main():
     k_avg = k0.elapsed_ms(k1) / args.steps
     alg_bytes = 8 * n + 8 * statistics.mean(out_per)
     achieved = alg_bytes / (k_avg / 1e3) / 1e9
-    log(f"fused kernel: avg {k_avg * 1e3:.1f} us, algorithmic {alg_bytes / 1e6:.1f} MB -> {achieved:.0f} GB/s")
-    traffic, traffic_src = measured_traffic()
+    log(f"{kinfo['kernel']}: avg {k_avg * 1e3:.1f} us, algorithmic {alg_bytes / 1e6:.1f} MB -> {achieved:.0f} GB/s")
+    traffic, traffic_src = measured_traffic(kinfo["kernel"])
 
     # p50/p99 chunk latency: streaming chunks of --latency-chunk through a graph
     lc = args.latency_chunk
@@ -586,6 +626,21 @@ def main():
         del gp, pouts
 
     others = None if args.no_others else bench_other_configs(fd, stream, args.steps)
+    if not args.no_others:
+        # the same workload on the channel-group kernel in FP32, and on the opt-in
+        # reduced-precision form (bf16 hi/lo planes), each labelled by its arithmetic
+        gf = chan32_graph_rate(fd, w, chans, d_in, stream, args.steps, args.warmup, "f32", kernel="groups")
+        gf.update({"dtype": "f32", "note": "channel-group kernel (FDSP_KERNEL_GROUPS_F32): each group family "
+                                           "stages its own tiles; FP32 MFMA and VALU"})
+        log(f"chan_32 on {gf['kernel']} (f32): {gf['value']} Msps, kernel {gf['kernel_avg_us']} us")
+        gb = chan32_graph_rate(fd, w, chans, d_in, stream, args.steps, args.warmup, "bf16x2")
+        gb.update({"dtype": "bf16x2-split", "note": "FDSP_CHAN_BATCHED_BF16X2, opt-in: groups of 32-tap windows "
+                                                    "with an even down below 32 multiply bf16 hi/lo splits (about "
+                                                    "16 significant bits, lo*lo dropped, FP32 accumulation); not "
+                                                    "an FP32 result"})
+        log(f"chan_32 bf16x2: {gb['value']} Msps, kernel {gb['kernel_avg_us']} us")
+        others["chan_32_groups_f32"] = gf
+        others["chan_32_bf16x2"] = gb
     serving = None if args.no_serving else bench_serving(fd)
 
     cpu = None
@@ -617,18 +672,18 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded CWGN + 32 tones, splitmix64/Box-Muller)",
         "config": {"workload": "chan_32: 2^24 cf32 samples at 20 Msps, 32 channels, outputs "
-                               "{12.5k,25k,100k,250k,1M,2.5M} sps, 32 Kaiser taps, one fused launch per step "
-                               "in a HIP graph",
+                               "{12.5k,25k,100k,250k,1M,2.5M} sps, 32 Kaiser taps, one fused FP32 launch per "
+                               "step in a HIP graph",
                    "input_samples": n, "channels": len(chans), "outputs_per_step": int(round(total_out / args.steps)),
-                   "segment_len": seg, "block_threads": block, "fused_groups": g.get_fusion()[0],
-                   "autotune_trials": trials},
+                   "kernel": kinfo["kernel"], "work_groups": kinfo["groups"], "mfma_groups": kinfo["mfma_groups"],
+                   "segment_len": seg, "block_threads": block, "autotune_trials": trials},
         "p50_chunk_latency_ms": round(p50, 5),
         "p99_chunk_latency_ms": round(p99, 5),
         "latency_chunk_samples": lc,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "chan_groups_kernel", "kernel_avg_us": round(k_avg * 1e3, 2),
+                     "kernel": kinfo["kernel"], "kernel_avg_us": round(k_avg * 1e3, 2),
                      "algorithmic_bytes": int(alg_bytes)},
         "plain_gpu": plain,
         "batched_vs_plain": round(value / plain["value"], 1) if plain else None,

@@ -303,10 +303,19 @@ int fdsp_fm_demod_apply_device(fdsp_fm_demod_t h, const fdsp_cf32 *d_in, size_t 
  * FDSP_CHAN_PLAIN keeps the reference's structure on the GPU: one multi-shift
  * NCO launch writing a full-rate copy per channel, then one resampler launch
  * per channel. FDSP_CHAN_BATCHED runs every channel in one fused launch that
- * reads the input once (shift folded into modulated taps, input staged in LDS).
- * Both produce the reference's samples and counts. */
-#define FDSP_CHAN_PLAIN   0
-#define FDSP_CHAN_BATCHED 1
+ * reads the input once (shift folded into modulated taps, input staged in LDS),
+ * in FP32 arithmetic throughout, as the reference's multiply-accumulates
+ * (src/transform/falcon_dsp_polyphase_resampler.cc:288,298). Both produce the
+ * reference's samples and counts.
+ * FDSP_CHAN_BATCHED_BF16X2 is the batched form with reduced-precision matrix
+ * products, opt-in only: channel groups with 32-tap windows and an even down
+ * below 32 multiply bf16 hi/lo splits of the input and taps (about 16
+ * significant bits; the hi*hi, hi*lo and lo*hi products, FP32 accumulation).
+ * Its outputs stay within 1e-4 x input RMS of the reference but are not FP32
+ * results. */
+#define FDSP_CHAN_PLAIN          0
+#define FDSP_CHAN_BATCHED        1
+#define FDSP_CHAN_BATCHED_BF16X2 2
 
 #define FDSP_TAIL_NONE          0
 #define FDSP_TAIL_DISCRIMINATOR 1 /* channel output is float angles (polar discriminator) */
@@ -371,6 +380,30 @@ int fdsp_channelizer_graph_destroy(fdsp_channelizer_t h);
 #define FDSP_TILING_STRIDED 1u
 int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segment_len, uint32_t block_threads);
 int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uint32_t *block_threads);
+/* Which kernel a batched channelizer launches (no reference counterpart:
+ * the reference has one CUDA structure):
+ *  - FDSP_KERNEL_SEGMENT_F32: the FDSP_CHAN_BATCHED default whenever every
+ *    channel's window (taps per phase) is at most 64. One workgroup per input
+ *    segment stages it once and computes every channel's outputs in it
+ *    (groups of 2..8 channels with up 1, equal down and <= 32 taps on FP32
+ *    matrix cores, the others on FP32 vector ALUs). Its tiling takes
+ *    segment_len 1024, 2048 or 4096 (256 threads); the autotuner times the
+ *    three.
+ *  - FDSP_KERNEL_GROUPS_F32: the channel-group kernel in FP32 (windows
+ *    above 64 taps), with the work splits described above.
+ *  - FDSP_KERNEL_GROUPS_BF16X2: FDSP_CHAN_BATCHED_BF16X2.
+ *  - FDSP_KERNEL_PLAIN: FDSP_CHAN_PLAIN.
+ * groups / mfma_groups (optional): the plan's work groups and how many of
+ * them run on matrix cores. */
+#define FDSP_KERNEL_SEGMENT_F32   0u
+#define FDSP_KERNEL_GROUPS_F32    1u
+#define FDSP_KERNEL_GROUPS_BF16X2 2u
+#define FDSP_KERNEL_PLAIN         3u
+int fdsp_channelizer_kernel_info(fdsp_channelizer_t h, uint32_t *kernel, uint32_t *groups, uint32_t *mfma_groups);
+/* Switch an FDSP_CHAN_BATCHED handle between FDSP_KERNEL_SEGMENT_F32 (when
+ * the channel set fits it) and FDSP_KERNEL_GROUPS_F32; both are FP32 and
+ * carry the same stream state. No graph may exist. */
+int fdsp_channelizer_set_kernel(fdsp_channelizer_t h, uint32_t kernel);
 /* The batched kernel's work split for a chunk of chunk_len samples: blocks
  * launched, blocks the GPU holds at once, channel groups, LDS bytes per block
  * (diagnostics for the autotuner; any output may be null). */
@@ -392,7 +425,8 @@ int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t 
 typedef struct fdsp_tiling_trial {
     uint32_t segment_len, block_threads;
     float us;
-    uint32_t flags; /* bit 0: groups weighed by measured work; bit 1: bf16-plane groups fused */
+    uint32_t flags; /* bit 0: groups weighed by measured work; bit 1: bf16-plane groups fused;
+                       bit 2: segment kernel (segment_len is its segment) */
 } fdsp_tiling_trial;
 int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len, uint32_t reps,
                               fdsp_tiling_trial *trials, size_t cap, size_t *n_trials);

@@ -23,6 +23,12 @@ FDSP_EAGAIN = -7
 
 FDSP_CHAN_PLAIN = 0
 FDSP_CHAN_BATCHED = 1
+FDSP_CHAN_BATCHED_BF16X2 = 2
+FDSP_KERNEL_SEGMENT_F32 = 0
+FDSP_KERNEL_GROUPS_F32 = 1
+FDSP_KERNEL_GROUPS_BF16X2 = 2
+FDSP_KERNEL_PLAIN = 3
+KERNEL_NAMES = {0: "chan_seg_kernel", 1: "chan_groups_kernel", 2: "chan_groups_kernel", 3: "nco_multi_kernel+resampler_kernel"}
 FDSP_TAIL_NONE = 0
 FDSP_TAIL_DISCRIMINATOR = 1
 
@@ -203,6 +209,8 @@ _SIGS = [
     ("fdsp_channelizer_graph_destroy", _i, [_vp]),
     ("fdsp_channelizer_set_tiling", _i, [_vp, _u32, _u32]),
     ("fdsp_channelizer_get_tiling", _i, [_vp, _P(_u32), _P(_u32)]),
+    ("fdsp_channelizer_kernel_info", _i, [_vp, _P(_u32), _P(_u32), _P(_u32)]),
+    ("fdsp_channelizer_set_kernel", _i, [_vp, _u32]),
     ("fdsp_channelizer_plan_info", _i, [_vp, _sz, _P(_u32), _P(_u32), _P(_u32), _P(_u32)]),
     ("fdsp_channelizer_autotune", _i, [_vp, _vp, _sz, _u32, _P(TilingTrial), _sz, _P(_sz)]),
     ("fdsp_channelizer_set_fusion", _i, [_vp, _i]),

@@ -1,6 +1,10 @@
 """In-tree build of libfdsp_hip.so (gfx950) and the C++ drop-in library.
 
-    python -m libfalcon_dsp_amd.build
+    python -m libfalcon_dsp_amd.build [--incremental]
+
+Each HIP source compiles to its own object under libfalcon_dsp_amd/_build/
+(in parallel), then the objects link into libfdsp_hip.so. --incremental
+recompiles only objects older than their source or a csrc/ header.
 
 Packed FP32 instructions are switched off for device code
 (-target-feature -packed-fp32-ops): the kernels are written for scalar FP32
@@ -16,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = [os.path.join(CSRC, f)
-               for f in ("fdsp_kernels.hip", "fdsp_chan_groups.hip", "fdsp_capi.hip", "fdsp_serve.hip", "fdsp_fm.hip",
+               for f in ("fdsp_kernels.hip", "fdsp_chan_groups.hip", "fdsp_chan_seg.hip", "fdsp_capi.hip", "fdsp_serve.hip", "fdsp_fm.hip",
                          "fdsp_iir.hip", "fdsp_predefined.hip")]
 PREDEFINED_INC = os.path.join(CSRC, "fdsp_predefined_table.inc")
 HIP_LIB = os.path.join(HERE, "libfdsp_hip.so")
@@ -25,9 +29,10 @@ DROPIN_LIB = os.path.join(HERE, "libfalcon_dsp.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+OBJ_DIR = os.path.join(HERE, "_build")
 
 
 def _run(cmd):
@@ -48,9 +53,25 @@ def build_predefined_table():
     print(f"+ designed {n} predefined filters -> {os.path.relpath(PREDEFINED_INC, ROOT)}", flush=True)
 
 
-def build_hip():
+def build_hip(incremental: bool = False):
+    from concurrent.futures import ThreadPoolExecutor
     build_predefined_table()
-    _run([HIPCC, *HIP_FLAGS, *HIP_SOURCES, "-o", HIP_LIB])
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".inc"))]
+    headers.append(os.path.join(ROOT, "include", "fdsp.h"))
+    newest_header = max(os.path.getmtime(h) for h in headers)
+    objs, todo = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.basename(src).replace(".hip", ".o"))
+        objs.append(obj)
+        if (not incremental or not os.path.exists(obj)
+                or os.path.getmtime(obj) < max(os.path.getmtime(src), newest_header)):
+            todo.append((src, obj))
+    jobs = max(1, min(len(todo), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(_run, [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]) for src, obj in todo]:
+            f.result()
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", HIP_LIB])
 
 
 def build_dropin():
@@ -83,18 +104,15 @@ def build_dropin_test():
                   "-Wl,-rpath,$ORIGIN/../../libfalcon_dsp_amd"])
 
 
-def build_oracle():
-    """The CPU checker (oracle/) and, where the reference sources exist, oracle/_ref."""
-    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
-
-
-def build_all():
-    build_hip()
+def build_all(incremental: bool = False):
+    """The product: the HIP library, the C++ drop-in library and the C++ test
+    programs that drive it. The CPU checker under oracle/ is built by its own
+    Makefile (make -C oracle), never by this module."""
+    build_hip(incremental)
     build_dropin()
     build_dropin_test()
-    build_oracle()
 
 
 if __name__ == "__main__":
-    build_all()
+    build_all(incremental="--incremental" in sys.argv[1:])
     sys.exit(0)

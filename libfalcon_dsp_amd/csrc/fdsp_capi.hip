@@ -992,7 +992,17 @@ struct fdsp_channelizer_s {
     bool pos_dirty = false;         // device position behind the host's (eager calls since the last graph launch)
     uint32_t taps_total = 0, hist_len = 0;
     uint32_t seg_len = FDSP_TILING_STRIDED, block = 256;
-    bool bf16_planes = true;  // MFMA groups with 8-aligned windows on bf16 matrix cores (FDSP_MFMA_FP32=1: off)
+    // FDSP_CHAN_BATCHED_BF16X2: the group kernel's MFMA groups with 8-aligned
+    // windows run on bf16 hi/lo planes (about 16 significant bits)
+    bool bf16_planes = false;
+    // segment-major FP32 kernel (fdsp_chan_seg.hip): the batched form's kernel
+    // whenever every channel fits it (windows of at most 64 taps)
+    bool seg_ok = false;   // the segment plan fits this channel set
+    bool use_seg = false;  // launches use it (fdsp_channelizer_set_kernel)
+    uint32_t seg_len_in = 2048, seg_halo = 0;
+    std::vector<SegGroup> seg_groups;
+    std::vector<float> seg_btab;
+    DevBuf d_seg_groups, d_seg_btab, d_ptaps;
     // block-time per tile of each group, measured by the autotuner (empty:
     // the tile_cost estimate weighs the groups)
     std::vector<double> gcost;
@@ -1033,7 +1043,10 @@ int chan_upload_outs(fdsp_channelizer_s *h, void *const *outs)
     // batched mode: output pointers live in the device channel table
     bool same = h->stat_valid;
     for (size_t c = 0; c < h->ch.size(); ++c) {
-        void *o = h->ch[c].tail ? h->ch[c].d_scratch.p : outs[c];
+        // the group kernel writes a tail channel's complex samples to scratch
+        // for the discriminator launch after it; the segment kernel fuses the
+        // discriminator and writes the angles itself
+        void *o = h->ch[c].tail && !h->use_seg ? h->ch[c].d_scratch.p : outs[c];
         if (h->stat[c].out != o) {
             h->stat[c].out = o;
             same = false;
@@ -1193,6 +1206,163 @@ float bf16_to_float(uint16_t b)
     float f;
     std::memcpy(&f, &u, sizeof(f));
     return f;
+}
+
+// Work plan of the segment-major FP32 kernel. Channels with up 1, at most 32
+// taps and the same down form MFMA groups of 2..8 (a larger set of one down
+// splits into near-equal groups); every other channel is a one-channel VALU
+// group. Each channel's modulated taps are padded in front to its window
+// (32 or 64) per filter phase. Leaves use_seg false when a window exceeds 64
+// taps or the plan exceeds the kernel's limits (the group kernel then runs).
+void build_seg_plan(fdsp_channelizer_s *h, const std::vector<float2> &taps)
+{
+    const uint32_t C = (uint32_t)h->ch.size();
+    h->seg_ok = h->use_seg = false;
+    h->seg_groups.clear();
+    h->seg_btab.clear();
+    if (C > kSegMaxChan)
+        return;
+    uint32_t max_win = 32, reach = 31;
+    for (auto &k : h->ch) {
+        if (k.cpp > 64)
+            return;
+        const uint32_t win = k.cpp <= 32 ? 32u : 64u;
+        max_win = std::max(max_win, win);
+        // a discriminator tail recomputes the output before a segment's first:
+        // its window reaches ceil(down/up) inputs further back
+        reach = std::max(reach, win - 1 + (k.tail ? (uint32_t)ceil_div(k.down, k.up) : 0u));
+    }
+    if (reach > kSegMaxHalo)
+        return;
+    // padded modulated taps
+    std::vector<float2> ptaps;
+    for (uint32_t c = 0; c < C; ++c) {
+        ChanHost &k = h->ch[c];
+        ChanStatic &s = h->stat[c];
+        s.win = k.cpp <= 32 ? 32u : 64u;
+        s.poff = (uint32_t)ptaps.size();
+        s.prev[0] = k.d_prev[0].as<float2>();
+        s.prev[1] = k.d_prev[1].as<float2>();
+        for (uint32_t ph = 0; ph < k.up; ++ph) {
+            for (uint32_t t = 0; t < s.win - k.cpp; ++t)
+                ptaps.push_back(make_float2(0.f, 0.f));
+            for (uint32_t t = 0; t < k.cpp; ++t)
+                ptaps.push_back(taps[s.taps_off + ph * k.cpp + t]);
+        }
+    }
+    // MFMA groups by down
+    std::vector<bool> done(C, false);
+    std::vector<SegGroup> mfma, valu;
+    for (uint32_t c = 0; c < C; ++c) {
+        const ChanHost &k = h->ch[c];
+        if (done[c] || k.up != 1 || k.cpp > 32 || k.tail)
+            continue;
+        std::vector<uint32_t> same;
+        for (uint32_t d = c; d < C; ++d) {
+            const ChanHost &o = h->ch[d];
+            if (!done[d] && o.up == 1 && o.cpp <= 32 && !o.tail && o.down == k.down)
+                same.push_back(d);
+        }
+        if (same.size() < 2)
+            continue;
+        const size_t n_g = (same.size() + kMfmaMaxChan - 1) / kMfmaMaxChan;
+        for (size_t g = 0, at = 0; g < n_g; ++g) {
+            const size_t take = (same.size() - at) / (n_g - g);
+            SegGroup gr{};
+            gr.kind = kSegMfma;
+            gr.m = (uint32_t)take;
+            gr.up = 1;
+            gr.down = k.down;
+            gr.win = 32;
+            gr.btab = (uint32_t)h->seg_btab.size();
+            for (size_t i = 0; i < take; ++i) {
+                gr.chan[i] = same[at + i];
+                done[same[at + i]] = true;
+            }
+            // B operand of K step s in lane (col, g): window position
+            // t = 8g + s/2, part s%2 (0 real, 1 imaginary); column 2c gives
+            // channel c's real output (Gr xr - Gi xi), 2c+1 its imaginary
+            // output (Gi xr + Gr xi)
+            for (uint32_t st = 0; st < 16; ++st)
+                for (uint32_t lane = 0; lane < 64; ++lane) {
+                    const uint32_t col = lane % 16, t = 8 * (lane / 16) + st / 2, part = st % 2;
+                    float v = 0.f;
+                    if (col / 2 < gr.m) {
+                        const ChanStatic &cs = h->stat[gr.chan[col / 2]];
+                        const float2 G = ptaps[cs.poff + t];
+                        if (col % 2 == 0)
+                            v = part == 0 ? G.x : -G.y;
+                        else
+                            v = part == 0 ? G.y : G.x;
+                    }
+                    h->seg_btab.push_back(v);
+                }
+            mfma.push_back(gr);
+            at += take;
+        }
+    }
+    for (uint32_t c = 0; c < C; ++c) {
+        if (done[c])
+            continue;
+        SegGroup gr{};
+        gr.kind = kSegValu;
+        gr.m = 1;
+        gr.up = h->ch[c].up;
+        gr.down = h->ch[c].down;
+        gr.win = h->stat[c].win;
+        gr.tail = h->ch[c].tail;
+        gr.chan[0] = c;
+        valu.push_back(gr);
+    }
+    // MFMA groups first (largest output share first), then the VALU channels
+    std::stable_sort(mfma.begin(), mfma.end(), [](const SegGroup &a, const SegGroup &b) { return a.down < b.down; });
+    h->seg_groups = mfma;
+    h->seg_groups.insert(h->seg_groups.end(), valu.begin(), valu.end());
+    if (h->seg_groups.size() > kSegMaxGroups)
+        return;
+    // staged history: every window's reach back, rounded to 16 samples
+    h->seg_halo = (reach + 15) & ~15u;
+    if (h->d_seg_groups.ensure(h->seg_groups.size() * sizeof(SegGroup)) ||
+        h->d_seg_btab.ensure(std::max<size_t>(h->seg_btab.size(), 1) * sizeof(float)) ||
+        h->d_ptaps.ensure(ptaps.size() * sizeof(float2)))
+        return;
+    if (hipMemcpy(h->d_seg_groups.p, h->seg_groups.data(), h->seg_groups.size() * sizeof(SegGroup),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        (!h->seg_btab.empty() && hipMemcpy(h->d_seg_btab.p, h->seg_btab.data(), h->seg_btab.size() * sizeof(float),
+                                           hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(h->d_ptaps.p, ptaps.data(), ptaps.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+        return;
+    h->seg_ok = h->use_seg = true;
+}
+
+// launch arguments of the segment kernel for a chunk (device_pos: graph mode,
+// position and parity from the device record)
+SegArgs seg_args(const fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n, bool device_pos,
+                 const ChanStatic *chans)
+{
+    SegArgs a{};
+    a.in = reinterpret_cast<const float2 *>(d_in);
+    a.n = n;
+    a.seg = h->seg_len_in;
+    a.halo = h->seg_halo;
+    a.hist_len = h->hist_len;
+    a.hist[0] = h->d_hist[0].as<float2>();
+    a.hist[1] = h->d_hist[1].as<float2>();
+    a.hist_w[0] = h->d_hist[0].as<float2>();
+    a.hist_w[1] = h->d_hist[1].as<float2>();
+    a.chans = chans;
+    a.ptaps = h->d_ptaps.as<float2>();
+    a.btab = h->d_seg_btab.as<float>();
+    a.groups = h->d_seg_groups.as<SegGroup>();
+    a.n_groups = (uint32_t)h->seg_groups.size();
+    a.n_chan = (uint32_t)h->ch.size();
+    a.pos = device_pos ? h->d_pos.as<ChanPos>() : nullptr;
+    a.p = h->p;
+    a.parity = h->parity;
+    a.out16 = true;
+    for (const ChanStatic &st : h->stat)
+        a.out16 = a.out16 && reinterpret_cast<uintptr_t>(st.out) % 16 == 0;
+    return a;
 }
 
 void build_groups(fdsp_channelizer_s *h, const std::vector<float2> &taps)
@@ -1591,7 +1761,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
 {
     if (!out || !chans || n_chan == 0 || fs == 0)
         return set_err(FDSP_EINVAL, "channelizer_create: need fs > 0 and at least one channel");
-    if (mode != FDSP_CHAN_PLAIN && mode != FDSP_CHAN_BATCHED)
+    if (mode != FDSP_CHAN_PLAIN && mode != FDSP_CHAN_BATCHED && mode != FDSP_CHAN_BATCHED_BF16X2)
         return set_err(FDSP_EINVAL, "unknown channelizer mode %d", mode);
     for (uint32_t c = 0; c < n_chan; ++c)
         if (chans[c].up_rate == 0 || chans[c].down_rate == 0 || !chans[c].taps || chans[c].n_taps == 0 ||
@@ -1601,7 +1771,8 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
     if (!h)
         return set_err(FDSP_ENOMEM, "host allocation failed");
     h->fs = fs;
-    h->mode = mode;
+    h->mode = mode == FDSP_CHAN_PLAIN ? FDSP_CHAN_PLAIN : FDSP_CHAN_BATCHED;
+    h->bf16_planes = mode == FDSP_CHAN_BATCHED_BF16X2;
     h->ch.resize(n_chan);
     int rc = FDSP_OK;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
@@ -1622,7 +1793,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             if (!rc)
                 rc = k.d_prev[1].ensure(sizeof(float2));
         }
-        if (!rc && mode == FDSP_CHAN_PLAIN) {
+        if (!rc && h->mode == FDSP_CHAN_PLAIN) {
             const size_t hb = (size_t)(k.cpp > 1 ? k.cpp - 1 : 1) * sizeof(float2);
             rc = k.d_bank.ensure(k.bank.size() * sizeof(float));
             if (!rc)
@@ -1636,7 +1807,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
                 rc = set_err(FDSP_EHIP, "channel %u: HIP setup failed", c);
         }
     }
-    if (!rc && mode == FDSP_CHAN_PLAIN) {
+    if (!rc && h->mode == FDSP_CHAN_PLAIN) {
         std::vector<NcoChan> nco;
         for (auto &k : h->ch)
             nco.push_back(k.nco);
@@ -1644,7 +1815,7 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
         if (!rc && hipMemcpy(h->d_nco.p, nco.data(), sizeof(NcoChan) * n_chan, hipMemcpyHostToDevice) != hipSuccess)
             rc = set_err(FDSP_EHIP, "NCO table upload failed");
     }
-    if (!rc && mode == FDSP_CHAN_BATCHED) {
+    if (!rc && h->mode == FDSP_CHAN_BATCHED) {
         // modulated taps of every channel, back to back
         std::vector<float2> taps;
         h->stat.resize(n_chan);
@@ -1671,8 +1842,6 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
         uint32_t max_w = max_cpp;
         for (auto &k : h->ch)
             max_w = std::max(max_w, mfma_window(k));
-        if (const char *e = std::getenv("FDSP_MFMA_FP32"))
-            h->bf16_planes = e[0] == '0';
         if (const char *e = std::getenv("FDSP_FUSE"))
             h->fuse = e[0] != '0';
         build_groups(h, taps);
@@ -1682,6 +1851,9 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             if (h->groups[g * kGroupWords + 7])
                 max_w = std::max(max_w, h->groups[g * kGroupWords + 3] + 8);
         h->hist_len = (max_w + 1) & ~1u;
+        if (!h->bf16_planes)
+            build_seg_plan(h, taps);
+        h->hist_len = std::max(h->hist_len, h->seg_halo);
         if (h->n_groups > kGroupsInline)
             rc = set_err(FDSP_ERANGE, "batched channelizer: %u channel groups exceed %u; use FDSP_CHAN_PLAIN",
                          h->n_groups, kGroupsInline);
@@ -1836,7 +2008,9 @@ extern "C" int fdsp_channelizer_needed_out_count(fdsp_channelizer_t h, uint32_t 
         return set_err(FDSP_EINVAL, "bad channel");
     const ChanHost &k = h->ch[chan];
     uint32_t cnt = chunk_count(h->p, in_count, k.up, k.down);
-    if (k.tail && cnt && !k.has_prev)
+    // the stream's first output (emitted with its first input) has no
+    // predecessor for the discriminator
+    if (k.tail && cnt && h->p == 0)
         --cnt;
     *out_count = cnt;
     return FDSP_OK;
@@ -1894,7 +2068,8 @@ extern "C" int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf
         const ChanHost &k = h->ch[c];
         raw[c] = chunk_count(h->p, n, k.up, k.down);
         size_t emit = raw[c];
-        if (k.tail && emit && !k.has_prev)
+        // the stream's first output has no predecessor for the discriminator
+        if (k.tail && emit && h->p == 0)
             --emit;
         if (emit > caps[c] || (emit && !d_outs[c]))
             return set_err(FDSP_ERANGE, "channel %zu: capacity %zu below the %zu samples this chunk emits", c,
@@ -1902,7 +2077,7 @@ extern "C" int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf
         counts[c] = emit;
     }
     for (size_t c = 0; c < C; ++c)
-        if (h->ch[c].tail)
+        if (h->ch[c].tail && !(h->mode == FDSP_CHAN_BATCHED && h->use_seg))
             if (int rc = h->ch[c].d_scratch.ensure((size_t)(raw[c] ? raw[c] : 1) * sizeof(float2)))
                 return rc;
 
@@ -1931,18 +2106,22 @@ extern "C" int fdsp_channelizer_apply_device(fdsp_channelizer_t h, const fdsp_cf
             return rc;
         }
         // eager form: position and parity travel as kernel arguments, one launch
-        GroupsArgs f;
-        if (int rc = groups_args(h, d_in, (uint32_t)n, false, &f))
-            return rc;
-        HIP_TRY(launch_chan_groups(f, s));
+        if (h->use_seg) {
+            HIP_TRY(launch_chan_seg(seg_args(h, d_in, (uint32_t)n, false, h->d_stat.as<ChanStatic>()), s));
+        } else {
+            GroupsArgs f;
+            if (int rc = groups_args(h, d_in, (uint32_t)n, false, &f))
+                return rc;
+            HIP_TRY(launch_chan_groups(f, s));
+        }
         HIP_TRY(hipEventRecord(h->use_ev, s));
         h->parity ^= 1;
         h->pos_dirty = true;
     }
-    // discriminator tails
+    // discriminator tails (fused into the segment kernel)
     for (size_t c = 0; c < C; ++c) {
         ChanHost &k = h->ch[c];
-        if (!k.tail || raw[c] == 0)
+        if (!k.tail || raw[c] == 0 || (h->mode == FDSP_CHAN_BATCHED && h->use_seg))
             continue;
         HIP_TRY(launch_discriminator(k.d_scratch.as<float2>(), raw[c], k.d_prev[k.prev_parity].as<float2>(),
                                      k.has_prev ? 1 : 0, k.d_prev[k.prev_parity ^ 1].as<float2>(),
@@ -2021,8 +2200,8 @@ extern "C" int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_
     uint32_t max_up = 1;
     for (size_t c = 0; c < C; ++c) {
         const ChanHost &k = h->ch[c];
-        if (k.tail)
-            return set_err(FDSP_EINVAL, "graph mode does not take discriminator-tail channels");
+        if (k.tail && !h->use_seg)
+            return set_err(FDSP_EINVAL, "graph mode takes discriminator-tail channels on the segment kernel only");
         max_up = std::max(max_up, k.up);
         // any chunk of chunk_len inputs emits at most ceil(chunk_len*up/down) outputs
         const uint64_t most = ceil_div((uint64_t)chunk_len * k.up, k.down);
@@ -2041,15 +2220,22 @@ extern "C" int fdsp_channelizer_graph_create(fdsp_channelizer_t h, size_t chunk_
         return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(h->d_stat_graph.p, gstat.data(), sizeof(ChanStatic) * C, hipMemcpyHostToDevice));
-    GroupsArgs f;
-    if (int rc = groups_args(h, d_in, (uint32_t)chunk_len, true, &f))
-        return rc;
-    f.chans = h->d_stat_graph.as<ChanStatic>();
-    f.out16 = true;
+    GroupsArgs f{};
+    SegArgs sa{};
+    bool out16 = true;
     for (const ChanStatic &st : gstat)
-        f.out16 = f.out16 && reinterpret_cast<uintptr_t>(st.out) % 16 == 0;
+        out16 = out16 && reinterpret_cast<uintptr_t>(st.out) % 16 == 0;
+    if (h->use_seg) {
+        sa = seg_args(h, d_in, (uint32_t)chunk_len, true, h->d_stat_graph.as<ChanStatic>());
+        sa.out16 = out16;
+    } else {
+        if (int rc = groups_args(h, d_in, (uint32_t)chunk_len, true, &f))
+            return rc;
+        f.chans = h->d_stat_graph.as<ChanStatic>();
+        f.out16 = out16;
+    }
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = launch_chan_groups(f, h->stream);
+    hipError_t e1 = h->use_seg ? launch_chan_seg(sa, h->stream) : launch_chan_groups(f, h->stream);
     hipError_t e2 = launch_chan_advance(h->d_pos.as<ChanPos>(), (uint32_t)chunk_len, h->stream);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(h->stream, &g);
@@ -2077,8 +2263,12 @@ extern "C" int fdsp_channelizer_graph_launch(fdsp_channelizer_t h, fdsp_stream_t
         HIP_TRY(hipMemcpy(h->d_pos.p, &pos, sizeof(pos), hipMemcpyHostToDevice));
         h->pos_dirty = false;
     }
-    for (size_t c = 0; c < h->ch.size(); ++c)
-        h->graph_counts[c] = chunk_count(h->p, h->graph_chunk, h->ch[c].up, h->ch[c].down);
+    for (size_t c = 0; c < h->ch.size(); ++c) {
+        size_t cnt = chunk_count(h->p, h->graph_chunk, h->ch[c].up, h->ch[c].down);
+        if (h->ch[c].tail && cnt && h->p == 0)
+            --cnt;  // the stream's first output has no predecessor
+        h->graph_counts[c] = cnt;
+    }
     HIP_TRY(hipGraphLaunch(h->graph_exec, s));
     h->p += h->graph_chunk;
     h->parity ^= 1;
@@ -2102,6 +2292,15 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
         return set_err(FDSP_ENOTINIT, "null handle");
     if (h->mode != FDSP_CHAN_BATCHED)
         return set_err(FDSP_EINVAL, "tiling applies to the batched mode");
+    if (h->use_seg) {
+        // segment kernel: the segment length is the only parameter
+        if (segment_len != 1024 && segment_len != 2048 && segment_len != 4096)
+            return set_err(FDSP_EINVAL, "the segment kernel takes segment_len 1024, 2048 or 4096");
+        if (h->graph_exec)
+            return set_err(FDSP_EBUSY, "destroy the graph before retiling");
+        h->seg_len_in = segment_len;
+        return FDSP_OK;
+    }
     if (segment_len > FDSP_TILING_STRIDED &&
         (segment_len < 256 || segment_len > (1u << 20) || (segment_len & (segment_len - 1))))
         return set_err(FDSP_EINVAL, "segment_len must be FDSP_TILING_CLAIM, FDSP_TILING_STRIDED or a power of two in "
@@ -2132,6 +2331,17 @@ extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len
         return set_err(FDSP_ENOTINIT, "null handle");
     if (h->mode != FDSP_CHAN_BATCHED)
         return set_err(FDSP_EINVAL, "plan info applies to the batched mode");
+    if (h->use_seg) {
+        if (blocks)
+            *blocks = (uint32_t)ceil_div(chunk_len, h->seg_len_in);
+        if (capacity)
+            *capacity = h->capacity;
+        if (groups)
+            *groups = (uint32_t)h->seg_groups.size();
+        if (lds_bytes)
+            *lds_bytes = (uint32_t)chan_seg_lds_bytes(h->seg_len_in, h->seg_halo);
+        return FDSP_OK;
+    }
     std::vector<uint32_t> t;
     block_table(h, (uint32_t)chunk_len, true, &t);
     if (blocks)
@@ -2182,6 +2392,51 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
     }
     if (int rc = chan_upload_outs(h, ptrs.data()))
         return rc;
+    if (h->use_seg) {
+        // segment kernel: time each segment length, keep the fastest
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        HIP_TRY(hipEventCreate(&t0));
+        if (hipEventCreate(&t1) != hipSuccess) {
+            (void)hipEventDestroy(t0);
+            return set_err(FDSP_EHIP, "event creation failed");
+        }
+        static const uint32_t kSegCand[] = {2048, 1024, 4096};
+        const uint32_t seg0 = h->seg_len_in;
+        float best = -1.f;
+        uint32_t best_seg = seg0;
+        size_t k = 0;
+        int rc = FDSP_OK;
+        for (uint32_t L : kSegCand) {
+            h->seg_len_in = L;
+            const SegArgs sa = seg_args(h, d_in, n, false, h->d_stat.as<ChanStatic>());
+            float ms = 0.f;
+            bool ok = launch_chan_seg(sa, h->stream) == hipSuccess && hipEventRecord(t0, h->stream) == hipSuccess;
+            for (uint32_t r = 0; ok && r < reps; ++r)
+                ok = launch_chan_seg(sa, h->stream) == hipSuccess;
+            ok = ok && hipEventRecord(t1, h->stream) == hipSuccess && hipEventSynchronize(t1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, t0, t1) == hipSuccess;
+            if (!ok) {
+                rc = set_err(FDSP_EHIP, "autotune launch failed");
+                break;
+            }
+            const float us = 1e3f * ms / (float)reps;
+            if (best < 0.f || us < best) {
+                best = us;
+                best_seg = L;
+            }
+            if (trials && k < cap)
+                trials[k] = fdsp_tiling_trial{L, 256u, us, 4u};
+            ++k;
+        }
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        h->stat_valid = false;
+        h->seg_len_in = rc ? seg0 : best_seg;
+        if (n_trials)
+            *n_trials = k;
+        return rc;
+    }
     // candidates: {split, threads, group weights measured (1) or estimated (0)}
     static const uint32_t kCand[][3] = {
         {FDSP_TILING_STRIDED, 256, 0}, {FDSP_TILING_STRIDED, 512, 0}, {FDSP_TILING_STRIDED, 128, 0},
@@ -2351,8 +2606,57 @@ extern "C" int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segme
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
     if (segment_len)
-        *segment_len = h->seg_len;
+        *segment_len = h->use_seg ? h->seg_len_in : h->seg_len;
     if (block_threads)
-        *block_threads = h->block;
+        *block_threads = h->use_seg ? 256u : h->block;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_set_kernel(fdsp_channelizer_t h, uint32_t kernel)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED || h->bf16_planes)
+        return set_err(FDSP_EINVAL, "set_kernel applies to the FP32 batched mode");
+    if (kernel != FDSP_KERNEL_SEGMENT_F32 && kernel != FDSP_KERNEL_GROUPS_F32)
+        return set_err(FDSP_EINVAL, "kernel must be FDSP_KERNEL_SEGMENT_F32 or FDSP_KERNEL_GROUPS_F32");
+    if (kernel == FDSP_KERNEL_SEGMENT_F32 && !h->seg_ok)
+        return set_err(FDSP_EINVAL, "the segment kernel does not fit this channel set (a window above 64 taps)");
+    if (h->graph_exec)
+        return set_err(FDSP_EBUSY, "destroy the graph before switching kernels");
+    for (auto &k : h->ch)
+        if (k.tail && h->p != 0)
+            return set_err(FDSP_EBUSY, "discriminator tails: switch kernels before the first chunk or after a reset");
+    // launches already queued on caller streams use the current tables
+    HIP_TRY(hipDeviceSynchronize());
+    h->use_seg = kernel == FDSP_KERNEL_SEGMENT_F32;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_channelizer_kernel_info(fdsp_channelizer_t h, uint32_t *kernel, uint32_t *groups,
+                                            uint32_t *mfma_groups)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    uint32_t kind = FDSP_KERNEL_PLAIN, ng = 0, nm = 0;
+    if (h->mode == FDSP_CHAN_BATCHED) {
+        if (h->use_seg) {
+            kind = FDSP_KERNEL_SEGMENT_F32;
+            ng = (uint32_t)h->seg_groups.size();
+            for (const SegGroup &g : h->seg_groups)
+                nm += g.kind == kSegMfma ? 1u : 0u;
+        } else {
+            kind = h->bf16_planes ? FDSP_KERNEL_GROUPS_BF16X2 : FDSP_KERNEL_GROUPS_F32;
+            ng = h->n_groups;
+            for (uint32_t g = 0; g < h->n_groups; ++g)
+                nm += h->groups[g * kGroupWords] == kGroupMfma || h->groups[g * kGroupWords] == kGroupFused ? 1u : 0u;
+        }
+    }
+    if (kernel)
+        *kernel = kind;
+    if (groups)
+        *groups = ng;
+    if (mfma_groups)
+        *mfma_groups = nm;
     return FDSP_OK;
 }

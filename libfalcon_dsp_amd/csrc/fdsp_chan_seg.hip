@@ -1,0 +1,506 @@
+// fdsp_chan_seg.hip -- segment-major fused multi-rate channelizer in FP32.
+// (Replaces falcon_dsp_multi_rate_channelizer{,_cuda}::apply, reference
+//  src/transform/falcon_dsp_multi_rate_channelizer.cc:146-164 and
+//  src/transform/falcon_dsp_multi_rate_channelizer_cuda.cu:375-514, whose
+//  per-channel work is falcon_dsp_freq_shift.cc:164-192 followed by
+//  falcon_dsp_polyphase_resampler.cc:256-319.)
+//
+// One workgroup owns one segment of L consecutive inputs of the chunk. It
+// stages the segment, plus the H samples before it, in LDS once, then
+// computes every channel's outputs whose anchors fall inside the segment.
+// The input is read from HBM once for all channels, every workgroup carries
+// the same mix of work, and no arithmetic is narrower than FP32.
+//
+// Shift-then-filter equals filter-with-modulated-taps then one phasor per
+// output (k(a): the NCO index of anchor a, d_t = cpp-1-t):
+//   y = e^{j2pi nu k(a)} * sum_t [Re(h_t) e^{-j2pi nu d_t}] x[a - d_t]
+// as long as no rollover (the index restarting at 0 after R-1) falls inside
+// the window. When one does (k(a) < cpp-1), the samples before it carry an
+// extra e^{j2pi nu R}, added as (e^{j2pi nu R} - 1) times their partial sum.
+//
+// Work groups (SegGroup records, built on the host):
+//   MFMA  2..8 channels with up 1, equal down and at most 32 taps. Their
+//         32-sample windows (taps zero-padded in front) are the A operand of
+//         v_mfma_f32_16x16x4f32: 16 outputs x 64 real K values, window sample
+//         8g + s/2 of row i in lane i + 16g at K step s, its real part on
+//         even steps and its imaginary part on odd ones. The modulated taps
+//         are the B operand: channel c's real output in column 2c, its
+//         imaginary output in 2c+1. A sub-tile is 16 outputs of every channel
+//         of the group.
+//   VALU  one channel (any up), window of at most 64 taps: one output per
+//         lane, taps zero-padded in front to kW = 32 or 64. With up 1 the taps
+//         are wavefront-uniform scalar loads; with up > 1 each lane reads the
+//         bank row of its filter phase (n*down) mod up.
+//
+// LDS layout of the staged span: span sample q at slot q + q/16 (an 8-byte
+// pad every 16 samples), so the 16 rows of an MFMA sub-tile, 8 samples
+// apart at down 8, fall on distinct banks.
+#include "fdsp_internal.h"
+
+namespace fdsp {
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) fvec2 *gvec2_ptr;
+typedef __attribute__((address_space(1))) fvec4 *gvec4_ptr;
+// records read at wavefront-uniform addresses: scalar (SMEM) loads
+template <class T> using cptr = const __attribute__((address_space(4))) T *;
+template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)p; }
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kWaves = kBlock / 64;
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t slot(uint32_t q) { return q + (q >> 4); }
+__device__ __forceinline__ float2 lds_x(const float2 *xs, uint32_t q) { return xs[slot(q)]; }
+
+// floor(a / d) for a < 2^52 from an FP64 quotient estimate and one exact
+// correction step (inv_d = 1/d rounded to double)
+__device__ __forceinline__ uint64_t udiv64(uint64_t a, uint32_t d, double inv_d)
+{
+    int64_t q = (int64_t)((double)a * inv_d);
+    const int64_t r = (int64_t)a - q * (int64_t)d;
+    if (r < 0)
+        --q;
+    else if (r >= (int64_t)d)
+        ++q;
+    return (uint64_t)q;
+}
+
+// Stream bookkeeping of one channel for this chunk (global input position
+// P), the same for every block but j_lo / j_hi: the chunk's outputs start at
+// stream output n0 = ceil(P*up/down); its output j has chunk-local anchor
+// floor((j*down + rho)/up) with rho = n0*down - up*P in [0, down); k0 is the
+// NCO index of output 0's anchor; outputs j_lo .. j_hi-1 have their anchors in
+// this block's segment, and the chunk emits j_end outputs in all.
+struct ChanRun {
+    uint32_t rho;
+    uint32_t k0;
+    uint32_t j_lo, j_hi;
+    uint32_t j_end;
+    uint32_t pad[3];
+};
+
+__device__ __forceinline__ ChanRun chan_run(const ChanStatic &c, uint64_t P, uint32_t lo, uint32_t hi, uint32_t n)
+{
+    const uint64_t up = c.up, down = c.down;
+    const uint64_t uP = up * P;
+    const uint64_t n0 = udiv64(uP + down - 1, c.down, c.inv_down);
+    ChanRun r;
+    r.rho = (uint32_t)(n0 * down - uP);
+    const auto first_at = [&](uint32_t x) -> uint32_t {
+        const uint64_t ux = up * x;
+        return ux > r.rho ? (uint32_t)udiv64(ux - r.rho + down - 1, c.down, c.inv_down) : 0u;
+    };
+    r.j_lo = first_at(lo);
+    r.j_hi = first_at(hi);
+    r.j_end = first_at(n);
+    r.k0 = (uint32_t)mod_u64(P + r.rho / c.up, c.r, c.inv_r);
+    return r;
+}
+
+// The lane's channel, for the epilogue of its outputs
+struct LaneChan {
+    double nu, inv_r;
+    uint32_t r, win, poff;
+    float2 wrap_m1;  // e^{j2pi nu R} - 1
+    gvec2_ptr out;
+};
+
+__device__ __forceinline__ LaneChan lane_chan(const ChanStatic &c)
+{
+    LaneChan l;
+    l.nu = c.nu;
+    l.inv_r = c.inv_r;
+    l.r = c.r;
+    l.win = c.win;
+    l.poff = c.poff;
+    l.wrap_m1 = make_float2(c.wrap.x - 1.f, c.wrap.y);
+    l.out = (gvec2_ptr)c.out;
+    return l;
+}
+
+struct SegDev {
+    const float2 *in;
+    uint32_t n;
+    uint32_t halo;      // H: staged samples before the segment (>= every window - 1, multiple of 16)
+    uint32_t hist_len;  // samples in each history buffer (>= H)
+    const float2 *hist0, *hist1;
+    float2 *histw0, *histw1;
+    const ChanStatic *chans;
+    const float2 *ptaps;  // per channel: up phases x win padded modulated taps, from ChanStatic::poff
+    const float *btab;    // MFMA B operands, 16 steps x 64 lanes per group
+    const SegGroup *groups;
+    uint32_t n_groups;
+    uint32_t n_chan;
+    const ChanPos *pos;  // device position (graph mode), or null: p and parity below
+    uint64_t p;
+    uint32_t parity;
+    uint32_t out16;      // every channel's output is 16-byte aligned
+};
+
+// the chunk's position, from the device record in graph mode
+struct SegPos {
+    uint64_t P;
+    uint32_t parity;
+};
+
+// y (filtered, not yet phased) -> the output of anchor NCO index k: rollover
+// correction when the window straddles a restart (window positions t <
+// win-1-k precede it; taps of phase ph; q: the window's first span sample),
+// then the phasor.
+__device__ __forceinline__ float2 finish_out(const SegDev &a, const LaneChan &c, uint32_t ph, const float2 *xs,
+                                             uint32_t q, uint32_t k, float2 y)
+{
+    if (k + 1 < c.win) {
+        const float2 *g = a.ptaps + c.poff + ph * c.win;
+        float2 part = make_float2(0.f, 0.f);
+        for (uint32_t t = 0; t + 1 + k < c.win; ++t)
+            cmac(part, g[t], lds_x(xs, q + t));
+        cmac(y, c.wrap_m1, part);
+    }
+    return cmul(nco_phasor((int64_t)k, c.nu), y);
+}
+
+// ---------------------------------------------------------------------------
+// MFMA group: sub-tile st (segment rows 16 st ..) of every channel of the group
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mfma_subtile(const SegDev &a, uint32_t down, uint32_t m, const ChanRun &run,
+                                             uint32_t k0, const LaneChan &lc, const float (&bv)[16],
+                                             const float2 *xs, uint32_t span_lo, uint32_t st, uint32_t lane)
+{
+    const uint32_t n_rows = run.j_hi - run.j_lo;
+    const uint32_t r0 = st * 16;
+    // A operand: row lane%16 (clamped to the last real row), window samples
+    // 8*(lane/16) .. +7
+    const uint32_t ri = min(r0 + (lane & 15), n_rows - 1);
+    const uint32_t qa = (run.j_lo + ri) * down + run.rho - 31 - span_lo + 8 * (lane >> 4);
+    float2 x[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+        x[j] = lds_x(xs, qa + j);
+    floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].x, bv[2 * j], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].y, bv[2 * j + 1], c1, 0, 0, 0);
+    }
+    const floatx4 acc = c0 + c1;
+    // D[i][col] sits in lane col + 16*(i/4), element i%4: lanes 2c and 2c+1
+    // hold channel c's real and imaginary parts of rows 4*(lane/16) .. +3.
+    // The even lane finishes the first two of those rows, the odd lane the
+    // last two (quad_perm 1,0,3,2 swaps the pair's values).
+    float oth[4];
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e)
+        oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
+    const uint32_t col = lane & 15;
+    const bool odd = col & 1;
+    const float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
+    const float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
+    const uint32_t rr = r0 + 4 * (lane >> 4) + (odd ? 2 : 0);  // segment row of y0
+    if ((col >> 1) >= m || rr >= n_rows)
+        return;
+    const uint32_t j = run.j_lo + rr;                           // chunk output index of y0
+    const uint32_t q0 = j * down + run.rho - 31 - span_lo;      // window start of y0's row
+    const uint32_t R = lc.r;
+    const uint32_t kk0 = (uint32_t)mod_u64((uint64_t)k0 + (uint64_t)j * down, R, lc.inv_r);
+    uint32_t kk1 = kk0 + down;
+    if (kk1 >= R)
+        kk1 = (uint32_t)mod_u64((uint64_t)kk1, R, lc.inv_r);
+    const float2 o0 = finish_out(a, lc, 0, xs, q0, kk0, y0);
+    const float2 o1 = finish_out(a, lc, 0, xs, q0 + down, kk1, y1);
+    if (rr + 1 < n_rows) {
+        if (a.out16 && (j & 1) == 0)
+            ((gvec4_ptr)(lc.out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+        else {
+            lc.out[j] = fvec2{o0.x, o0.y};
+            lc.out[j + 1] = fvec2{o1.x, o1.y};
+        }
+    } else {
+        lc.out[j] = fvec2{o0.x, o0.y};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// VALU channel: batch b of one channel's outputs in the segment
+// ---------------------------------------------------------------------------
+// Without a tail, lane l computes segment row 64 b + l. With a discriminator
+// tail (reference src/transform/falcon_dsp_polar_discriminator.cc:134-171),
+// batch b covers rows 63 b - 1 .. 63 b + 62: lane 0 recomputes the row before
+// the batch (the segment's row -1 sits in the halo, or, at the chunk's first
+// output, is the carried last sample of the previous chunk), so every other
+// lane takes y[j-1] from its neighbour and emits angle(y[j] conj(y[j-1])).
+// The stream's first output has no predecessor and emits nothing; the
+// chunk's last output is carried to the next chunk.
+template <uint32_t kW>
+__device__ __forceinline__ void valu_batch(const SegDev &a, uint32_t ci, bool tail, const ChanRun &run,
+                                           const SegPos &pos, const float2 *xs, uint32_t span_lo, uint32_t b,
+                                           uint32_t lane)
+{
+    const cptr<ChanStatic> cs = as_const(a.chans) + ci;
+    const uint32_t up = cs->up, down = cs->down;
+    const int32_t n_rows = (int32_t)(run.j_hi - run.j_lo);
+    const int32_t rr = tail ? (int32_t)(b * 63 + lane) - 1 : (int32_t)(b * 64 + lane);
+    // the row computed: rows past the segment's last take the last; row -1 of
+    // the chunk's first segment (no window: the carried sample replaces it)
+    // takes row 0
+    const int32_t rc = min(rr, n_rows - 1);
+    const uint32_t j = (uint32_t)((int32_t)run.j_lo + (rc < 0 && run.j_lo == 0 ? 0 : rc));
+    float2 acc = make_float2(0.f, 0.f);
+    uint32_t anchor, ph;
+    if (up == 1) {
+        anchor = j * down + run.rho;
+        ph = 0;
+        const uint32_t q = anchor + 1 - kW - span_lo;
+        const cptr<float> g = (cptr<float>)(a.ptaps + cs->poff);
+#pragma unroll
+        for (uint32_t t = 0; t < kW; ++t)
+            cmac(acc, make_float2(g[2 * t], g[2 * t + 1]), lds_x(xs, q + t));
+    } else {
+        const uint64_t num = (uint64_t)j * down + run.rho;
+        anchor = (uint32_t)udiv64(num, up, cs->inv_up);
+        ph = (uint32_t)(num - (uint64_t)anchor * up);
+        const uint32_t q = anchor + 1 - kW - span_lo;
+        const float2 *g = a.ptaps + cs->poff + ph * kW;
+#pragma unroll 8
+        for (uint32_t t = 0; t < kW; ++t)
+            cmac(acc, g[t], lds_x(xs, q + t));
+    }
+    const LaneChan lc = lane_chan(a.chans[ci]);
+    // NCO index of the anchor; output 0's anchor is floor(rho/up)
+    const uint32_t k = (uint32_t)mod_u64((uint64_t)run.k0 + (anchor - run.rho / up), lc.r, lc.inv_r);
+    float2 y = finish_out(a, lc, ph, xs, anchor + 1 - kW - span_lo, k, acc);
+    if (!tail) {
+        if (rr < n_rows)
+            lc.out[j] = fvec2{y.x, y.y};
+        return;
+    }
+    // discriminator tail
+    const bool first_row = rr < 0 && run.j_lo == 0;  // the previous chunk's last output
+    if (first_row && pos.P > 0)
+        y = cs->prev[pos.parity][0];
+    const float2 yp = make_float2(__shfl_up(y.x, 1), __shfl_up(y.y, 1));
+    // lane 0 only supplies its row (the previous batch's last) to lane 1
+    if (lane == 0 || rr >= n_rows)
+        return;
+    // the chunk's last output is the next chunk's predecessor
+    if (j + 1 == run.j_end)
+        cs->prev[pos.parity ^ 1u][0] = y;
+    // output j of the chunk has a predecessor unless it is the stream's first
+    const uint32_t drop = pos.P == 0 ? 1u : 0u;
+    if (j < drop)
+        return;
+    const float2 z = cmul(y, make_float2(yp.x, -yp.y));
+    ((__attribute__((address_space(1))) float *)lc.out)[j - drop] = atan2f(z.y, z.x);
+}
+
+// ---------------------------------------------------------------------------
+// the kernel: one segment per workgroup
+// ---------------------------------------------------------------------------
+template <uint32_t kSeg>
+__global__ __launch_bounds__(kBlock) void chan_seg_kernel(SegDev a)
+{
+    extern __shared__ float2 xs[];
+    __shared__ ChanRun runs[kSegMaxChan];
+    __shared__ uint32_t item0[kSegMaxGroups + 1];
+
+    SegPos pos{a.p, a.parity};
+    if (a.pos) {
+        pos.P = a.pos->p;
+        pos.parity = a.pos->parity;
+    }
+    const uint64_t P = pos.P;
+    const uint32_t parity = pos.parity;
+    const float2 *hist = parity ? a.hist1 : a.hist0;
+    const uint32_t seg_lo = blockIdx.x * kSeg;  // chunk-local first input of the segment
+    const uint32_t seg_hi = min(a.n, seg_lo + kSeg);
+    const uint32_t H = a.halo;
+    // chunk-local index of span sample 0 (wraps below zero in the first
+    // segment; window starts q = anchor - (w-1) - span_lo stay >= 0 in
+    // modular arithmetic since H >= w-1)
+    const uint32_t span_lo = seg_lo - H;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uniform(tid >> 6);
+
+    // 1. stage the span (chunk-local inputs seg_lo - H .. seg_hi - 1, the part
+    // before the chunk from the history), two samples per thread and load
+    {
+        const int64_t base = (int64_t)seg_lo - (int64_t)H;
+        const uint32_t pairs = (seg_hi - seg_lo + H + 1) / 2;
+        const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15) == 0;
+        constexpr uint32_t kRounds = (kSeg + kSegMaxHalo) / 2 / kBlock + 1;
+        float4 v[kRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kRounds; ++r) {
+            const uint32_t pr = tid + r * kBlock;
+            const int64_t i = base + 2 * (int64_t)pr;
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (pr < pairs) {
+                if (i >= 0) {
+                    if (in16 && i + 1 < (int64_t)a.n) {
+                        w = *reinterpret_cast<const float4 *>(a.in + i);
+                    } else {
+                        if (i < (int64_t)a.n) {
+                            const float2 u = a.in[i];
+                            w.x = u.x;
+                            w.y = u.y;
+                        }
+                        if (i + 1 < (int64_t)a.n) {
+                            const float2 u = a.in[i + 1];
+                            w.z = u.x;
+                            w.w = u.y;
+                        }
+                    }
+                } else {
+                    // i even and i + 1 <= -1: both samples from the history
+                    const float2 u0 = hist[a.hist_len + i], u1 = hist[a.hist_len + i + 1];
+                    w = make_float4(u0.x, u0.y, u1.x, u1.y);
+                }
+            }
+            v[r] = w;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kRounds; ++r) {
+            const uint32_t pr = tid + r * kBlock;
+            if (pr < pairs) {
+                const uint32_t q = 2 * pr;  // even: q and q+1 sit in one 16-sample row
+                xs[slot(q)] = make_float2(v[r].x, v[r].y);
+                xs[slot(q) + 1] = make_float2(v[r].z, v[r].w);
+            }
+        }
+    }
+
+    // 2. bookkeeping: one thread per channel, and the groups' work item
+    // counts scanned across wave 0
+    for (uint32_t c = tid; c < a.n_chan; c += kBlock)
+        runs[c] = chan_run(a.chans[c], P, seg_lo, seg_hi, a.n);
+    if (wave == 0) {
+        uint32_t cnt = 0;
+        if (lane < a.n_groups) {
+            const SegGroup &gr = a.groups[lane];
+            const ChanRun r = chan_run(a.chans[gr.chan[0]], P, seg_lo, seg_hi, a.n);
+            const uint32_t rows = r.j_hi - r.j_lo;
+            cnt = gr.kind == kSegMfma ? (rows + 15) / 16 : gr.tail ? (rows + 62) / 63 : (rows + 63) / 64;
+        }
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(cnt, d);
+            if (lane >= d)
+                cnt += o;
+        }
+        if (lane < a.n_groups)
+            item0[lane + 1] = cnt;
+        if (lane == 0)
+            item0[0] = 0;
+    }
+    __syncthreads();
+
+    // 3. work items in group order, item t to wave t mod kWaves
+    const cptr<SegGroup> groups = as_const(a.groups);
+    for (uint32_t g = 0; g < a.n_groups; ++g) {
+        const uint32_t i0 = item0[g], i1 = item0[g + 1];
+        uint32_t i = i0 + (wave + kWaves - i0 % kWaves) % kWaves;
+        if (i >= i1)
+            continue;
+        const cptr<SegGroup> gr = groups + g;
+        const ChanRun run = runs[gr->chan[0]];
+        if (gr->kind == kSegMfma) {
+            const uint32_t m = gr->m;
+            float bv[16];
+            const float *bt = a.btab + gr->btab;
+#pragma unroll
+            for (uint32_t s = 0; s < 16; ++s)
+                bv[s] = bt[s * 64 + lane];
+            // the lane's channel (lanes past the group's columns take its last)
+            const uint32_t cl = min((lane & 15) >> 1, m - 1);
+            const uint32_t my = gr->chan[cl];
+            const LaneChan lc = lane_chan(a.chans[my]);
+            const uint32_t k0 = runs[my].k0;
+            for (; i < i1; i += kWaves)
+                mfma_subtile(a, gr->down, m, run, k0, lc, bv, xs, span_lo, i - i0, lane);
+        } else {
+            const uint32_t ci = gr->chan[0];
+            const bool tail = gr->tail != 0;
+            if (gr->win == 32) {
+                for (; i < i1; i += kWaves)
+                    valu_batch<32>(a, ci, tail, run, pos, xs, span_lo, i - i0, lane);
+            } else {
+                for (; i < i1; i += kWaves)
+                    valu_batch<64>(a, ci, tail, run, pos, xs, span_lo, i - i0, lane);
+            }
+        }
+    }
+
+    // 4. the block holding the chunk's end writes the next history (the
+    // other buffer: every block of this launch reads `hist`), and carries the
+    // discriminator's last sample of every tail channel with no output in
+    // this chunk
+    if (seg_hi == a.n) {
+        for (uint32_t g = tid; g < a.n_groups; g += kBlock) {
+            const SegGroup &gr = a.groups[g];
+            if (gr.tail && runs[gr.chan[0]].j_end == 0) {
+                const ChanStatic &c = a.chans[gr.chan[0]];
+                c.prev[parity ^ 1u][0] = c.prev[parity][0];
+            }
+        }
+        float2 *hw = parity ? a.histw0 : a.histw1;
+        for (uint32_t t = tid; t < a.hist_len; t += kBlock) {
+            const int64_t i = (int64_t)a.n - (int64_t)a.hist_len + t;
+            hw[t] = i >= 0 ? a.in[i] : hist[a.hist_len + i];
+        }
+    }
+}
+
+}  // namespace
+
+size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo)
+{
+    const uint32_t span = seg + halo;
+    return (size_t)(span + span / 16 + 2) * sizeof(float2);
+}
+
+hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
+{
+    if (s.n == 0)
+        return hipSuccess;
+    SegDev a{};
+    a.in = s.in;
+    a.n = s.n;
+    a.halo = s.halo;
+    a.hist_len = s.hist_len;
+    a.hist0 = s.hist[0];
+    a.hist1 = s.hist[1];
+    a.histw0 = s.hist_w[0];
+    a.histw1 = s.hist_w[1];
+    a.chans = s.chans;
+    a.ptaps = s.ptaps;
+    a.btab = s.btab;
+    a.groups = s.groups;
+    a.n_groups = s.n_groups;
+    a.n_chan = s.n_chan;
+    a.pos = s.pos;
+    a.p = s.p;
+    a.parity = s.parity;
+    a.out16 = s.out16 ? 1u : 0u;
+    const uint32_t blocks = (s.n + s.seg - 1) / s.seg;
+    const size_t lds = chan_seg_lds_bytes(s.seg, s.halo);
+    switch (s.seg) {
+    case 1024:
+        hipLaunchKernelGGL(chan_seg_kernel<1024>, dim3(blocks), dim3(kBlock), lds, stream, a);
+        break;
+    case 2048:
+        hipLaunchKernelGGL(chan_seg_kernel<2048>, dim3(blocks), dim3(kBlock), lds, stream, a);
+        break;
+    case 4096:
+        hipLaunchKernelGGL(chan_seg_kernel<4096>, dim3(blocks), dim3(kBlock), lds, stream, a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fdsp

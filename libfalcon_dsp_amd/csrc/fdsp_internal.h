@@ -96,6 +96,10 @@ struct ChanStatic {
     uint32_t tail;
     float2 wrap;      // e^{j 2 pi nu R}: phase jump across a rollover inside a window
     void *out;        // chunk output base (set per launch)
+    uint32_t win;     // segment kernel: window (cpp padded in front to 32 or 64)
+    uint32_t poff;    // segment kernel: offset of the padded modulated taps (up*win complex)
+    float2 *prev[2];  // segment kernel, discriminator tail: the stream's last complex output,
+                      // read from [parity], written to [parity ^ 1]
 };
 
 // device-resident stream position of a batched channelizer (graph mode)
@@ -233,6 +237,45 @@ struct GroupsArgs {
                                 //   tiles one by one, starting in their table group
 };
 hipError_t launch_chan_groups(const GroupsArgs &a, hipStream_t s);
+
+// Work plan of the segment-major FP32 channelizer (fdsp_chan_seg.hip): one
+// workgroup per segment of `seg` inputs, staged once with `halo` samples of
+// history before it; records in the order the waves take their work.
+enum : uint32_t {
+    kSegMfma = 0,  // 2..8 channels, up 1, equal down, <= 32 taps: FP32 MFMA sub-tiles of 16 outputs
+    kSegValu = 1,  // one channel, window <= 64: one output per lane
+};
+constexpr uint32_t kSegMaxChan = 256;    // channels of one plan at most
+constexpr uint32_t kSegMaxGroups = 64;   // work groups of one plan at most
+constexpr uint32_t kSegMaxHalo = 256;    // staged history samples at most
+struct SegGroup {
+    uint32_t kind;     // kSegMfma or kSegValu
+    uint32_t m;        // channels
+    uint32_t up, down;
+    uint32_t win;      // window (padded taps per phase): 32 or 64
+    uint32_t btab;     // MFMA: offset of the group's B operands (16 x 64 floats)
+    uint32_t tail;     // VALU: the channel ends in the polar discriminator
+    uint32_t pad;
+    uint32_t chan[8];  // channel indices
+};
+struct SegArgs {
+    const float2 *in;
+    uint32_t n;
+    uint32_t seg, halo, hist_len;
+    const float2 *hist[2];
+    float2 *hist_w[2];
+    const ChanStatic *chans;
+    const float2 *ptaps;
+    const float *btab;
+    const SegGroup *groups;
+    uint32_t n_groups, n_chan;
+    const ChanPos *pos;
+    uint64_t p;
+    uint32_t parity;
+    bool out16;
+};
+hipError_t launch_chan_seg(const SegArgs &a, hipStream_t s);
+size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo);
 // resident blocks of the fused kernel per CU for a block size, build and LDS size
 int chan_groups_occupancy(uint32_t block, bool wide, size_t lds_bytes);
 // advance the device position by n inputs and flip the history parity

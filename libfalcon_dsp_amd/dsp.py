@@ -605,10 +605,12 @@ class MultiRateChannelizer(_Handle):
     multi_rate_channelizer_channel_s (output_sample_rate_in_sps,
     freq_shift_in_hz, up_rate, down_rate, taps) and optionally
     ``tail="discriminator"``. ``batched=True`` selects the fused one-launch
-    kernel; ``batched=False`` the reference-structured per-channel form."""
+    kernel; ``batched=False`` the reference-structured per-channel form.
+    ``arith``: "f32" (the default: FP32 arithmetic throughout) or "bf16x2"
+    (the batched form's opt-in reduced-precision matrix products)."""
     _destroy = "fdsp_channelizer_destroy"
 
-    def __init__(self, fs: int, channels: Sequence[dict], batched: bool = True):
+    def __init__(self, fs: int, channels: Sequence[dict], batched: bool = True, arith: str = "f32"):
         super().__init__()
         self.fs = int(fs)
         self.n = len(channels)
@@ -624,7 +626,12 @@ class MultiRateChannelizer(_Handle):
             d.taps = self._taps[i].ctypes.data
             d.n_taps = self._taps[i].size
             d.tail = self.tails[i]
-        self.mode = N.FDSP_CHAN_BATCHED if batched else N.FDSP_CHAN_PLAIN
+        if arith not in ("f32", "bf16x2"):
+            raise ValueError(f"arith must be 'f32' or 'bf16x2', not {arith!r}")
+        if not batched:
+            self.mode = N.FDSP_CHAN_PLAIN
+        else:
+            self.mode = N.FDSP_CHAN_BATCHED if arith == "f32" else N.FDSP_CHAN_BATCHED_BF16X2
         check(lib().fdsp_channelizer_create(self.fs, descs, self.n, self.mode, C.byref(self.h)))
         self.ups = [int(c["up_rate"]) for c in channels]
         self.downs = [int(c["down_rate"]) for c in channels]
@@ -652,6 +659,17 @@ class MultiRateChannelizer(_Handle):
         check(lib().fdsp_channelizer_get_tiling(self.h, C.byref(s), C.byref(b)))
         return s.value, b.value
 
+    def set_kernel(self, kernel: str):
+        """"segment" (FDSP_KERNEL_SEGMENT_F32) or "groups" (FDSP_KERNEL_GROUPS_F32)."""
+        k = {"segment": N.FDSP_KERNEL_SEGMENT_F32, "groups": N.FDSP_KERNEL_GROUPS_F32}[kernel]
+        check(lib().fdsp_channelizer_set_kernel(self.h, k))
+
+    def kernel_info(self) -> dict:
+        """Which kernel the batched form launches (fdsp_channelizer_kernel_info)."""
+        k, g, m = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().fdsp_channelizer_kernel_info(self.h, C.byref(k), C.byref(g), C.byref(m)))
+        return {"kind": k.value, "kernel": N.KERNEL_NAMES[k.value], "groups": g.value, "mfma_groups": m.value}
+
     def autotune(self, d_in: DeviceBuffer, chunk_len: int, reps: int = 10, in_offset: int = 0) -> List[dict]:
         """Time every candidate work split on a device chunk and keep the
         fastest (fdsp_channelizer_autotune); returns the trials."""
@@ -660,7 +678,8 @@ class MultiRateChannelizer(_Handle):
         k = C.c_size_t()
         check(lib().fdsp_channelizer_autotune(self.h, d_in.at(in_offset), chunk_len, reps, tr, cap, C.byref(k)))
         return [{"segment_len": t.segment_len, "block_threads": t.block_threads, "us": round(t.us, 2),
-                 "measured_weights": bool(t.flags & 1), "fused": bool(t.flags & 2)} for t in tr[: k.value]]
+                 "measured_weights": bool(t.flags & 1), "fused": bool(t.flags & 2),
+                 "segment_kernel": bool(t.flags & 4)} for t in tr[: k.value]]
 
     def set_fusion(self, on: bool):
         """Phase grouping: bf16-plane groups share staged spans (fdsp_channelizer_set_fusion)."""

@@ -14,6 +14,7 @@
 #include "transform/falcon_dsp_freq_shift_cuda.h"
 #include "transform/falcon_dsp_iir_filter.h"
 #include "signals/falcon_dsp_fm_demod.h"
+#include "signals/falcon_dsp_fm_demod_cuda.h"
 #include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
 #include "transform/falcon_dsp_polyphase_resampler_cuda.h"
@@ -517,6 +518,25 @@ bool falcon_dsp_fm_demodulator::demod_mono(std::vector<std::complex<float>>& in,
     }
     out.resize(got);
     return got > 0;
+}
+
+// GPU-named FM demodulator (reference src/signals/falcon_dsp_fm_demod_cuda.cu:91-265):
+// the same pipeline as the base class, which already runs on the GPU
+falcon_dsp_fm_demodulator_cuda::falcon_dsp_fm_demodulator_cuda(void) : falcon_dsp_fm_demodulator() {}
+
+bool falcon_dsp_fm_demodulator_cuda::initialize(uint32_t input_sample_rate_in_sps,
+                                                int32_t signal_offset_from_dc_in_hz,
+                                                float deemphasis_time_constant_in_usecs)
+{
+    return falcon_dsp_fm_demodulator::initialize(input_sample_rate_in_sps, signal_offset_from_dc_in_hz,
+                                                 deemphasis_time_constant_in_usecs);
+}
+
+void falcon_dsp_fm_demodulator_cuda::reset_state(void) { falcon_dsp_fm_demodulator::reset_state(); }
+
+bool falcon_dsp_fm_demodulator_cuda::demod_mono(std::vector<std::complex<float>>& in, std::vector<int16_t>& out)
+{
+    return falcon_dsp_fm_demodulator::demod_mono(in, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -12,10 +12,12 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels through the C ABI)")
     # a fresh checkout has no built libraries: compile them once (the HIP
     # library cross-compiles for gfx950 without a GPU)
-    need = [os.path.join(ROOT, "libfalcon_dsp_amd", "libfdsp_hip.so"), os.path.join(ROOT, "oracle", "liboracle.so")]
-    if not all(os.path.exists(p) for p in need):
+    if not os.path.exists(os.path.join(ROOT, "libfalcon_dsp_amd", "libfdsp_hip.so")):
         from libfalcon_dsp_amd import build
         build.build_all()
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"], check=True)
 
 
 _gpu_state = {}

@@ -15,6 +15,7 @@
 #include "transform/falcon_dsp_freq_shift_cuda.h"
 #include "transform/falcon_dsp_iir_filter.h"
 #include "signals/falcon_dsp_fm_demod.h"
+#include "signals/falcon_dsp_fm_demod_cuda.h"
 #include "transform/falcon_dsp_multi_rate_channelizer_cuda.h"
 #include "transform/falcon_dsp_polar_discriminator.h"
 #include "transform/falcon_dsp_polyphase_resampler_cuda.h"
@@ -441,6 +442,28 @@ static void test_fm_and_predefined()
     f.reset_state();
     ok = ok && f.demod_mono(x, y2) && y2 == y;
     check(ok, "fm demodulator: tau check, initialise once, 8000 samples peaking at 10000, reset repeats");
+
+    // the GPU-named class (reference hdr/signals/falcon_dsp_fm_demod_cuda.h:92-106)
+    // equals the base class on the same chunks, streamed, and through a reset
+    falcon_dsp_fm_demodulator base;
+    falcon_dsp_fm_demodulator_cuda gpu, gpu_bad;
+    falcon_dsp_fm_demodulator &as_base = gpu;  // called through the base interface, as callers may
+    ok = !gpu_bad.initialize(fs, -150000, 60e-6f) && base.initialize(fs, -150000) && gpu.initialize(fs, -150000) &&
+         !gpu.initialize(fs, -150000);
+    std::vector<cf> empty;
+    std::vector<int16_t> ya, yb;
+    ok = ok && !gpu.demod_mono(empty, yb) && yb.empty();
+    const size_t cuts[] = {0, 57000, 57001, 120000, 200000};
+    for (int pass = 0; pass < 2 && ok; ++pass) {
+        for (size_t k = 0; k + 1 < sizeof(cuts) / sizeof(cuts[0]); ++k) {
+            std::vector<cf> part(x.begin() + cuts[k], x.begin() + cuts[k + 1]);
+            const bool ra = base.demod_mono(part, ya), rb = as_base.demod_mono(part, yb);
+            ok = ok && ra == rb && ya == yb;
+        }
+        base.reset_state();
+        as_base.reset_state();
+    }
+    check(ok, "fm demodulator cuda: initialise once, equals the base class chunk by chunk and after reset");
 }
 
 int main()

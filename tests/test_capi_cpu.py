@@ -113,5 +113,8 @@ def test_dropin_library_exports_reference_classes():
                  "falcon_dsp::falcon_dsp_multi_rate_channelizer::apply",
                  "falcon_dsp::falcon_dsp_multi_rate_channelizer_cuda::initialize", "falcon_dsp::resample(",
                  "falcon_dsp::upfirdn_cuda(", "falcon_dsp::fir_filter_cuda(", "falcon_dsp::polar_discriminator(",
-                 "falcon_dsp::freq_shift_cuda(", "falcon_dsp::calculate_gcd", "falcon_dsp::rat_approx"]:
+                 "falcon_dsp::freq_shift_cuda(", "falcon_dsp::calculate_gcd", "falcon_dsp::rat_approx",
+                 "falcon_dsp::falcon_dsp_fm_demodulator::demod_mono",
+                 "falcon_dsp::falcon_dsp_fm_demodulator_cuda::initialize",
+                 "falcon_dsp::falcon_dsp_fm_demodulator_cuda::demod_mono"]:
         assert name in syms, name

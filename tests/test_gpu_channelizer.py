@@ -121,14 +121,18 @@ def test_grouped_channels_chunked():
     _compare(fs, _chans_grouped(fs), x, [434_780, 5, 1, 131_071, 3], True)
 
 
+# the channel-group kernel's work splits (FDSP_KERNEL_GROUPS_F32)
 @pytest.mark.parametrize("seg,block", [(256, 128), (2048, 512), (16384, 256), (fd.TILING_STRIDED, 256),
                                        (fd.TILING_STRIDED, 512), (fd.TILING_CLAIM, 128), (fd.TILING_CLAIM, 512)])
 def test_grouped_tilings_agree(seg, block):
     fs = 1_000_000
     chans = _chans_grouped(fs)
     x = cwgn(22, 200_000)
-    base = fd.MultiRateChannelizer(fs, chans).apply(x)
+    b0 = fd.MultiRateChannelizer(fs, chans)
+    b0.set_kernel("groups")
+    base = b0.apply(x)
     g = fd.MultiRateChannelizer(fs, chans)
+    g.set_kernel("groups")
     g.set_tiling(seg, block)
     # the work split only changes where the NCO phasor recurrence restarts
     # (a few float32 rounding steps), so tilings agree to ~1e-7
@@ -220,13 +224,13 @@ def test_workload_chan_32_slice():
 
 
 def test_fused_groups_match_oracle():
-    # two bf16-plane groups (down 8 and down 20) share staged spans when fused;
-    # chunks of every size, including ones shorter than a tile, and an
-    # unaligned input start (which takes the unfused form)
+    # opt-in bf16x2 form: two bf16-plane groups (down 8 and down 20) share
+    # staged spans when fused; chunks of every size, including ones shorter
+    # than a tile, and an unaligned input start (which takes the unfused form)
     w = Chan32(n=1 << 19)
     chans = [c for c in w.channels() if c["down_rate"] in (8, 20)]
     x = w.input()
-    g = fd.MultiRateChannelizer(w.fs, chans)
+    g = fd.MultiRateChannelizer(w.fs, chans, arith="bf16x2")
     on, members = g.get_fusion()
     assert not on and members == 2
     g.set_fusion(True)
@@ -240,7 +244,7 @@ def test_fused_groups_match_oracle():
     # device form on an odd sample offset (8-byte aligned input)
     n = 70_001
     d = fd.DeviceBuffer.from_array(x[: n + 1])
-    g2 = fd.MultiRateChannelizer(w.fs, chans)
+    g2 = fd.MultiRateChannelizer(w.fs, chans, arith="bf16x2")
     g2.set_fusion(True)
     outs = [fd.DeviceBuffer(m) for m in g2.max_out(n)]
     counts = g2.apply_device(d, n, outs, in_offset=1)
@@ -251,13 +255,15 @@ def test_fused_groups_match_oracle():
         assert_close_rms(outs[c].download(counts[c]), rs[c], xr, what=f"unaligned chan {c}")
 
 
-def test_autotune_keeps_stream_state():
+@pytest.mark.parametrize("kernel,least", [("segment", 3), ("groups", 9)])
+def test_autotune_keeps_stream_state(kernel, least):
     # tuning launches run on a chunk of the caller's data without moving the
     # stream: the chunks around it still continue the oracle's stream
     fs = 1_000_000
     chans = _chans_grouped(fs)
     x = cwgn(23, 300_000)
     g = fd.MultiRateChannelizer(fs, chans)
+    g.set_kernel(kernel)
     ref = O.Channelizer(fs, chans)
     for lo, hi in [(0, 100_000), (100_000, 300_000)]:
         ys, rs = g.apply(x[lo:hi]), ref.apply(x[lo:hi])
@@ -268,7 +274,8 @@ def test_autotune_keeps_stream_state():
             d = fd.DeviceBuffer.from_array(x[:131_072])
             trials = g.autotune(d, 131_072, reps=3)
             timed = [t for t in trials if t["us"] > 0]
-            assert len(trials) >= 9 and timed
+            assert len(trials) >= least and timed
+            assert all(t["segment_kernel"] == (kernel == "segment") for t in trials)
             best = min(timed, key=lambda t: t["us"])
             assert g.get_tiling() == (best["segment_len"], best["block_threads"])
 
@@ -279,8 +286,11 @@ def test_tilings_agree(seg, block):
     fs = 20_000_000
     chans = _chans_small(fs)
     x = cwgn(10, 120_000)
-    base = fd.MultiRateChannelizer(fs, chans).apply(x)
+    b0 = fd.MultiRateChannelizer(fs, chans)
+    b0.set_kernel("groups")
+    base = b0.apply(x)
     g = fd.MultiRateChannelizer(fs, chans)
+    g.set_kernel("groups")
     g.set_tiling(seg, block)
     # the work split only changes where the NCO phasor recurrence restarts
     # (a few float32 rounding steps), so tilings agree to ~1e-7
