@@ -1243,6 +1243,10 @@ void build_seg_plan(fdsp_channelizer_s *h, const std::vector<float2> &taps)
         s.poff = (uint32_t)ptaps.size();
         s.prev[0] = k.d_prev[0].as<float2>();
         s.prev[1] = k.d_prev[1].as<float2>();
+        s.dk = (uint32_t)((16ull * kSegWaves * k.down) % s.r);
+        s.d1 = k.down % s.r;
+        s.e_dk = phasor64(s.nu * (double)s.dk);
+        s.e_d1 = phasor64(s.nu * (double)s.d1);
         for (uint32_t ph = 0; ph < k.up; ++ph) {
             for (uint32_t t = 0; t < s.win - k.cpp; ++t)
                 ptaps.push_back(make_float2(0.f, 0.f));
@@ -1280,12 +1284,12 @@ void build_seg_plan(fdsp_channelizer_s *h, const std::vector<float2> &taps)
                 done[same[at + i]] = true;
             }
             // B operand of K step s in lane (col, g): window position
-            // t = 8g + s/2, part s%2 (0 real, 1 imaginary); column 2c gives
+            // t = g + 4(s/2), part s%2 (0 real, 1 imaginary); column 2c gives
             // channel c's real output (Gr xr - Gi xi), 2c+1 its imaginary
             // output (Gi xr + Gr xi)
             for (uint32_t st = 0; st < 16; ++st)
                 for (uint32_t lane = 0; lane < 64; ++lane) {
-                    const uint32_t col = lane % 16, t = 8 * (lane / 16) + st / 2, part = st % 2;
+                    const uint32_t col = lane % 16, t = lane / 16 + 4 * (st / 2), part = st % 2;
                     float v = 0.f;
                     if (col / 2 < gr.m) {
                         const ChanStatic &cs = h->stat[gr.chan[col / 2]];
@@ -1314,10 +1318,12 @@ void build_seg_plan(fdsp_channelizer_s *h, const std::vector<float2> &taps)
         gr.chan[0] = c;
         valu.push_back(gr);
     }
-    // MFMA groups first (largest output share first), then the VALU channels
+    // the waves take work items in plan order: the VALU channels first (their
+    // one-output-per-lane batches are the longest latency chains, so they
+    // start first), then the MFMA groups, largest output share first
     std::stable_sort(mfma.begin(), mfma.end(), [](const SegGroup &a, const SegGroup &b) { return a.down < b.down; });
-    h->seg_groups = mfma;
-    h->seg_groups.insert(h->seg_groups.end(), valu.begin(), valu.end());
+    h->seg_groups = valu;
+    h->seg_groups.insert(h->seg_groups.end(), mfma.begin(), mfma.end());
     if (h->seg_groups.size() > kSegMaxGroups)
         return;
     // staged history: every window's reach back, rounded to 16 samples
@@ -2339,7 +2345,7 @@ extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len
         if (groups)
             *groups = (uint32_t)h->seg_groups.size();
         if (lds_bytes)
-            *lds_bytes = (uint32_t)chan_seg_lds_bytes(h->seg_len_in, h->seg_halo);
+            *lds_bytes = (uint32_t)chan_seg_lds_bytes(h->seg_len_in, h->seg_halo, (uint32_t)h->ch.size());
         return FDSP_OK;
     }
     std::vector<uint32_t> t;

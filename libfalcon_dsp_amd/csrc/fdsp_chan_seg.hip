@@ -21,20 +21,24 @@
 // Work groups (SegGroup records, built on the host):
 //   MFMA  2..8 channels with up 1, equal down and at most 32 taps. Their
 //         32-sample windows (taps zero-padded in front) are the A operand of
-//         v_mfma_f32_16x16x4f32: 16 outputs x 64 real K values, window sample
-//         8g + s/2 of row i in lane i + 16g at K step s, its real part on
-//         even steps and its imaginary part on odd ones. The modulated taps
-//         are the B operand: channel c's real output in column 2c, its
-//         imaginary output in 2c+1. A sub-tile is 16 outputs of every channel
-//         of the group.
+//         v_mfma_f32_16x16x4f32: 16 outputs x 64 real K values. Lane i + 16g
+//         holds window samples g, g+4, ..., g+28 of row i, the real part of
+//         sample g+4j at K step 2j and its imaginary part at step 2j+1. The
+//         modulated taps are the B operand in the same K order: channel c's
+//         real output in column 2c, its imaginary output in 2c+1. A sub-tile
+//         is 16 outputs of every channel of the group.
 //   VALU  one channel (any up), window of at most 64 taps: one output per
 //         lane, taps zero-padded in front to kW = 32 or 64. With up 1 the taps
 //         are wavefront-uniform scalar loads; with up > 1 each lane reads the
 //         bank row of its filter phase (n*down) mod up.
 //
-// LDS layout of the staged span: span sample q at slot q + q/16 (an 8-byte
-// pad every 16 samples), so the 16 rows of an MFMA sub-tile, 8 samples
-// apart at down 8, fall on distinct banks.
+// LDS layout of the staged span: transposed by 4, span sample q at slot
+// (q mod 4)*kM + q/4 (kM odd). Samples 4 apart are adjacent, so a lane's
+// eight MFMA samples (or a VALU window's samples of one residue) are eight
+// consecutive slots, read from one base address with immediate offsets; the
+// 16 rows of a down-8 sub-tile read slots 2 apart, on distinct banks.
+#include <algorithm>
+
 #include "fdsp_internal.h"
 
 namespace fdsp {
@@ -53,9 +57,11 @@ template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p) { ret
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
 
+// rows of the transposed span (odd), fixed per segment length
+__host__ __device__ constexpr uint32_t seg_rows(uint32_t seg) { return ((seg + kSegMaxHalo) / 4 + 1) | 1u; }
+
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint32_t slot(uint32_t q) { return q + (q >> 4); }
-__device__ __forceinline__ float2 lds_x(const float2 *xs, uint32_t q) { return xs[slot(q)]; }
+template <uint32_t kM> __device__ __forceinline__ uint32_t slot(uint32_t q) { return (q & 3u) * kM + (q >> 2); }
 
 // floor(a / d) for a < 2^52 from an FP64 quotient estimate and one exact
 // correction step (inv_d = 1/d rounded to double)
@@ -152,6 +158,7 @@ struct SegPos {
 // correction when the window straddles a restart (window positions t <
 // win-1-k precede it; taps of phase ph; q: the window's first span sample),
 // then the phasor.
+template <uint32_t kM>
 __device__ __forceinline__ float2 finish_out(const SegDev &a, const LaneChan &c, uint32_t ph, const float2 *xs,
                                              uint32_t q, uint32_t k, float2 y)
 {
@@ -159,70 +166,146 @@ __device__ __forceinline__ float2 finish_out(const SegDev &a, const LaneChan &c,
         const float2 *g = a.ptaps + c.poff + ph * c.win;
         float2 part = make_float2(0.f, 0.f);
         for (uint32_t t = 0; t + 1 + k < c.win; ++t)
-            cmac(part, g[t], lds_x(xs, q + t));
+            cmac(part, g[t], xs[slot<kM>(q + t)]);
         cmac(y, c.wrap_m1, part);
     }
     return cmul(nco_phasor((int64_t)k, c.nu), y);
 }
 
 // ---------------------------------------------------------------------------
-// MFMA group: sub-tile st (segment rows 16 st ..) of every channel of the group
+// MFMA group: this wave's sub-tiles st, st + kWaves, ... < n_st (sub-tile s:
+// segment rows 16 s .. 16 s + 15 of every channel of the group)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mfma_subtile(const SegDev &a, uint32_t down, uint32_t m, const ChanRun &run,
-                                             uint32_t k0, const LaneChan &lc, const float (&bv)[16],
-                                             const float2 *xs, uint32_t span_lo, uint32_t st, uint32_t lane)
+// One sub-tile's A operand (eight consecutive slots from p) into two MFMA
+// chains (real-part and imaginary-part K steps); the caller adds them.
+__device__ __forceinline__ void mfma_tile(const float2 *p, const float (&bv)[16], floatx4 &c0, floatx4 &c1)
 {
-    const uint32_t n_rows = run.j_hi - run.j_lo;
-    const uint32_t r0 = st * 16;
-    // A operand: row lane%16 (clamped to the last real row), window samples
-    // 8*(lane/16) .. +7
-    const uint32_t ri = min(r0 + (lane & 15), n_rows - 1);
-    const uint32_t qa = (run.j_lo + ri) * down + run.rho - 31 - span_lo + 8 * (lane >> 4);
     float2 x[8];
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j)
-        x[j] = lds_x(xs, qa + j);
-    floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+        x[j] = p[j];
+    c0 = floatx4{0.f, 0.f, 0.f, 0.f};
+    c1 = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) {
         c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].x, bv[2 * j], c0, 0, 0, 0);
         c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].y, bv[2 * j + 1], c1, 0, 0, 0);
     }
-    const floatx4 acc = c0 + c1;
+}
+
+template <uint32_t kM>
+__device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, const ChanRun &run,
+                                           const ChanRun *runs, const float2 *xs, uint32_t span_lo, uint32_t st,
+                                           uint32_t n_st, uint32_t lane)
+{
+    const uint32_t down = gr->down, m = gr->m;
+    const uint32_t n_rows = run.j_hi - run.j_lo;
+    // opaque per call: keeps the group's loads inside the persistent segment
+    // loop instead of hoisted (and every group's operands held) across it
+    asm volatile("" : "+v"(lane));
+    float bv[16];
+    {
+        const float *bt = a.btab + gr->btab;
+#pragma unroll
+        for (uint32_t s = 0; s < 16; ++s)
+            bv[s] = bt[s * 64 + lane];
+    }
     // D[i][col] sits in lane col + 16*(i/4), element i%4: lanes 2c and 2c+1
     // hold channel c's real and imaginary parts of rows 4*(lane/16) .. +3.
     // The even lane finishes the first two of those rows, the odd lane the
-    // last two (quad_perm 1,0,3,2 swaps the pair's values).
-    float oth[4];
+    // last two (quad_perm 1,0,3,2 swaps the pair's values). Lanes past the
+    // group's columns run with its last channel and store nothing.
+    const uint32_t col = lane & 15, g = lane >> 4;
+    const bool odd = col & 1, live = (col >> 1) < m;
+    const uint32_t ci = gr->chan[min(col >> 1, m - 1)];
+    const ChanStatic &cs = a.chans[ci];
+    const LaneChan lc = lane_chan(cs);
+    const uint32_t R = lc.r, dk = cs.dk, d1 = cs.d1;
+    const float2 e_dk = cs.e_dk, e_d1 = cs.e_d1;
+    const uint32_t ro = 4 * g + (odd ? 2u : 0u);  // sub-tile row of the lane's first output
+    // NCO index k of the lane's first row, advanced by dk = (16 kWaves down)
+    // mod R per step, and its phasor ph = e^{j2pi nu k}, advanced by e^{j2pi
+    // nu dk}; the second row is d1 = down mod R later. Where an index wraps
+    // past R the phase jumps (the reference's rollover): that lane's phasor
+    // is evaluated afresh in FP64.
+    uint32_t k = (uint32_t)mod_u64((uint64_t)runs[ci].k0 + (uint64_t)(run.j_lo + 16 * st + ro) * down, R, lc.inv_r);
+    float2 ph = nco_phasor((int64_t)k, lc.nu);
+    // A operand: row lane%16 reads window samples g + 4j, eight consecutive
+    // slots; rows of a whole sub-tile are unclamped, so the next one's base is
+    // (16 kWaves down)/4 slots on. A partial sub-tile clamps its rows to the
+    // segment's last.
+    const uint32_t i = lane & 15;
+    const auto a_base = [&](uint32_t s) -> uint32_t {
+        const uint32_t row = min(16 * s + i, n_rows - 1);
+        return slot<kM>((run.j_lo + row) * down + run.rho - 31 - span_lo + g);
+    };
+    const bool pair16 = a.out16 && (run.j_lo & 1) == 0;
+    // epilogue of sub-tile s from its two chains
+    const auto finish = [&](uint32_t s, const floatx4 &c0, const floatx4 &c1) {
+        const floatx4 acc = c0 + c1;
+        float oth[4];
 #pragma unroll
-    for (uint32_t e = 0; e < 4; ++e)
-        oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
-    const uint32_t col = lane & 15;
-    const bool odd = col & 1;
-    const float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
-    const float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
-    const uint32_t rr = r0 + 4 * (lane >> 4) + (odd ? 2 : 0);  // segment row of y0
-    if ((col >> 1) >= m || rr >= n_rows)
-        return;
-    const uint32_t j = run.j_lo + rr;                           // chunk output index of y0
-    const uint32_t q0 = j * down + run.rho - 31 - span_lo;      // window start of y0's row
-    const uint32_t R = lc.r;
-    const uint32_t kk0 = (uint32_t)mod_u64((uint64_t)k0 + (uint64_t)j * down, R, lc.inv_r);
-    uint32_t kk1 = kk0 + down;
-    if (kk1 >= R)
-        kk1 = (uint32_t)mod_u64((uint64_t)kk1, R, lc.inv_r);
-    const float2 o0 = finish_out(a, lc, 0, xs, q0, kk0, y0);
-    const float2 o1 = finish_out(a, lc, 0, xs, q0 + down, kk1, y1);
-    if (rr + 1 < n_rows) {
-        if (a.out16 && (j & 1) == 0)
-            ((gvec4_ptr)(lc.out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
-        else {
-            lc.out[j] = fvec2{o0.x, o0.y};
-            lc.out[j + 1] = fvec2{o1.x, o1.y};
+        for (uint32_t e = 0; e < 4; ++e)
+            oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
+        float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
+        float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
+        const uint32_t rr = 16 * s + ro;  // segment row of y0
+        uint32_t k1 = k + d1;
+        float2 ph1 = cmul(ph, e_d1);
+        if (k1 >= R) {  // rare: the second row's index wraps
+            k1 -= R;
+            ph1 = nco_phasor((int64_t)k1, lc.nu);
         }
-    } else {
-        lc.out[j] = fvec2{o0.x, o0.y};
+        const bool roll0 = rr < n_rows && k + 1 < lc.win, roll1 = rr + 1 < n_rows && k1 + 1 < lc.win;
+        if (roll0 || roll1) {
+            // rare: a window of a row that exists straddles a rollover; its
+            // samples before it carry e^{j2pi nu R}
+            const uint32_t q0 = (run.j_lo + rr) * down + run.rho - 31 - span_lo;
+            const float2 *gt = a.ptaps + lc.poff;
+            float2 p0 = make_float2(0.f, 0.f), p1 = make_float2(0.f, 0.f);
+            for (uint32_t t = 0; roll0 && t + 1 + k < lc.win; ++t)
+                cmac(p0, gt[t], xs[slot<kM>(q0 + t)]);
+            for (uint32_t t = 0; roll1 && t + 1 + k1 < lc.win; ++t)
+                cmac(p1, gt[t], xs[slot<kM>(q0 + down + t)]);
+            cmac(y0, lc.wrap_m1, p0);
+            cmac(y1, lc.wrap_m1, p1);
+        }
+        const float2 o0 = cmul(ph, y0), o1 = cmul(ph1, y1);
+        if (live && rr < n_rows) {
+            const uint32_t j = run.j_lo + rr;
+            if (rr + 1 < n_rows) {
+                if (pair16)
+                    ((gvec4_ptr)(lc.out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+                else {
+                    lc.out[j] = fvec2{o0.x, o0.y};
+                    lc.out[j + 1] = fvec2{o1.x, o1.y};
+                }
+            } else {
+                lc.out[j] = fvec2{o0.x, o0.y};
+            }
+        }
+    };
+    // software pipeline: the next sub-tile's LDS reads and MFMAs are issued
+    // before this one's epilogue, so no wave waits on its own products
+    uint32_t ab = a_base(st);
+    floatx4 c0, c1;
+    mfma_tile(xs + ab, bv, c0, c1);
+    for (uint32_t sn = st + kWaves; sn < n_st; sn += kWaves) {
+        ab = 16 * (sn + 1) <= n_rows ? ab + 4 * kWaves * down : a_base(sn);
+        floatx4 n0, n1;
+        mfma_tile(xs + ab, bv, n0, n1);
+        finish(st, c0, c1);
+        c0 = n0;
+        c1 = n1;
+        st = sn;
+        k += dk;
+        ph = cmul(ph, e_dk);
+        if (k >= R) {  // rare: the index wrapped
+            k -= R;
+            ph = nco_phasor((int64_t)k, lc.nu);
+        }
     }
+    finish(st, c0, c1);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,76 +319,116 @@ __device__ __forceinline__ void mfma_subtile(const SegDev &a, uint32_t down, uin
 // lane takes y[j-1] from its neighbour and emits angle(y[j] conj(y[j-1])).
 // The stream's first output has no predecessor and emits nothing; the
 // chunk's last output is carried to the next chunk.
-template <uint32_t kW>
-__device__ __forceinline__ void valu_batch(const SegDev &a, uint32_t ci, bool tail, const ChanRun &run,
+// The window's samples of residue u (window positions u, u+4, ...) are
+// consecutive slots from base u.
+template <uint32_t kW, uint32_t kM>
+__device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool tail, const ChanRun &run,
                                            const SegPos &pos, const float2 *xs, uint32_t span_lo, uint32_t b,
-                                           uint32_t lane)
+                                           uint32_t n_b, uint32_t lane)
 {
+    // opaque per call: keeps the channel's loads inside the segment loop
+    asm volatile("" : "+s"(ci));
     const cptr<ChanStatic> cs = as_const(a.chans) + ci;
     const uint32_t up = cs->up, down = cs->down;
     const int32_t n_rows = (int32_t)(run.j_hi - run.j_lo);
-    const int32_t rr = tail ? (int32_t)(b * 63 + lane) - 1 : (int32_t)(b * 64 + lane);
-    // the row computed: rows past the segment's last take the last; row -1 of
-    // the chunk's first segment (no window: the carried sample replaces it)
-    // takes row 0
-    const int32_t rc = min(rr, n_rows - 1);
-    const uint32_t j = (uint32_t)((int32_t)run.j_lo + (rc < 0 && run.j_lo == 0 ? 0 : rc));
-    float2 acc = make_float2(0.f, 0.f);
-    uint32_t anchor, ph;
-    if (up == 1) {
-        anchor = j * down + run.rho;
-        ph = 0;
-        const uint32_t q = anchor + 1 - kW - span_lo;
-        const cptr<float> g = (cptr<float>)(a.ptaps + cs->poff);
-#pragma unroll
-        for (uint32_t t = 0; t < kW; ++t)
-            cmac(acc, make_float2(g[2 * t], g[2 * t + 1]), lds_x(xs, q + t));
-    } else {
-        const uint64_t num = (uint64_t)j * down + run.rho;
-        anchor = (uint32_t)udiv64(num, up, cs->inv_up);
-        ph = (uint32_t)(num - (uint64_t)anchor * up);
-        const uint32_t q = anchor + 1 - kW - span_lo;
-        const float2 *g = a.ptaps + cs->poff + ph * kW;
-#pragma unroll 8
-        for (uint32_t t = 0; t < kW; ++t)
-            cmac(acc, g[t], lds_x(xs, q + t));
-    }
     const LaneChan lc = lane_chan(a.chans[ci]);
-    // NCO index of the anchor; output 0's anchor is floor(rho/up)
-    const uint32_t k = (uint32_t)mod_u64((uint64_t)run.k0 + (anchor - run.rho / up), lc.r, lc.inv_r);
-    float2 y = finish_out(a, lc, ph, xs, anchor + 1 - kW - span_lo, k, acc);
-    if (!tail) {
-        if (rr < n_rows)
-            lc.out[j] = fvec2{y.x, y.y};
-        return;
+    // NCO index of the segment's first output's anchor (output 0's anchor is
+    // floor(rho/up)); later anchors of the segment are fewer than R inputs on
+    // (the host keeps channels with ceil(down/up) + segment >= R off this path)
+    const uint32_t a_lo = (uint32_t)(((uint64_t)run.j_lo * down + run.rho) / up);
+    const uint32_t k_lo = (uint32_t)mod_u64((uint64_t)run.k0 + (a_lo - run.rho / up), lc.r, lc.inv_r);
+    for (; b < n_b; b += kWaves) {
+        const int32_t rr = tail ? (int32_t)(b * 63 + lane) - 1 : (int32_t)(b * 64 + lane);
+        // the row computed: rows past the segment's last take the last; row -1
+        // of the chunk's first segment (no window: the carried sample replaces
+        // it) takes row 0
+        const int32_t rc = min(rr, n_rows - 1);
+        const uint32_t j = (uint32_t)((int32_t)run.j_lo + (rc < 0 && run.j_lo == 0 ? 0 : rc));
+        uint32_t anchor, ph;
+        if (up == 1) {
+            anchor = j * down + run.rho;
+            ph = 0;
+        } else {
+            const uint64_t num = (uint64_t)j * down + run.rho;
+            anchor = (uint32_t)udiv64(num, up, cs->inv_up);
+            ph = (uint32_t)(num - (uint64_t)anchor * up);
+        }
+        const uint32_t q = anchor + 1 - kW - span_lo;
+        const float2 *base[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            base[u] = xs + slot<kM>(q + u);
+        // four independent chains: even and odd taps, real and imaginary;
+        // eight taps (two per residue) per step
+        float2 acc0 = make_float2(0.f, 0.f), acc1 = make_float2(0.f, 0.f);
+        if (up == 1) {
+            const cptr<float> g = (cptr<float>)(a.ptaps + cs->poff);
+#pragma unroll 1
+            for (uint32_t m = 0; m < kW / 4; m += 2) {
+#pragma unroll
+                for (uint32_t e = 0; e < 8; e += 2) {
+                    const uint32_t t = 4 * m + e;  // taps t, t+1 of residues e%4, e%4 + 1
+                    cmac(acc0, make_float2(g[2 * t], g[2 * t + 1]), base[e & 3][m + (e >> 2)]);
+                    cmac(acc1, make_float2(g[2 * t + 2], g[2 * t + 3]), base[(e + 1) & 3][m + (e >> 2)]);
+                }
+            }
+        } else {
+            const float2 *g = a.ptaps + cs->poff + ph * kW;
+#pragma unroll 1
+            for (uint32_t m = 0; m < kW / 4; m += 2) {
+#pragma unroll
+                for (uint32_t e = 0; e < 8; e += 2) {
+                    const uint32_t t = 4 * m + e;
+                    cmac(acc0, g[t], base[e & 3][m + (e >> 2)]);
+                    cmac(acc1, g[t + 1], base[(e + 1) & 3][m + (e >> 2)]);
+                }
+            }
+        }
+        const float2 acc = make_float2(acc0.x + acc1.x, acc0.y + acc1.y);
+        // NCO index of the anchor (the overlap row before the segment is at
+        // most ceil(down/up) inputs earlier)
+        uint32_t k = k_lo + (anchor - a_lo);
+        if ((int32_t)(anchor - a_lo) < 0)
+            k += lc.r;
+        if (k >= lc.r)
+            k -= lc.r;
+        float2 y = finish_out<kM>(a, lc, ph, xs, q, k, acc);
+        if (!tail) {
+            if (rr < n_rows)
+                lc.out[j] = fvec2{y.x, y.y};
+            continue;
+        }
+        // discriminator tail (see above)
+        const bool first_row = rr < 0 && run.j_lo == 0;  // the previous chunk's last output
+        if (first_row && pos.P > 0)
+            y = cs->prev[pos.parity][0];
+        const float2 yp = make_float2(__shfl_up(y.x, 1), __shfl_up(y.y, 1));
+        // lane 0 only supplies its row (the previous batch's last) to lane 1
+        if (lane == 0 || rr >= n_rows)
+            continue;
+        // the chunk's last output is the next chunk's predecessor
+        if (j + 1 == run.j_end)
+            cs->prev[pos.parity ^ 1u][0] = y;
+        // output j of the chunk has a predecessor unless it is the stream's first
+        const uint32_t drop = pos.P == 0 ? 1u : 0u;
+        if (j < drop)
+            continue;
+        const float2 z = cmul(y, make_float2(yp.x, -yp.y));
+        ((__attribute__((address_space(1))) float *)lc.out)[j - drop] = atan2f(z.y, z.x);
     }
-    // discriminator tail
-    const bool first_row = rr < 0 && run.j_lo == 0;  // the previous chunk's last output
-    if (first_row && pos.P > 0)
-        y = cs->prev[pos.parity][0];
-    const float2 yp = make_float2(__shfl_up(y.x, 1), __shfl_up(y.y, 1));
-    // lane 0 only supplies its row (the previous batch's last) to lane 1
-    if (lane == 0 || rr >= n_rows)
-        return;
-    // the chunk's last output is the next chunk's predecessor
-    if (j + 1 == run.j_end)
-        cs->prev[pos.parity ^ 1u][0] = y;
-    // output j of the chunk has a predecessor unless it is the stream's first
-    const uint32_t drop = pos.P == 0 ? 1u : 0u;
-    if (j < drop)
-        return;
-    const float2 z = cmul(y, make_float2(yp.x, -yp.y));
-    ((__attribute__((address_space(1))) float *)lc.out)[j - drop] = atan2f(z.y, z.x);
 }
 
 // ---------------------------------------------------------------------------
 // the kernel: one segment per workgroup
 // ---------------------------------------------------------------------------
+// at most 5 waves per SIMD fit the LDS of a 2048-sample segment anyway
 template <uint32_t kSeg>
-__global__ __launch_bounds__(kBlock) void chan_seg_kernel(SegDev a)
+__global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
 {
+    constexpr uint32_t kM = seg_rows(kSeg);
+    // dynamic LDS: the transposed span (4 kM samples), then one ChanRun per channel
     extern __shared__ float2 xs[];
-    __shared__ ChanRun runs[kSegMaxChan];
+    ChanRun *runs = reinterpret_cast<ChanRun *>(xs + 4 * kM);
     __shared__ uint32_t item0[kSegMaxGroups + 1];
 
     SegPos pos{a.p, a.parity};
@@ -326,49 +449,62 @@ __global__ __launch_bounds__(kBlock) void chan_seg_kernel(SegDev a)
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uniform(tid >> 6);
 
     // 1. stage the span (chunk-local inputs seg_lo - H .. seg_hi - 1, the part
-    // before the chunk from the history), two samples per thread and load
+    // before the chunk from the history), two samples per thread and load,
+    // into LDS transposed by 4
     {
         const int64_t base = (int64_t)seg_lo - (int64_t)H;
         const uint32_t pairs = (seg_hi - seg_lo + H + 1) / 2;
         const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15) == 0;
         constexpr uint32_t kRounds = (kSeg + kSegMaxHalo) / 2 / kBlock + 1;
         float4 v[kRounds];
+        if (base >= 0 && seg_lo + kSeg <= a.n && in16) {
+            // interior segment: every pair lies in the chunk, 16-byte aligned
+            // (base and the input are even / 16-byte aligned)
+            const float4 *src = reinterpret_cast<const float4 *>(a.in + base);
 #pragma unroll
-        for (uint32_t r = 0; r < kRounds; ++r) {
-            const uint32_t pr = tid + r * kBlock;
-            const int64_t i = base + 2 * (int64_t)pr;
-            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (pr < pairs) {
-                if (i >= 0) {
-                    if (in16 && i + 1 < (int64_t)a.n) {
-                        w = *reinterpret_cast<const float4 *>(a.in + i);
-                    } else {
-                        if (i < (int64_t)a.n) {
-                            const float2 u = a.in[i];
-                            w.x = u.x;
-                            w.y = u.y;
-                        }
-                        if (i + 1 < (int64_t)a.n) {
-                            const float2 u = a.in[i + 1];
-                            w.z = u.x;
-                            w.w = u.y;
-                        }
-                    }
-                } else {
-                    // i even and i + 1 <= -1: both samples from the history
-                    const float2 u0 = hist[a.hist_len + i], u1 = hist[a.hist_len + i + 1];
-                    w = make_float4(u0.x, u0.y, u1.x, u1.y);
-                }
+            for (uint32_t r = 0; r < kRounds; ++r) {
+                const uint32_t pr = tid + r * kBlock;
+                v[r] = pr < pairs ? src[pr] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            v[r] = w;
+        } else {
+#pragma unroll
+            for (uint32_t r = 0; r < kRounds; ++r) {
+                const uint32_t pr = tid + r * kBlock;
+                const int64_t i = base + 2 * (int64_t)pr;
+                float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (pr < pairs) {
+                    if (i >= 0) {
+                        if (in16 && i + 1 < (int64_t)a.n) {
+                            w = *reinterpret_cast<const float4 *>(a.in + i);
+                        } else {
+                            if (i < (int64_t)a.n) {
+                                const float2 u = a.in[i];
+                                w.x = u.x;
+                                w.y = u.y;
+                            }
+                            if (i + 1 < (int64_t)a.n) {
+                                const float2 u = a.in[i + 1];
+                                w.z = u.x;
+                                w.w = u.y;
+                            }
+                        }
+                    } else {
+                        // i even and i + 1 <= -1: both samples from the history
+                        const float2 u0 = hist[a.hist_len + i], u1 = hist[a.hist_len + i + 1];
+                        w = make_float4(u0.x, u0.y, u1.x, u1.y);
+                    }
+                }
+                v[r] = w;
+            }
         }
 #pragma unroll
         for (uint32_t r = 0; r < kRounds; ++r) {
             const uint32_t pr = tid + r * kBlock;
             if (pr < pairs) {
-                const uint32_t q = 2 * pr;  // even: q and q+1 sit in one 16-sample row
-                xs[slot(q)] = make_float2(v[r].x, v[r].y);
-                xs[slot(q) + 1] = make_float2(v[r].z, v[r].w);
+                // q even: samples q and q+1 share a transposed column
+                const uint32_t q = 2 * pr;
+                xs[slot<kM>(q)] = make_float2(v[r].x, v[r].y);
+                xs[slot<kM>(q) + kM] = make_float2(v[r].z, v[r].w);
             }
         }
     }
@@ -398,39 +534,27 @@ __global__ __launch_bounds__(kBlock) void chan_seg_kernel(SegDev a)
     }
     __syncthreads();
 
-    // 3. work items in group order, item t to wave t mod kWaves
+    // 3. work items in group order, item t to wave (t + block) mod kWaves: the
+    // rotation spreads the segment's leftover items over the SIMDs (wave w of
+    // every block runs on the same SIMD)
     const cptr<SegGroup> groups = as_const(a.groups);
+    const uint32_t rot = blockIdx.x % kWaves;
     for (uint32_t g = 0; g < a.n_groups; ++g) {
         const uint32_t i0 = item0[g], i1 = item0[g + 1];
-        uint32_t i = i0 + (wave + kWaves - i0 % kWaves) % kWaves;
+        uint32_t i = i0 + (wave + 2 * kWaves - (i0 + rot) % kWaves) % kWaves;
         if (i >= i1)
             continue;
         const cptr<SegGroup> gr = groups + g;
         const ChanRun run = runs[gr->chan[0]];
         if (gr->kind == kSegMfma) {
-            const uint32_t m = gr->m;
-            float bv[16];
-            const float *bt = a.btab + gr->btab;
-#pragma unroll
-            for (uint32_t s = 0; s < 16; ++s)
-                bv[s] = bt[s * 64 + lane];
-            // the lane's channel (lanes past the group's columns take its last)
-            const uint32_t cl = min((lane & 15) >> 1, m - 1);
-            const uint32_t my = gr->chan[cl];
-            const LaneChan lc = lane_chan(a.chans[my]);
-            const uint32_t k0 = runs[my].k0;
-            for (; i < i1; i += kWaves)
-                mfma_subtile(a, gr->down, m, run, k0, lc, bv, xs, span_lo, i - i0, lane);
+            mfma_group<kM>(a, gr, run, runs, xs, span_lo, i - i0, i1 - i0, lane);
         } else {
             const uint32_t ci = gr->chan[0];
             const bool tail = gr->tail != 0;
-            if (gr->win == 32) {
-                for (; i < i1; i += kWaves)
-                    valu_batch<32>(a, ci, tail, run, pos, xs, span_lo, i - i0, lane);
-            } else {
-                for (; i < i1; i += kWaves)
-                    valu_batch<64>(a, ci, tail, run, pos, xs, span_lo, i - i0, lane);
-            }
+            if (gr->win == 32)
+                valu_group<32, kM>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
+            else
+                valu_group<64, kM>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
         }
     }
 
@@ -454,12 +578,20 @@ __global__ __launch_bounds__(kBlock) void chan_seg_kernel(SegDev a)
     }
 }
 
+template <uint32_t kSeg>
+hipError_t launch_seg(const SegDev &a, uint32_t seg, size_t lds, hipStream_t stream)
+{
+    const uint32_t blocks = (a.n + seg - 1) / seg;
+    hipLaunchKernelGGL(chan_seg_kernel<kSeg>, dim3(blocks), dim3(kBlock), lds, stream, a);
+    return hipGetLastError();
+}
+
 }  // namespace
 
-size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo)
+size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo, uint32_t n_chan)
 {
-    const uint32_t span = seg + halo;
-    return (size_t)(span + span / 16 + 2) * sizeof(float2);
+    (void)halo;  // the transposed span is sized for the largest halo
+    return (size_t)4 * seg_rows(seg) * sizeof(float2) + (size_t)n_chan * sizeof(ChanRun);
 }
 
 hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
@@ -485,22 +617,17 @@ hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
     a.p = s.p;
     a.parity = s.parity;
     a.out16 = s.out16 ? 1u : 0u;
-    const uint32_t blocks = (s.n + s.seg - 1) / s.seg;
-    const size_t lds = chan_seg_lds_bytes(s.seg, s.halo);
+    const size_t lds = chan_seg_lds_bytes(s.seg, s.halo, s.n_chan);
     switch (s.seg) {
     case 1024:
-        hipLaunchKernelGGL(chan_seg_kernel<1024>, dim3(blocks), dim3(kBlock), lds, stream, a);
-        break;
+        return launch_seg<1024>(a, s.seg, lds, stream);
     case 2048:
-        hipLaunchKernelGGL(chan_seg_kernel<2048>, dim3(blocks), dim3(kBlock), lds, stream, a);
-        break;
+        return launch_seg<2048>(a, s.seg, lds, stream);
     case 4096:
-        hipLaunchKernelGGL(chan_seg_kernel<4096>, dim3(blocks), dim3(kBlock), lds, stream, a);
-        break;
+        return launch_seg<4096>(a, s.seg, lds, stream);
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace fdsp

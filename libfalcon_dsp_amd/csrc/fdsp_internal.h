@@ -100,6 +100,10 @@ struct ChanStatic {
     uint32_t poff;    // segment kernel: offset of the padded modulated taps (up*win complex)
     float2 *prev[2];  // segment kernel, discriminator tail: the stream's last complex output,
                       // read from [parity], written to [parity ^ 1]
+    uint32_t dk;      // segment kernel, MFMA groups: (16 kSegWaves down) mod R, a wave's NCO index step
+    uint32_t d1;      // down mod R
+    float2 e_dk;      // e^{j 2 pi nu dk}
+    float2 e_d1;      // e^{j 2 pi nu d1}
 };
 
 // device-resident stream position of a batched channelizer (graph mode)
@@ -246,6 +250,7 @@ enum : uint32_t {
     kSegValu = 1,  // one channel, window <= 64: one output per lane
 };
 constexpr uint32_t kSegMaxChan = 256;    // channels of one plan at most
+constexpr uint32_t kSegWaves = 4;        // waves per workgroup (256 threads)
 constexpr uint32_t kSegMaxGroups = 64;   // work groups of one plan at most
 constexpr uint32_t kSegMaxHalo = 256;    // staged history samples at most
 struct SegGroup {
@@ -275,7 +280,7 @@ struct SegArgs {
     bool out16;
 };
 hipError_t launch_chan_seg(const SegArgs &a, hipStream_t s);
-size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo);
+size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo, uint32_t n_chan);
 // resident blocks of the fused kernel per CU for a block size, build and LDS size
 int chan_groups_occupancy(uint32_t block, bool wide, size_t lds_bytes);
 // advance the device position by n inputs and flip the history parity

@@ -3,9 +3,12 @@ batched default, against the oracle (reference
 src/transform/falcon_dsp_multi_rate_channelizer.cc:146-164 over
 falcon_dsp_freq_shift.cc:164-192 and falcon_dsp_polyphase_resampler.cc:256-319).
 
-Every output's arithmetic depends only on its own window, its channel's
-taps and its exact NCO index, so the outputs are bit-identical across
-segment lengths and across chunkings of the stream. The arithmetic is FP32
+Every output's window, taps and NCO index are exact whatever the segment
+length or the chunking of the stream; the NCO phasor advances by an FP32
+recurrence from an FP64 seed at each wave's first sub-tile of a segment, so
+segment lengths and chunkings agree to a few FP32 roundings (F32_TOL), and
+the same chunking gives bit-identical results eagerly and in graph mode.
+The arithmetic is FP32
 throughout (FP32 matrix cores and vector ALUs), as the reference's
 multiply-accumulates are (polyphase_resampler.cc:288,298): on the 32-channel
 workload its error against the oracle stays below F32_TOL x input RMS, a
@@ -87,19 +90,21 @@ def test_mixed_channel_set_against_oracle():
 
 
 @pytest.mark.parametrize("seg", [1024, 4096])
-def test_segment_lengths_bit_identical(seg):
+def test_segment_lengths_agree(seg):
     fs = 20_000_000
     chans = _mixed(fs)
     x = cwgn(32, 200_000)
     _, base = _run(fs, chans, x, [x.size], seg=2048)
     _, other = _run(fs, chans, x, [x.size], seg=seg)
     for c in range(len(chans)):
-        np.testing.assert_array_equal(other[c], base[c], err_msg=f"chan {c}")
+        assert other[c].size == base[c].size
+        assert_close_rms(other[c], base[c], rms(x), tol=F32_TOL, what=f"chan {c}")
 
 
-def test_chunkings_bit_identical():
+def test_chunkings_agree():
     # the NCO index and the window of every output are exact whatever the
-    # chunk boundaries, so any split of the stream gives the same samples
+    # chunk boundaries, so any split of the stream gives the same samples up
+    # to the phasor recurrence's FP32 rounding
     fs = 1_000_000
     chans = [channel(fs, 23_000, 50_000), channel(fs, -7, 50_000), channel(fs, 333_333, 250_000),
              channel(fs, 100_001, 12_500, taps_per_up=48)]
@@ -107,7 +112,8 @@ def test_chunkings_bit_identical():
     _, whole = _run(fs, chans, x, [x.size])
     _, split = _run(fs, chans, x, [434_780, 5, 1, 131_071, 3])
     for c in range(len(chans)):
-        np.testing.assert_array_equal(split[c], whole[c], err_msg=f"chan {c}")
+        assert split[c].size == whole[c].size
+        assert_close_rms(split[c], whole[c], rms(x), tol=F32_TOL, what=f"chan {c}")
 
 
 def test_fp32_faithful_on_the_chan32_slice():
