@@ -100,6 +100,20 @@ int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fd
 /* output of shift c starts at d_out + c * out_stride (in samples) */
 int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t n,
                                  fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream);
+/* n_chunks consecutive chunks of chunk_len samples from d_in in one launch
+ * (no reference counterpart: the reference makes one apply call per chunk,
+ * falcon_dsp_freq_shift_cuda.cu:355-488). The same outputs as n_chunks
+ * apply_device calls: shift c of chunk k at d_out + c*out_stride +
+ * k*chunk_len, the index carried across the chunks. */
+int fdsp_freq_shift_apply_chunks_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t chunk_len,
+                                        size_t n_chunks, fdsp_cf32 *d_out, size_t out_stride,
+                                        fdsp_stream_t stream);
+/* Restores a stream position: every shift continues as if samples_handled
+ * samples had been processed since reset (its index samples_handled mod R),
+ * as the reference's m_samples_handled carries it
+ * (hdr/transform/falcon_dsp_freq_shift.h:117; the reference has no setter).
+ * The position of a one-shift handle round-trips through get_state. */
+int fdsp_freq_shift_set_state(fdsp_freq_shift_t h, uint64_t samples_handled);
 /* ci16 in, ci16 out. Replaces falcon_dsp_freq_shift::apply(vector<complex<int16_t>>&, ...)
  * (reference src/transform/falcon_dsp_freq_shift.cc:133-162) and
  * falcon_dsp_freq_shift_cuda::apply(ci16) (falcon_dsp_freq_shift_cuda.cu:308-329):
@@ -131,6 +145,21 @@ int fdsp_resampler_apply(fdsp_resampler_t h, const fdsp_cf32 *in, size_t n, fdsp
                          size_t cap, size_t *n_out);
 int fdsp_resampler_apply_device(fdsp_resampler_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
                                 size_t cap, size_t *n_out, fdsp_stream_t stream);
+/* n_chunks consecutive chunks of chunk_len samples from d_in in one launch
+ * (no reference counterpart: the reference makes one apply call per chunk,
+ * falcon_dsp_polyphase_resampler_cuda.cu:459-619). The same outputs, back to
+ * back in d_out, as n_chunks apply_device calls; counts[k] receives chunk k's
+ * output count. */
+int fdsp_resampler_apply_chunks_device(fdsp_resampler_t h, const fdsp_cf32 *d_in, size_t chunk_len,
+                                       size_t n_chunks, fdsp_cf32 *d_out, size_t cap, size_t *counts,
+                                       fdsp_stream_t stream);
+/* Restores the reference's carried state (polyphase_resampler_params_s
+ * coeff_phase, xOffset and state, hdr/transform/falcon_dsp_polyphase_resampler.h:157-163;
+ * the reference has no setter): history holds the last coeffs_per_phase - 1
+ * inputs, oldest first. FDSP_EINVAL when no stream position yields the two
+ * fields. A state read with get_state plus the last inputs round-trips. */
+int fdsp_resampler_set_state(fdsp_resampler_t h, uint32_t coeff_phase, uint32_t x_offset,
+                             const fdsp_cf32 *history, size_t n_history);
 
 /* ---- FIR filter (complex taps) ----
  * Replaces falcon_dsp_fir_filter(coeffs) / apply(cf32) / reset_state

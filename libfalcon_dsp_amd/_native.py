@@ -209,6 +209,10 @@ _SIGS = [
     ("fdsp_channelizer_graph_destroy", _i, [_vp]),
     ("fdsp_channelizer_set_tiling", _i, [_vp, _u32, _u32]),
     ("fdsp_channelizer_get_tiling", _i, [_vp, _P(_u32), _P(_u32)]),
+    ("fdsp_freq_shift_apply_chunks_device", _i, [_vp, _vp, _sz, _sz, _vp, _sz, _vp]),
+    ("fdsp_freq_shift_set_state", _i, [_vp, _u64]),
+    ("fdsp_resampler_apply_chunks_device", _i, [_vp, _vp, _sz, _sz, _vp, _sz, _P(_sz), _vp]),
+    ("fdsp_resampler_set_state", _i, [_vp, _u32, _u32, _vp, _sz]),
     ("fdsp_channelizer_kernel_info", _i, [_vp, _P(_u32), _P(_u32), _P(_u32)]),
     ("fdsp_channelizer_set_kernel", _i, [_vp, _u32]),
     ("fdsp_channelizer_plan_info", _i, [_vp, _sz, _P(_u32), _P(_u32), _P(_u32), _P(_u32)]),

@@ -317,6 +317,18 @@ extern "C" int fdsp_freq_shift_reset(fdsp_freq_shift_t h)
     return FDSP_OK;
 }
 
+extern "C" int fdsp_freq_shift_set_state(fdsp_freq_shift_t h, uint64_t samples_handled)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (samples_handled >= (1ull << 52))
+        return set_err(FDSP_ERANGE, "set_state: position above 2^52");
+    // launches queued on caller streams read the position as an argument,
+    // so only the host copy changes
+    h->start = samples_handled;
+    return FDSP_OK;
+}
+
 extern "C" int fdsp_freq_shift_get_state(fdsp_freq_shift_t h, uint32_t chan, uint64_t *samples_handled)
 {
     if (!h)
@@ -414,6 +426,22 @@ extern "C" int fdsp_freq_shift_apply_device(fdsp_freq_shift_t h, const fdsp_cf32
 {
     return freq_shift_device(h, reinterpret_cast<const float2 *>(d_in), n, reinterpret_cast<float2 *>(d_out),
                              out_stride, stream);
+}
+
+extern "C" int fdsp_freq_shift_apply_chunks_device(fdsp_freq_shift_t h, const fdsp_cf32 *d_in, size_t chunk_len,
+                                                   size_t n_chunks, fdsp_cf32 *d_out, size_t out_stride,
+                                                   fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n_chunks == 0)
+        return FDSP_OK;
+    if (chunk_len == 0)
+        return set_err(FDSP_EINVAL, "apply_chunks: chunk_len > 0 needed");
+    // the shift of consecutive chunks continues the index across them, so the
+    // chunks' outputs (shift s, chunk k at d_out + s*out_stride + k*chunk_len)
+    // are those of one call over their concatenation
+    return fdsp_freq_shift_apply_device(h, d_in, chunk_len * n_chunks, d_out, out_stride, stream);
 }
 
 extern "C" int fdsp_freq_shift_apply(fdsp_freq_shift_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *const *outs)
@@ -543,6 +571,44 @@ extern "C" int fdsp_resampler_get_state(fdsp_resampler_t h, uint32_t *coeff_phas
     return FDSP_OK;
 }
 
+extern "C" int fdsp_resampler_set_state(fdsp_resampler_t h, uint32_t coeff_phase, uint32_t x_offset,
+                                        const fdsp_cf32 *history, size_t n_history)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    const size_t hlen = h->cpp > 1 ? h->cpp - 1 : 0;
+    if (coeff_phase >= h->up)
+        return set_err(FDSP_EINVAL, "set_state: coeff_phase %u is not below up = %u", coeff_phase, h->up);
+    if (n_history != hlen || (hlen && !history))
+        return set_err(FDSP_EINVAL, "set_state: the history holds coeffs_per_phase - 1 = %zu samples", hlen);
+    // The carried fields follow from the inputs consumed p: the next output m
+    // = ceil(p up/down) reads anchor (m down)/up, so m down = up (p + xOffset)
+    // + coeff_phase. The smallest m >= 1 giving an integer p >= 0 that
+    // reproduces (coeff_phase, xOffset) is taken (any such p yields the same
+    // outputs; they repeat with period up*down in m).
+    uint64_t p = 0;
+    bool found = coeff_phase == 0 && x_offset == 0;  // the state after reset
+    for (uint64_t m = 1; !found && m <= (uint64_t)h->up * h->down + h->up; ++m) {
+        const uint64_t num = m * h->down;
+        if (num % h->up != coeff_phase || num / h->up < x_offset)
+            continue;
+        p = num / h->up - x_offset;
+        uint32_t ph, xo;
+        uint64_t nn;
+        resampler_fields(p, h->up, h->down, &ph, &xo, &nn);
+        found = ph == coeff_phase && xo == x_offset;
+    }
+    if (!found)
+        return set_err(FDSP_EINVAL, "set_state: no stream position gives coeff_phase %u and xOffset %u (up %u, down %u)",
+                       coeff_phase, x_offset, h->up, h->down);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipDeviceSynchronize());  // launches on caller streams may still read the history
+    if (hlen)
+        HIP_TRY(hipMemcpy(h->d_hist[h->parity].p, history, hlen * sizeof(float2), hipMemcpyHostToDevice));
+    h->p = p;
+    return FDSP_OK;
+}
+
 extern "C" int fdsp_resampler_apply_device(fdsp_resampler_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
                                            size_t cap, size_t *n_out, fdsp_stream_t stream)
 {
@@ -568,6 +634,32 @@ extern "C" int fdsp_resampler_apply_device(fdsp_resampler_t h, const fdsp_cf32 *
     h->p += n;
     *n_out = count;
     return FDSP_OK;
+}
+
+extern "C" int fdsp_resampler_apply_chunks_device(fdsp_resampler_t h, const fdsp_cf32 *d_in, size_t chunk_len,
+                                                  size_t n_chunks, fdsp_cf32 *d_out, size_t cap, size_t *counts,
+                                                  fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (!counts || (n_chunks && (!d_in || chunk_len == 0)))
+        return set_err(FDSP_EINVAL, "apply_chunks: need counts, an input and chunk_len > 0");
+    // the resampler is chunk-invariant: n_chunks consecutive chunks emit, back
+    // to back, exactly the outputs of their concatenation, so one launch over
+    // the concatenation is the same as n_chunks calls
+    const uint64_t total = (uint64_t)chunk_len * n_chunks;
+    uint64_t p = h->p;
+    size_t sum = 0;
+    for (size_t k = 0; k < n_chunks; ++k, p += chunk_len) {
+        counts[k] = chunk_count(p, chunk_len, h->up, h->down);
+        sum += counts[k];
+    }
+    if (sum > cap || (sum && !d_out))
+        return set_err(FDSP_ERANGE, "output capacity %zu below the %zu samples these chunks emit", cap, sum);
+    size_t got = 0;
+    if (int rc = fdsp_resampler_apply_device(h, d_in, total, d_out, cap, &got, stream))
+        return rc;
+    return got == sum ? FDSP_OK : set_err(FDSP_EHIP, "apply_chunks: %zu outputs, expected %zu", got, sum);
 }
 
 extern "C" int fdsp_resampler_apply(fdsp_resampler_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out, size_t cap,

@@ -248,6 +248,16 @@ class FreqShift(_Handle):
         check(lib().fdsp_freq_shift_get_state(self.h, chan, C.byref(v)))
         return v.value
 
+    def set_state(self, samples_handled: int):
+        """Continue as if samples_handled samples had been processed (fdsp_freq_shift_set_state)."""
+        check(lib().fdsp_freq_shift_set_state(self.h, int(samples_handled)))
+
+    def apply_chunks_device(self, d_in: DeviceBuffer, chunk_len: int, n_chunks: int, d_out: DeviceBuffer,
+                            out_stride: int, stream: Optional[Stream] = None, in_offset: int = 0):
+        """n_chunks consecutive chunks in one launch (fdsp_freq_shift_apply_chunks_device)."""
+        check(lib().fdsp_freq_shift_apply_chunks_device(self.h, d_in.at(in_offset), chunk_len, n_chunks, d_out.ptr,
+                                                        out_stride, stream.h if stream else None))
+
     def apply(self, x):
         x = _cf32(x)
         outs = [np.empty(x.size, dtype=np.complex64) for _ in range(self.n_shifts)]
@@ -321,6 +331,22 @@ class PolyphaseResampler(_Handle):
         check(lib().fdsp_resampler_apply_device(self.h, d_in.at(in_offset), n, d_out.ptr, d_out.count,
                                                 C.byref(got), stream.h if stream else None))
         return got.value
+
+    def apply_chunks_device(self, d_in: DeviceBuffer, chunk_len: int, n_chunks: int, d_out: DeviceBuffer,
+                            stream: Optional[Stream] = None, in_offset: int = 0) -> List[int]:
+        """n_chunks consecutive chunks in one launch (fdsp_resampler_apply_chunks_device);
+        the outputs land back to back, the per-chunk counts are returned."""
+        counts = (C.c_size_t * max(n_chunks, 1))()
+        check(lib().fdsp_resampler_apply_chunks_device(self.h, d_in.at(in_offset), chunk_len, n_chunks, d_out.ptr,
+                                                       d_out.count, counts, stream.h if stream else None))
+        return list(counts[:n_chunks])
+
+    def set_state(self, coeff_phase: int, x_offset: int, history):
+        """Restore the reference's coeff_phase, xOffset and state (the last
+        coeffs_per_phase - 1 inputs, oldest first) (fdsp_resampler_set_state)."""
+        h = _cf32(history)
+        check(lib().fdsp_resampler_set_state(self.h, int(coeff_phase), int(x_offset), _ptr(h) if h.size else None,
+                                             h.size))
 
 
 class FirFilter(_Handle):

@@ -82,3 +82,44 @@ def test_device_form_matches_host_form():
 def test_empty_input_is_an_error():
     with pytest.raises(fd.FdspError):
         fd.FreqShift(1_000_000, 1000).apply(np.zeros(0, np.complex64))
+
+
+def test_apply_chunks_equals_per_chunk_calls():
+    # 16 chunks of 2^14 through 16 shifts in one launch
+    # (fdsp_freq_shift_apply_chunks_device) equal 16 per-chunk calls
+    w = Nco16(n=1 << 18, chunk=1 << 14)
+    x = w.input()
+    b = fd.FreqShift(w.fs, w.shifts())
+    d_in = fd.DeviceBuffer.from_array(x)
+    oa = fd.DeviceBuffer(16 * w.n)
+    ob = fd.DeviceBuffer(16 * w.n)
+    s = fd.Stream()
+    a2 = fd.FreqShift(w.fs, w.shifts())
+    for k in range(w.n // w.chunk):
+        tmp = fd.DeviceBuffer(16 * w.chunk)
+        a2.apply_device(d_in, w.chunk, tmp, w.chunk, s, in_offset=k * w.chunk)
+        s.sync()
+        t = tmp.download(16 * w.chunk).reshape(16, w.chunk)
+        for c in range(16):
+            oa.upload(t[c], offset=c * w.n + k * w.chunk)
+    b.apply_chunks_device(d_in, w.chunk, w.n // w.chunk, ob, w.n, s)
+    s.sync()
+    np.testing.assert_array_equal(ob.download(16 * w.n), oa.download(16 * w.n))
+    for c in range(16):
+        assert b.samples_handled(c) == a2.samples_handled(c)
+
+
+def test_set_state_continues_the_stream():
+    # a handle set to another's position continues its index exactly
+    fs = 1_000_000
+    x = cwgn(44, 600_000)
+    a, ref = fd.FreqShift(fs, 23_000), O.FreqShift(fs, 23_000)
+    a.apply(x[:434_000])
+    ref.apply(x[:434_000])
+    b = fd.FreqShift(fs, 23_000)
+    b.set_state(434_000)
+    assert b.samples_handled() == a.samples_handled()
+    ya, yb, r = a.apply(x[434_000:]), b.apply(x[434_000:]), ref.apply(x[434_000:])
+    np.testing.assert_array_equal(yb, ya)
+    assert_close_rms(yb, r, rms(x))
+    assert b.samples_handled() == int(ref.samples_handled)

@@ -121,3 +121,63 @@ def test_capacity_too_small_is_an_error():
         r.apply_device(d_in, 100, d_out)
     # the failed call left the state untouched
     assert r.state()[2] == 0
+
+
+def test_apply_chunks_equals_per_chunk_calls():
+    # one launch over 16 consecutive chunks (fdsp_resampler_apply_chunks_device)
+    # gives the per-chunk calls' outputs, back to back, and their counts
+    w = Resampler1ch()
+    x = w.input()
+    a = fd.PolyphaseResampler(w.up, w.down, w.taps())
+    b = fd.PolyphaseResampler(w.up, w.down, w.taps())
+    d_in = fd.DeviceBuffer.from_array(x)
+    ob = fd.DeviceBuffer(w.n * w.up // w.down + 16)
+    s = fd.Stream()
+    for rep in range(2):  # the second pass continues the stream
+        got_a = []
+        for k in range(w.n // w.chunk):
+            seg = fd.DeviceBuffer(w.chunk * w.up // w.down + 2)
+            cnt = a.apply_device(d_in, w.chunk, seg, s, in_offset=k * w.chunk)
+            s.sync()
+            got_a.append(seg.download(cnt))
+        counts = b.apply_chunks_device(d_in, w.chunk, w.n // w.chunk, ob, s)
+        s.sync()
+        assert counts == [g.size for g in got_a]
+        np.testing.assert_array_equal(ob.download(sum(counts)), np.concatenate(got_a))
+        assert a.state() == b.state()
+
+
+@pytest.mark.parametrize("up,down,cut", [(4, 5, 70_001), (1, 7, 3), (3, 2, 99_999), (160, 147, 12_345)])
+def test_set_state_round_trip(up, down, cut):
+    # a handle restored from another's (coeff_phase, xOffset) and its last
+    # coeffs_per_phase - 1 inputs continues the stream exactly as the original
+    # and as the oracle (fdsp_resampler_set_state)
+    taps = kaiser_sinc(24 * up, 0.5 / max(up, down), 8.0, gain=up).astype(np.complex64)
+    x = cwgn(43, 200_000)
+    a = fd.PolyphaseResampler(up, down, taps)
+    ref = O.Resampler(up, down, taps)
+    a.apply(x[:cut])
+    ref.apply(x[:cut])
+    ph, xo, _, _ = a.state()
+    cpp = a.coeffs_per_phase()
+    # the reference's state: the last cpp - 1 inputs, zeros before the stream's start
+    hist = np.concatenate([np.zeros(max(0, cpp - 1 - cut), np.complex64), x[max(0, cut - (cpp - 1)):cut]])
+    b = fd.PolyphaseResampler(up, down, taps)
+    b.set_state(ph, xo, hist)
+    assert b.state()[:2] == (ph, xo)
+    for lo, hi in chunks(x.size - cut, [50_000, 1, 17]):
+        ya, yb = a.apply(x[cut + lo:cut + hi]), b.apply(x[cut + lo:cut + hi])
+        r = ref.apply(x[cut + lo:cut + hi])
+        np.testing.assert_array_equal(yb, ya)
+        assert yb.size == r.size
+        assert_close_rms(yb, r, rms(x))
+        assert b.state()[:2] == a.state()[:2] == ref.state_fields
+
+
+def test_set_state_rejects_bad_fields():
+    r = fd.PolyphaseResampler(4, 5, np.ones(96, np.complex64))
+    with pytest.raises(fd.FdspError):
+        r.set_state(4, 0, np.zeros(23, np.complex64))   # coeff_phase >= up
+    with pytest.raises(fd.FdspError):
+        r.set_state(0, 0, np.zeros(5, np.complex64))    # history is cpp - 1 = 23 samples
+    assert r.state()[:2] == (0, 0)
