@@ -848,7 +848,8 @@ extern "C" int fdsp_iir_reset(fdsp_iir_t h)
     if (!h)
         return set_err(FDSP_ENOTINIT, "null handle");
     std::lock_guard<std::mutex> lk(h->mu);
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    // launches on caller streams may still write the copy being cleared
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemset(h->d_xh[h->parity].p, 0, h->d_xh[h->parity].bytes));
     HIP_TRY(hipMemset(h->d_yh[h->parity].p, 0, h->d_yh[h->parity].bytes));
     return FDSP_OK;
@@ -909,10 +910,10 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     if (int rc = check_chunk(n, 1))
         return rc;
     // outputs per lane: short segments give short chunks enough waves, long
-    // ones keep the warm-up overhead down on long chunks (8 up to 2^21
+    // ones keep the warm-up overhead down on long chunks (8 below 2^21
     // samples: measured fastest on 2^20-sample chunks of a 4th-order filter)
     uint32_t seg = 8;
-    while (seg < kMaxSeg && (size_t)seg * 2 * 131072 <= n)
+    while (seg < kMaxSeg && (size_t)seg * 2 * 131072 < n)
         seg *= 2;
     const size_t nseg = (n + seg - 1) / seg, nblk = (n + kLanes * seg - 1) / (kLanes * seg);
     if (int rc = h->d_rs.ensure(nseg * h->p * sizeof(float2)))

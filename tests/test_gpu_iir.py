@@ -212,3 +212,22 @@ def test_errors():
     g = fd.IirFilter([1.0], [1.0, -0.5])
     with pytest.raises(fd.FdspError):
         g.apply(np.zeros(0, np.complex64))
+
+
+def test_reset_after_apply_on_a_caller_stream():
+    # reset waits for a launch still running on the caller's stream (which
+    # writes the carried state) before it clears that state
+    from scipy.signal import butter
+    b, a = butter(4, 0.2)
+    x = cwgn(95, 1 << 22)
+    f, g = fd.IirFilter(b, a), fd.IirFilter(b, a)
+    d_in = fd.DeviceBuffer.from_array(x)
+    d_out = fd.DeviceBuffer(x.size)
+    s = fd.Stream()
+    f.apply_device(d_in, x.size, d_out, s)  # no wait before the reset
+    f.reset_state()
+    ya, yb = fd.DeviceBuffer(5000), fd.DeviceBuffer(5000)
+    f.apply_device(d_in, 5000, ya, s)
+    g.apply_device(d_in, 5000, yb, s)
+    s.sync()
+    np.testing.assert_array_equal(ya.download(5000), yb.download(5000))
