@@ -266,6 +266,21 @@ int fdsp_iir_apply_f32_device(fdsp_iir_t h, const float *d_in, size_t n, float *
 int fdsp_iir_apply_ci16_device(fdsp_iir_t h, const fdsp_ci16 *d_in, size_t n, fdsp_ci16 *d_out,
                                fdsp_stream_t stream);
 int fdsp_iir_apply_i16_device(fdsp_iir_t h, const int16_t *d_in, size_t n, int16_t *d_out, fdsp_stream_t stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). The same
+ * filtering as fdsp_iir_apply_device, with the kernels' per-segment state
+ * records and block flags in a 16 MiB array that the library holds on each
+ * device until the process ends, in place of buffers grown per handle. It
+ * calls no allocator. A chunk needs 2 * roundup256(ceil(n/seg) * P * 8) +
+ * 8 * ceil(n / (64 seg)) bytes of it (P: the order rounded up to a power of
+ * two; seg: 8 for n <= 2^21, 16 for n <= 2^22, 32 for n <= 2^23, else 64):
+ * chunks of up to 2^22 samples fit at orders 1-2, 2^20 at orders 3-4, 2^19 at
+ * 5-8 and 2^18 at 9-16; a larger chunk fails with FDSP_ERANGE and changes
+ * nothing. Every such call on one device
+ * shares the array: calls that overlap on a device (other streams, other
+ * threads) race and must be serialised by the caller. */
+int fdsp_iir_apply_device_noalloc(fdsp_iir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                  fdsp_stream_t stream);
 
 /* ---- first-order IIR (real samples) ----
  * Replaces falcon_dsp_iir_filter::{initialize(b = [b0], a = [1, a1]),

@@ -178,6 +178,7 @@ _SIGS = [
     ("fdsp_iir_apply_ci16", _i, [_vp, _vp, _sz, _vp]),
     ("fdsp_iir_apply_i16", _i, [_vp, _vp, _sz, _vp]),
     ("fdsp_iir_apply_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_iir_apply_device_noalloc", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_iir_apply_f32_device", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_iir_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_iir_apply_i16_device", _i, [_vp, _vp, _sz, _vp, _vp]),

@@ -901,7 +901,7 @@ extern "C" int fdsp_iir_set_mode(fdsp_iir_t h, int mode)
 namespace fdsp {
 
 int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint32_t tout, void *d_out,
-                    hipStream_t s)
+                    hipStream_t s, void *work, size_t work_bytes)
 {
     if (n == 0)
         return set_err(FDSP_EINVAL, "empty input");
@@ -916,12 +916,31 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     while (seg < kMaxSeg && (size_t)seg * 2 * 131072 < n)
         seg *= 2;
     const size_t nseg = (n + seg - 1) / seg, nblk = (n + kLanes * seg - 1) / (kLanes * seg);
-    if (int rc = h->d_rs.ensure(nseg * h->p * sizeof(float2)))
-        return rc;
-    if (int rc = h->d_re.ensure(nseg * h->p * sizeof(float2)))
-        return rc;
-    if (int rc = h->d_bad.ensure(nblk * sizeof(unsigned long long)))
-        return rc;
+    // every record is written by this call's iir_kernel before anything reads
+    // it, so the buffers need no clearing
+    const size_t rec_bytes = (nseg * h->p * sizeof(float2) + 255) & ~(size_t)255;
+    float2 *rs, *re;
+    unsigned long long *bad;
+    if (work) {
+        const size_t need = 2 * rec_bytes + nblk * sizeof(unsigned long long);
+        if (need > work_bytes)
+            return set_err(FDSP_ERANGE,
+                           "iir: a %zu-sample chunk of an order-%u filter needs %zu bytes of work memory, %zu given",
+                           n, h->nfb, need, work_bytes);
+        rs = static_cast<float2 *>(work);
+        re = reinterpret_cast<float2 *>(static_cast<unsigned char *>(work) + rec_bytes);
+        bad = reinterpret_cast<unsigned long long *>(static_cast<unsigned char *>(work) + 2 * rec_bytes);
+    } else {
+        if (int rc = h->d_rs.ensure(nseg * h->p * sizeof(float2)))
+            return rc;
+        if (int rc = h->d_re.ensure(nseg * h->p * sizeof(float2)))
+            return rc;
+        if (int rc = h->d_bad.ensure(nblk * sizeof(unsigned long long)))
+            return rc;
+        rs = h->d_rs.as<float2>();
+        re = h->d_re.as<float2>();
+        bad = h->d_bad.as<unsigned long long>();
+    }
     IirLaunch g{};
     g.x = d_in;
     g.out = d_out;
@@ -930,9 +949,9 @@ int iir_apply_typed(fdsp_iir_t h, uint32_t tin, const void *d_in, size_t n, uint
     g.yh = h->d_yh[h->parity].as<float2>();
     g.xh_next = h->d_xh[h->parity ^ 1].as<float2>();
     g.yh_next = h->d_yh[h->parity ^ 1].as<float2>();
-    g.rs = h->d_rs.as<float2>();
-    g.re = h->d_re.as<float2>();
-    g.bad = h->d_bad.as<unsigned long long>();
+    g.rs = rs;
+    g.re = re;
+    g.bad = bad;
     g.failed = h->d_failed.as<uint32_t>();
     g.n = (uint32_t)n;
     g.nb = h->nb;
@@ -1026,6 +1045,56 @@ extern "C" int fdsp_iir_apply_device(fdsp_iir_t h, const fdsp_cf32 *d_in, size_t
 {
     return apply_device(h, kSampleCf32, d_in, n, d_out, stream);
 }
+// ---------------------------------------------------------------------------
+// the form that allocates nothing
+// ---------------------------------------------------------------------------
+namespace {
+
+// Work memory of fdsp_iir_apply_device_noalloc: one copy of this array on
+// every device that runs the library, held until the process ends. All such
+// calls on one device share it, so calls that overlap on a device race.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+constexpr int kScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices];  // the array's address on each device, looked up once
+
+// the current device's copy of `scratch` (the host-side name of a __device__
+// array is no device address)
+int scratch_on_current_device(void **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_err(FDSP_EINVAL, "iir noalloc: device %d beyond the %d supported", dev, kScratchDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return set_err(FDSP_EHIP, "iir noalloc: no device address for the work array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FDSP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdsp_iir_apply_device_noalloc(fdsp_iir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
+                                             fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    void *work = nullptr;
+    if (int rc = scratch_on_current_device(&work))
+        return rc;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return iir_apply_typed(h, kSampleCf32, d_in, n, kSampleCf32, d_out, pick(h->stream, stream), work,
+                           kScratchBytes);
+}
+
 extern "C" int fdsp_iir_apply_f32_device(fdsp_iir_t h, const float *d_in, size_t n, float *d_out,
                                          fdsp_stream_t stream)
 {

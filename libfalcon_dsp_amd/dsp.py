@@ -524,6 +524,12 @@ class IirFilter(_Handle):
               "ci16": "fdsp_iir_apply_ci16_device", "i16": "fdsp_iir_apply_i16_device"}[kind]
         check(getattr(lib(), fn)(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
 
+    def apply_device_noalloc(self, d_in: DeviceBuffer, n: int, d_out: DeviceBuffer, stream: Optional[Stream] = None):
+        """Complex float buffers; the kernels' work memory is the library's static
+        per-device array (fdsp_iir_apply_device_noalloc): nothing is allocated,
+        and overlapping calls on one device race."""
+        check(lib().fdsp_iir_apply_device_noalloc(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
+
 
 def iir_filter(b, a, x) -> np.ndarray:
     """One-shot IIR filter (reference iir_filter.cc:85-100): a fresh filter over x."""
