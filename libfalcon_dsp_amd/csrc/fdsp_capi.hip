@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <sys/stat.h>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -28,6 +29,18 @@
 
 using namespace fdsp;
 
+namespace {
+// 64-bit FNV-1a over byte ranges
+struct Fnv64 {
+    uint64_t h = 0xcbf29ce484222325ull;
+    void add(const void *p, size_t n)
+    {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; ++i)
+            h = (h ^ b[i]) * 0x100000001b3ull;
+    }
+};
+}  // namespace
 
 // ===========================================================================
 // misc + plumbing
@@ -1084,6 +1097,7 @@ struct fdsp_channelizer_s {
     bool pos_dirty = false;         // device position behind the host's (eager calls since the last graph launch)
     uint32_t taps_total = 0, hist_len = 0;
     uint32_t seg_len = FDSP_TILING_STRIDED, block = 256;
+    uint64_t tune_hash = 0;  // FNV-1a of fs, the mode and every channel's description (tuning-cache key)
     // FDSP_CHAN_BATCHED_BF16X2: the group kernel's MFMA groups with 8-aligned
     // windows run on bf16 hi/lo planes (about 16 significant bits)
     bool bf16_planes = false;
@@ -1871,6 +1885,19 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
     h->fs = fs;
     h->mode = mode == FDSP_CHAN_PLAIN ? FDSP_CHAN_PLAIN : FDSP_CHAN_BATCHED;
     h->bf16_planes = mode == FDSP_CHAN_BATCHED_BF16X2;
+    {
+        Fnv64 f;
+        f.add(&fs, sizeof fs);
+        f.add(&mode, sizeof mode);
+        for (uint32_t c = 0; c < n_chan; ++c) {
+            const fdsp_channel_desc &d = chans[c];
+            const uint32_t w[6] = {d.output_sample_rate_in_sps, (uint32_t)d.freq_shift_in_hz, d.up_rate,
+                                   d.down_rate, d.n_taps, d.tail};
+            f.add(w, sizeof w);
+            f.add(d.taps, (size_t)d.n_taps * sizeof(fdsp_cf32));
+        }
+        h->tune_hash = f.h;
+    }
     h->ch.resize(n_chan);
     int rc = FDSP_OK;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
@@ -2453,22 +2480,136 @@ extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len
     return FDSP_OK;
 }
 
-extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len,
-                                         uint32_t reps, fdsp_tiling_trial *trials, size_t cap, size_t *n_trials)
+// ---------------------------------------------------------------------------
+// tuning cache: one line per tuned setting,
+//   v1 <key> = <segment_len> <block_threads> <fused> <us> [<group weight> ...]
+// the last line of a key wins. The file: FDSP_TUNE_CACHE when set ("" or
+// "off": no cache), else $XDG_CACHE_HOME/fdsp/tuning.txt or
+// $HOME/.cache/fdsp/tuning.txt. A cache that cannot be read or written is
+// skipped silently (the trials then run every time).
+// ---------------------------------------------------------------------------
+struct TuneRecord {
+    uint32_t seg = 0, block = 256, fused = 0;
+    float us = 0.f;
+    std::vector<double> weights;
+};
+
+static std::string tune_cache_path(bool create_dirs)
 {
-    if (!h)
-        return set_err(FDSP_ENOTINIT, "null handle");
-    if (h->mode != FDSP_CHAN_BATCHED)
-        return set_err(FDSP_EINVAL, "autotuning applies to the batched mode");
-    if (!d_in || chunk_len == 0 || reps == 0)
-        return set_err(FDSP_EINVAL, "autotune: need a device chunk and reps > 0");
-    if (h->graph_exec)
-        return set_err(FDSP_EBUSY, "destroy the graph before tuning");
-    uint32_t max_up = 1;
-    for (auto &k : h->ch)
-        max_up = std::max(max_up, k.up);
-    if (int rc = check_chunk(chunk_len, max_up))
-        return rc;
+    if (const char *e = std::getenv("FDSP_TUNE_CACHE"))
+        return std::strcmp(e, "off") == 0 ? std::string() : std::string(e);
+    std::string dir;
+    if (const char *x = std::getenv("XDG_CACHE_HOME"); x && *x)
+        dir = x;
+    else if (const char *home = std::getenv("HOME"); home && *home)
+        dir = std::string(home) + "/.cache";
+    else
+        return std::string();
+    if (create_dirs) {
+        (void)mkdir(dir.c_str(), 0755);
+        (void)mkdir((dir + "/fdsp").c_str(), 0755);
+    }
+    return dir + "/fdsp/tuning.txt";
+}
+
+static int tune_key(const fdsp_channelizer_s *h, size_t chunk_len, bool in16, std::string *key)
+{
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    char arch[64];
+    std::snprintf(arch, sizeof arch, "%s", prop.gcnArchName);
+    for (char *c = arch; *c; ++c)
+        if (*c == ' ' || *c == '=' || *c == ':')
+            *c = '_';
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%016llx %s cu%d n%zu a%d %s", (unsigned long long)h->tune_hash, arch, cus,
+                  chunk_len, in16 ? 1 : 0, h->use_seg ? "seg" : h->bf16_planes ? "groups-bf16x2" : "groups");
+    *key = buf;
+    return FDSP_OK;
+}
+
+static bool tune_cache_get(const std::string &key, TuneRecord *out)
+{
+    const std::string path = tune_cache_path(false);
+    if (path.empty())
+        return false;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f)
+        return false;
+    const std::string head = "v1 " + key + " = ";
+    char line[4096];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        if (std::strncmp(line, head.c_str(), head.size()) != 0)
+            continue;
+        TuneRecord r;
+        const char *p = line + head.size();
+        char *end = nullptr;
+        const unsigned long seg = std::strtoul(p, &end, 10);
+        if (end == p)
+            continue;
+        p = end;
+        const unsigned long block = std::strtoul(p, &end, 10);
+        if (end == p)
+            continue;
+        p = end;
+        const unsigned long fused = std::strtoul(p, &end, 10);
+        if (end == p)
+            continue;
+        p = end;
+        r.us = std::strtof(p, &end);
+        if (end == p)
+            continue;
+        p = end;
+        for (;;) {
+            const double w = std::strtod(p, &end);
+            if (end == p)
+                break;
+            r.weights.push_back(w);
+            p = end;
+        }
+        r.seg = (uint32_t)seg;
+        r.block = (uint32_t)block;
+        r.fused = (uint32_t)fused;
+        *out = r;
+        found = true;  // later lines override earlier ones
+    }
+    std::fclose(f);
+    return found;
+}
+
+static void tune_cache_put(const std::string &key, const TuneRecord &r)
+{
+    const std::string path = tune_cache_path(true);
+    if (path.empty())
+        return;
+    FILE *f = std::fopen(path.c_str(), "a");
+    if (!f)
+        return;
+    std::string line = "v1 " + key + " = " + std::to_string(r.seg) + " " + std::to_string(r.block) + " " +
+                       std::to_string(r.fused);
+    char num[64];
+    std::snprintf(num, sizeof num, " %.3f", (double)r.us);
+    line += num;
+    for (double w : r.weights) {
+        std::snprintf(num, sizeof num, " %.17g", w);
+        line += num;
+    }
+    line += "\n";
+    // one write of the whole line: appends from concurrent processes do not interleave
+    (void)std::fwrite(line.data(), 1, line.size(), f);
+    std::fclose(f);
+}
+
+// The autotuner's trials for chunk_len on d_in; the fastest setting is
+// applied, and its time returned in *best_us.
+static int autotune_run(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len, uint32_t reps,
+                        fdsp_tiling_trial *trials, size_t cap, size_t *n_trials, float *best_us)
+{
+    *best_us = -1.f;
     // the splits are timed unfused; fusion is a trial of its own at the end
     HIP_TRY(hipDeviceSynchronize());
     if (int rc = use_fusion(h, false))
@@ -2531,6 +2672,7 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
         (void)hipEventDestroy(t1);
         h->stat_valid = false;
         h->seg_len_in = rc ? seg0 : best_seg;
+        *best_us = best;
         if (n_trials)
             *n_trials = k;
         return rc;
@@ -2666,9 +2808,64 @@ extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *
     h->stat_valid = false;
     h->fuse_on = !rc && best_fused;
     const int rc2 = fdsp_channelizer_set_tiling(h, rc ? seg0 : best_seg, rc ? block0 : best_block);
+    *best_us = best;
     if (n_trials)
         *n_trials = k;
     return rc ? rc : rc2;
+}
+
+extern "C" int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len,
+                                         uint32_t reps, fdsp_tiling_trial *trials, size_t cap, size_t *n_trials)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (h->mode != FDSP_CHAN_BATCHED)
+        return set_err(FDSP_EINVAL, "autotuning applies to the batched mode");
+    if (!d_in || chunk_len == 0 || reps == 0)
+        return set_err(FDSP_EINVAL, "autotune: need a device chunk and reps > 0");
+    if (h->graph_exec)
+        return set_err(FDSP_EBUSY, "destroy the graph before tuning");
+    uint32_t max_up = 1;
+    for (auto &k : h->ch)
+        max_up = std::max(max_up, k.up);
+    if (int rc = check_chunk(chunk_len, max_up))
+        return rc;
+    // a setting tuned before for this channel set, chunk length, input
+    // alignment, kernel and device is taken from the tuning cache
+    std::string key;
+    if (int rc = tune_key(h, chunk_len, reinterpret_cast<uintptr_t>(d_in) % 16 == 0, &key))
+        return rc;
+    TuneRecord rec;
+    if (tune_cache_get(key, &rec)) {
+        HIP_TRY(hipDeviceSynchronize());
+        if (h->use_seg) {
+            h->seg_len_in = rec.seg;
+            h->stat_valid = false;
+        } else {
+            h->gcost = rec.weights.size() == h->n_groups ? rec.weights : std::vector<double>();
+            h->fuse_on = rec.fused != 0;
+            if (int rc = fdsp_channelizer_set_tiling(h, rec.seg, rec.block))
+                return rc;
+        }
+        if (trials && cap)
+            trials[0] = fdsp_tiling_trial{rec.seg, rec.block, rec.us,
+                                          (rec.weights.empty() ? 0u : 1u) | (rec.fused ? 2u : 0u) |
+                                              (h->use_seg ? 4u : 0u) | 8u};
+        if (n_trials)
+            *n_trials = 1;
+        return FDSP_OK;
+    }
+    float best = -1.f;
+    const int rc = autotune_run(h, d_in, chunk_len, reps, trials, cap, n_trials, &best);
+    if (!rc && best >= 0.f) {
+        rec.seg = h->use_seg ? h->seg_len_in : h->seg_len;
+        rec.block = h->use_seg ? 256u : h->block;
+        rec.fused = h->fuse_on ? 1u : 0u;
+        rec.us = best;
+        rec.weights = h->use_seg ? std::vector<double>() : h->gcost;
+        tune_cache_put(key, rec);
+    }
+    return rc;
 }
 
 extern "C" int fdsp_channelizer_set_fusion(fdsp_channelizer_t h, int on)

@@ -531,6 +531,46 @@ class IirFilter(_Handle):
         check(lib().fdsp_iir_apply_device_noalloc(self.h, d_in.ptr, n, d_out.ptr, stream.h if stream else None))
 
 
+class IirBank(_Handle):
+    """n_streams independent falcon_dsp_iir_filter streams sharing (b, a),
+    each bit-identical to the reference's sequential loop (one stream per GPU
+    lane; fdsp_iir_bank_*). apply takes an (n_streams, n) array."""
+    _destroy = "fdsp_iir_bank_destroy"
+
+    def __init__(self, b, a, n_streams: int):
+        super().__init__()
+        self._b = np.ascontiguousarray(b, dtype=np.complex64)
+        self._a = np.ascontiguousarray(a, dtype=np.complex64)
+        self.n_streams = int(n_streams)
+        check(lib().fdsp_iir_bank_create(_ptr(self._b), self._b.size, _ptr(self._a), self._a.size,
+                                         self.n_streams, C.byref(self.h)))
+
+    def reset_state(self):
+        check(lib().fdsp_iir_bank_reset(self.h))
+
+    def apply(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex64)
+        if x.ndim != 2 or x.shape[0] != self.n_streams:
+            raise ValueError(f"need an ({self.n_streams}, n) array")
+        out = np.empty_like(x)
+        check(lib().fdsp_iir_bank_apply(self.h, _ptr(x), x.shape[1], _ptr(out)))
+        return out
+
+    def apply_ci16(self, x) -> np.ndarray:
+        """x: (n_streams, n, 2) int16 (re, im) pairs; returns the same layout"""
+        x = np.ascontiguousarray(x, dtype=np.int16)
+        if x.ndim != 3 or x.shape[0] != self.n_streams or x.shape[2] != 2:
+            raise ValueError(f"need an ({self.n_streams}, n, 2) int16 array")
+        out = np.empty_like(x)
+        check(lib().fdsp_iir_bank_apply_ci16(self.h, _ptr(x), x.shape[1], _ptr(out)))
+        return out
+
+    def apply_device(self, d_in: DeviceBuffer, n: int, in_stride: int, d_out: DeviceBuffer, out_stride: int,
+                     stream: Optional[Stream] = None, ci16: bool = False):
+        fn = lib().fdsp_iir_bank_apply_ci16_device if ci16 else lib().fdsp_iir_bank_apply_device
+        check(fn(self.h, d_in.ptr, n, in_stride, d_out.ptr, out_stride, stream.h if stream else None))
+
+
 def iir_filter(b, a, x) -> np.ndarray:
     """One-shot IIR filter (reference iir_filter.cc:85-100): a fresh filter over x."""
     x = np.asarray(x)
@@ -704,14 +744,16 @@ class MultiRateChannelizer(_Handle):
 
     def autotune(self, d_in: DeviceBuffer, chunk_len: int, reps: int = 10, in_offset: int = 0) -> List[dict]:
         """Time every candidate work split on a device chunk and keep the
-        fastest (fdsp_channelizer_autotune); returns the trials."""
+        fastest (fdsp_channelizer_autotune); returns the trials. A pick stored
+        in the tuning cache (FDSP_TUNE_CACHE) comes back as one trial with
+        "cached" set."""
         cap = 32
         tr = (N.TilingTrial * cap)()
         k = C.c_size_t()
         check(lib().fdsp_channelizer_autotune(self.h, d_in.at(in_offset), chunk_len, reps, tr, cap, C.byref(k)))
         return [{"segment_len": t.segment_len, "block_threads": t.block_threads, "us": round(t.us, 2),
                  "measured_weights": bool(t.flags & 1), "fused": bool(t.flags & 2),
-                 "segment_kernel": bool(t.flags & 4)} for t in tr[: k.value]]
+                 "segment_kernel": bool(t.flags & 4), "cached": bool(t.flags & 8)} for t in tr[: k.value]]
 
     def set_fusion(self, on: bool):
         """Phase grouping: bf16-plane groups share staged spans (fdsp_channelizer_set_fusion)."""

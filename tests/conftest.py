@@ -10,6 +10,9 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels through the C ABI)")
+    # the autotuner's persistent cache stays out of the tests unless a test
+    # names a file of its own (every autotune call then runs its trials)
+    os.environ["FDSP_TUNE_CACHE"] = "off"
     # a fresh checkout has no built libraries: compile them once (the HIP
     # library cross-compiles for gfx950 without a GPU)
     if not os.path.exists(os.path.join(ROOT, "libfalcon_dsp_amd", "libfdsp_hip.so")):

@@ -280,6 +280,36 @@ def test_autotune_keeps_stream_state(kernel, least):
             assert g.get_tiling() == (best["segment_len"], best["block_threads"])
 
 
+@pytest.mark.parametrize("kernel", ["segment", "groups"])
+def test_autotune_pick_persists(kernel, tmp_path, monkeypatch):
+    # the pick goes into the tuning cache file; a new channelizer with the same
+    # channels, chunk length and kernel takes it from there without trials,
+    # and a different chunk length (another key) runs its trials again
+    cache = tmp_path / "tuning.txt"
+    monkeypatch.setenv("FDSP_TUNE_CACHE", str(cache))
+    fs = 1_000_000
+    chans = _chans_grouped(fs)
+    d = fd.DeviceBuffer.from_array(cwgn(24, 131_072))
+    a = fd.MultiRateChannelizer(fs, chans)
+    a.set_kernel(kernel)
+    first = a.autotune(d, 131_072, reps=2)
+    assert len(first) > 1 and not any(t["cached"] for t in first)
+    assert cache.read_text().count("\n") == 1
+    b = fd.MultiRateChannelizer(fs, chans)
+    b.set_kernel(kernel)
+    again = b.autotune(d, 131_072, reps=2)
+    assert len(again) == 1 and again[0]["cached"]
+    assert b.get_tiling() == a.get_tiling() == (again[0]["segment_len"], again[0]["block_threads"])
+    assert b.get_fusion()[0] == a.get_fusion()[0]
+    # the cached plan computes what the tuned one does
+    x = cwgn(25, 100_000)
+    for ya, yb in zip(a.apply(x), b.apply(x)):
+        np.testing.assert_array_equal(yb, ya)
+    other = b.autotune(d, 65_536, reps=2)
+    assert len(other) > 1 and not any(t["cached"] for t in other)
+    assert cache.read_text().count("\n") == 2
+
+
 @pytest.mark.parametrize("seg,block", [(256, 128), (1024, 256), (2048, 512), (512, 512), (4096, 512),
                                        (fd.TILING_STRIDED, 128), (fd.TILING_CLAIM, 512)])
 def test_tilings_agree(seg, block):
