@@ -282,6 +282,30 @@ int fdsp_iir_apply_i16_device(fdsp_iir_t h, const int16_t *d_in, size_t n, int16
 int fdsp_iir_apply_device_noalloc(fdsp_iir_t h, const fdsp_cf32 *d_in, size_t n, fdsp_cf32 *d_out,
                                   fdsp_stream_t stream);
 
+/* ---- IIR bank: many streams, each exact ----
+ * n_streams independent falcon_dsp_iir_filter instances sharing one (b, a)
+ * (reference src/transform/falcon_dsp_iir_filter.cc:104-396 per stream):
+ * each stream's outputs equal the reference's sequential loop on that
+ * stream, bit for bit up to the sign of zero, for every filter (the bank
+ * runs one stream per GPU lane, so no filter needs the serial plan). Rows:
+ * stream s's n samples at d_in + s * in_stride, its outputs at d_out + s *
+ * out_stride (strides in samples, >= n); the host forms take n_streams
+ * contiguous rows of n. Every stream advances by n samples per call; its
+ * carried inputs and outputs stay on the device. int16 outputs are truncated
+ * toward zero and saturated, as fdsp_iir_apply_ci16. Limits: 1 <= nb <= 17,
+ * 1 <= na <= 17, a[0] != 0. Calls on one bank must be ordered (one stream). */
+typedef struct fdsp_iir_bank_s *fdsp_iir_bank_t;
+int fdsp_iir_bank_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 *a, uint32_t na, uint32_t n_streams,
+                         fdsp_iir_bank_t *out);
+int fdsp_iir_bank_destroy(fdsp_iir_bank_t h);
+int fdsp_iir_bank_reset(fdsp_iir_bank_t h);
+int fdsp_iir_bank_apply(fdsp_iir_bank_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out);
+int fdsp_iir_bank_apply_ci16(fdsp_iir_bank_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out);
+int fdsp_iir_bank_apply_device(fdsp_iir_bank_t h, const fdsp_cf32 *d_in, size_t n, size_t in_stride,
+                               fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream);
+int fdsp_iir_bank_apply_ci16_device(fdsp_iir_bank_t h, const fdsp_ci16 *d_in, size_t n, size_t in_stride,
+                                    fdsp_ci16 *d_out, size_t out_stride, fdsp_stream_t stream);
+
 /* ---- first-order IIR (real samples) ----
  * Replaces falcon_dsp_iir_filter::{initialize(b = [b0], a = [1, a1]),
  * apply(vector<float>&, vector<float>&), reset_state} (reference
@@ -465,12 +489,20 @@ int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len, uint32_t 
  * write scratch outputs and leave the stream position and history as they
  * were. trials
  * (optional, cap entries, up to 17) receives every candidate's average kernel
- * time (us < 0: the split does not fit this channel set). */
+ * time (us < 0: the split does not fit this channel set).
+ * The pick is stored in a tuning cache file, keyed by the channel set (fs,
+ * mode, every channel's description and taps), chunk_len, the input's 16-byte
+ * alignment, the kernel and the device (architecture and CU count); a later
+ * call with the same key, in any process, applies the stored pick without
+ * running trials and reports it as the one trial, with flag bit 3 set. The
+ * file is FDSP_TUNE_CACHE when that is set ("off": no cache), else
+ * $XDG_CACHE_HOME/fdsp/tuning.txt or $HOME/.cache/fdsp/tuning.txt; one text
+ * line per pick, the last line of a key wins (delete it to re-tune). */
 typedef struct fdsp_tiling_trial {
     uint32_t segment_len, block_threads;
     float us;
     uint32_t flags; /* bit 0: groups weighed by measured work; bit 1: bf16-plane groups fused;
-                       bit 2: segment kernel (segment_len is its segment) */
+                       bit 2: segment kernel (segment_len is its segment); bit 3: from the tuning cache */
 } fdsp_tiling_trial;
 int fdsp_channelizer_autotune(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chunk_len, uint32_t reps,
                               fdsp_tiling_trial *trials, size_t cap, size_t *n_trials);

@@ -7,7 +7,7 @@ include/fdsp.h.
 """
 from ._native import FdspError, LIB_PATH, lib  # noqa: F401
 from .dsp import (  # noqa: F401
-    CI16, DeviceBuffer, Event, FirBank, FmDemodulator, Iir1, IirFilter, fm_deemphasis, iir_filter, resample_fir_params, FILTER_REMEZ, FILTER_FIRLS, FILTER_TAPS_OPTIMAL, FirFilter, FreqShift, MultiRateChannelizer, PolarDiscriminator,
+    CI16, DeviceBuffer, Event, FirBank, FmDemodulator, Iir1, IirBank, IirFilter, fm_deemphasis, iir_filter, resample_fir_params, FILTER_REMEZ, FILTER_FIRLS, FILTER_TAPS_OPTIMAL, FirFilter, FreqShift, MultiRateChannelizer, PolarDiscriminator,
     PolyphaseResampler, Server, Stream, TILING_CLAIM, TILING_STRIDED, calculate_gcd, calculate_lcm, device_count,
     device_sync, fir_filter, freq_shift, get_freq_shift_params, polar_discriminator, rat_approx, reduce_rates, resample, upfirdn,
 )

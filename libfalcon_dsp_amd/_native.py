@@ -182,6 +182,13 @@ _SIGS = [
     ("fdsp_iir_apply_f32_device", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_iir_apply_ci16_device", _i, [_vp, _vp, _sz, _vp, _vp]),
     ("fdsp_iir_apply_i16_device", _i, [_vp, _vp, _sz, _vp, _vp]),
+    ("fdsp_iir_bank_create", _i, [_vp, _u32, _vp, _u32, _u32, _P(_vp)]),
+    ("fdsp_iir_bank_destroy", _i, [_vp]),
+    ("fdsp_iir_bank_reset", _i, [_vp]),
+    ("fdsp_iir_bank_apply", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir_bank_apply_ci16", _i, [_vp, _vp, _sz, _vp]),
+    ("fdsp_iir_bank_apply_device", _i, [_vp, _vp, _sz, _sz, _vp, _sz, _vp]),
+    ("fdsp_iir_bank_apply_ci16_device", _i, [_vp, _vp, _sz, _sz, _vp, _sz, _vp]),
     ("fdsp_resample_fir_params", _i, [_u32, _u32, C.c_int32, _u32, _P(_u32), _P(_u32), _vp, _sz, _P(_sz)]),
     ("fdsp_iir1_create", _i, [C.c_float, C.c_float, _P(_vp)]),
     ("fdsp_iir1_destroy", _i, [_vp]),

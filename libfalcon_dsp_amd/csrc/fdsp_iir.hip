@@ -1164,3 +1164,366 @@ extern "C" int fdsp_iir1_apply(fdsp_iir1_t h, const float *in, size_t n, float *
         return set_err(FDSP_ENOTINIT, "null handle");
     return fdsp_iir_apply_f32(h->iir, in, n, out);
 }
+
+// ===========================================================================
+// IIR bank: many independent streams through one filter. One stream's exact
+// recurrence is sequential, so each stream runs on one lane. A block of four
+// waves owns 64 streams and walks their samples in tiles of kBankT as a
+// pipeline: in phase q, wave 0 runs the recurrence of tile q-2 (lane =
+// stream, in place over the tile's b-parts), while waves 1-3 write tile q's
+// inputs (loaded in phase q-1) and its carried inputs into LDS, issue the
+// loads of tile q+1, form the b-parts of tile q-1 and store the outputs of
+// tile q-3, all with accesses coalesced along each stream's row. A barrier
+// ends each phase. The arithmetic is that of the single-stream kernels (mac,
+// fb_step): outputs equal the reference's loop (falcon_dsp_iir_filter.cc:
+// 235-396) run on each stream, up to the sign of zero.
+// ===========================================================================
+namespace {
+
+constexpr uint32_t kBankT = 16;          // samples per tile
+constexpr uint32_t kBankMaxB = 17;       // b taps: up to 16 carried inputs (LDS below 64 KiB)
+constexpr uint32_t kBankH = kBankMaxB - 1;  // carried-input slots per stream in the state array
+constexpr uint32_t kBankVRow = kBankT + 1;  // LDS row of b-parts / outputs (odd)
+constexpr uint32_t kBankProd = 192;      // producer threads (waves 1-3)
+constexpr uint32_t kBankPer = (64 * kBankT + kBankProd - 1) / kBankProd;  // tile entries per producer
+
+// LDS row of inputs: h carried inputs, then the tile (odd length)
+__host__ __device__ constexpr uint32_t bank_xrow(uint32_t h) { return (h + kBankT) | 1u; }
+__host__ __device__ constexpr size_t bank_lds_bytes(uint32_t h)
+{
+    return (size_t)(2 * 64 * bank_xrow(h) + 3 * 64 * kBankVRow) * sizeof(float2);
+}
+
+struct BankLaunch {
+    const void *in;
+    void *out;
+    size_t in_stride, out_stride;  // samples from one stream's row to the next
+    uint32_t n, n_streams, nb;
+    const float2 *b;  // nb taps
+    float2 *xh;       // per stream kBankH carried inputs, oldest first (the last nb - 1 used)
+    float2 *yh;       // per stream kMaxFb carried outputs, newest first
+};
+
+template <uint32_t T> __device__ __forceinline__ float2 bank_load(const void *p, size_t i)
+{
+    if constexpr (T == kSampleCf32) {
+        return static_cast<const float2 *>(p)[i];
+    } else {
+        const short2 s = static_cast<const short2 *>(p)[i];
+        return make_float2((float)s.x, (float)s.y);
+    }
+}
+template <uint32_t T> __device__ __forceinline__ void bank_store(void *p, size_t i, float2 v)
+{
+    if constexpr (T == kSampleCf32)
+        static_cast<float2 *>(p)[i] = v;
+    else
+        static_cast<short2 *>(p)[i] = make_short2(i16_trunc(v.x), i16_trunc(v.y));
+}
+
+template <int P, bool RA, uint32_t TS>
+__global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
+{
+    extern __shared__ float2 bank_lds[];
+    const uint32_t h = g.nb - 1, xrow = bank_xrow(h);
+    float2 *xt = bank_lds;                     // [2][64][xrow]: tile q in xt + (q % 2) * 64 * xrow
+    float2 *vt = bank_lds + 2 * 64 * xrow;     // [3][64][kBankVRow]: tile q in vt + (q % 3) * 64 * kBankVRow
+    const uint32_t tid = threadIdx.x, s0 = blockIdx.x * 64;
+    const uint32_t rows = min(64u, g.n_streams - s0);
+    const uint32_t nt = (g.n + kBankT - 1) / kBankT;
+    const auto tile_cnt = [&](uint32_t q) { return min(kBankT, g.n - q * kBankT); };
+    float2 st[P];  // wave 0: the stream's carried outputs, newest first
+    float2 pf[kBankPer];  // waves 1-3: the next tile's inputs in flight
+    if (tid < 64) {
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            st[k] = tid < rows ? g.yh[(size_t)(s0 + tid) * kMaxFb + k] : make_float2(0.f, 0.f);
+    }
+    const uint32_t p = tid - 64;  // producer index (waves 1-3)
+    const auto load_tile = [&](uint32_t q) {
+        const uint32_t t0 = q * kBankT, cnt = tile_cnt(q);
+#pragma unroll
+        for (uint32_t i = 0; i < kBankPer; ++i) {
+            const uint32_t e = i * kBankProd + p, r = e / kBankT, c = e % kBankT;
+            pf[i] = r < rows && c < cnt ? bank_load<TS>(g.in, (size_t)(s0 + r) * g.in_stride + t0 + c)
+                                        : make_float2(0.f, 0.f);
+        }
+    };
+    if (tid >= 64 && nt > 0)
+        load_tile(0);
+    for (uint32_t q = 0; q < nt + 3; ++q) {
+        if (tid < 64) {
+            // the recurrence of tile q - 2 along the lane's stream
+            if (q >= 2 && q - 2 < nt && tid < rows) {
+                const uint32_t cnt = tile_cnt(q - 2);
+                float2 *vr = vt + ((q - 2) % 3) * 64 * kBankVRow + tid * kBankVRow;
+#pragma unroll 4
+                for (uint32_t t = 0; t < cnt; ++t)
+                    vr[t] = fb_step<P, true, RA>(vr[t], st, fb);
+            }
+        } else {
+            if (q < nt) {
+                // tile q's inputs after its carried inputs: the last h of
+                // (carried inputs, tile q - 1), or the state array for tile 0
+                float2 *xq = xt + (q % 2) * 64 * xrow;
+                const uint32_t cnt = tile_cnt(q);
+#pragma unroll
+                for (uint32_t i = 0; i < kBankPer; ++i) {
+                    const uint32_t e = i * kBankProd + p, r = e / kBankT, c = e % kBankT;
+                    if (r < 64 && c < cnt)
+                        xq[r * xrow + h + c] = pf[i];
+                }
+                for (uint32_t e = p; e < 64 * h; e += kBankProd) {
+                    const uint32_t r = e / h, k = e - r * h;
+                    if (r < rows)
+                        xq[r * xrow + k] = q == 0 ? g.xh[(size_t)(s0 + r) * kBankH + kBankH - h + k]
+                                                  : xt[((q - 1) % 2) * 64 * xrow + r * xrow + tile_cnt(q - 1) + k];
+                }
+                if (q + 1 < nt)
+                    load_tile(q + 1);
+            }
+            if (q >= 1 && q - 1 < nt) {
+                // b-parts of tile q - 1: ((0 + b0 x[n]) + b1 x[n-1]) + ...
+                const float2 *xq = xt + ((q - 1) % 2) * 64 * xrow;
+                float2 *vq = vt + ((q - 1) % 3) * 64 * kBankVRow;
+                const uint32_t cnt = tile_cnt(q - 1);
+#pragma unroll 2
+                for (uint32_t i = 0; i < kBankPer; ++i) {
+                    const uint32_t e = i * kBankProd + p, r = e / kBankT, c = e % kBankT;
+                    if (r < rows && c < cnt) {
+                        const float2 *xr = xq + r * xrow + h + c;
+                        float2 acc = make_float2(0.f, 0.f);
+                        for (uint32_t k = 0; k < g.nb; ++k)
+                            acc = mac<true, RA>(acc, xr[-(int32_t)k], g.b[k]);
+                        vq[r * kBankVRow + c] = acc;
+                    }
+                }
+            }
+            if (q >= 3 && q - 3 < nt) {
+                // outputs of tile q - 3
+                const float2 *vq = vt + ((q - 3) % 3) * 64 * kBankVRow;
+                const uint32_t t0 = (q - 3) * kBankT, cnt = tile_cnt(q - 3);
+#pragma unroll
+                for (uint32_t i = 0; i < kBankPer; ++i) {
+                    const uint32_t e = i * kBankProd + p, r = e / kBankT, c = e % kBankT;
+                    if (r < rows && c < cnt)
+                        bank_store<TS>(g.out, (size_t)(s0 + r) * g.out_stride + t0 + c, vq[r * kBankVRow + c]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // carried state: the last h inputs of (carried inputs, last tile), and the outputs
+    if (tid < 64 && tid < rows) {
+        const float2 *xl = xt + ((nt - 1) % 2) * 64 * xrow + tid * xrow + tile_cnt(nt - 1);
+        for (uint32_t k = 0; k < h; ++k)
+            g.xh[(size_t)(s0 + tid) * kBankH + kBankH - h + k] = xl[k];
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            g.yh[(size_t)(s0 + tid) * kMaxFb + k] = st[k];
+    }
+}
+
+template <int P, uint32_t TS> hipError_t launch_bank_p(const BankLaunch &g, const float2 *a, bool ra, hipStream_t s)
+{
+    Fb<P> fb;
+    for (int k = 0; k < P; ++k)
+        fb.a[k] = a[k];
+    const dim3 grid((g.n_streams + 63) / 64);
+    const size_t lds = bank_lds_bytes(g.nb - 1);
+    if (ra)
+        hipLaunchKernelGGL((iir_bank_kernel<P, true, TS>), grid, dim3(256), lds, s, g, fb);
+    else
+        hipLaunchKernelGGL((iir_bank_kernel<P, false, TS>), grid, dim3(256), lds, s, g, fb);
+    return hipGetLastError();
+}
+
+template <uint32_t TS> hipError_t launch_bank(const BankLaunch &g, uint32_t p, const float2 *a, bool ra, hipStream_t s)
+{
+    switch (p) {
+    case 1:
+        return launch_bank_p<1, TS>(g, a, ra, s);
+    case 2:
+        return launch_bank_p<2, TS>(g, a, ra, s);
+    case 4:
+        return launch_bank_p<4, TS>(g, a, ra, s);
+    case 8:
+        return launch_bank_p<8, TS>(g, a, ra, s);
+    default:
+        return launch_bank_p<16, TS>(g, a, ra, s);
+    }
+}
+
+}  // namespace
+
+struct fdsp_iir_bank_s {
+    std::mutex mu;
+    uint32_t nb = 0, nfb = 0, p = 1, n_streams = 0;
+    bool real_coeffs = true;
+    std::vector<float2> b, a;  // normalised taps; a[1..p] (zero past nfb)
+    DevBuf d_b, d_xh, d_yh, d_in, d_out;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int fdsp_iir_bank_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 *a, uint32_t na,
+                                    uint32_t n_streams, fdsp_iir_bank_t *out)
+{
+    if (!out || !b || !a)
+        return set_err(FDSP_EINVAL, "iir_bank_create: null argument");
+    if (nb == 0 || na == 0 || n_streams == 0)
+        return set_err(FDSP_EINVAL, "iir_bank_create: need b and a coefficients and at least one stream");
+    if (nb > kBankMaxB || na - 1 > kMaxFb)
+        return set_err(FDSP_EINVAL, "iir_bank_create: at most %u b and %u + 1 a coefficients", kBankMaxB, kMaxFb);
+    if (a[0].re == 0.f && a[0].im == 0.f)
+        return set_err(FDSP_EINVAL, "iir_bank_create: a[0] must be nonzero");
+    auto *h = new (std::nothrow) fdsp_iir_bank_s;
+    if (!h)
+        return set_err(FDSP_ENOMEM, "host allocation failed");
+    h->nb = nb;
+    h->nfb = na - 1;
+    h->n_streams = n_streams;
+    while (h->p < h->nfb)
+        h->p *= 2;
+    // iir_filter.cc:137-150: divide through by a[0] unless its real part is 1
+    const std::complex<float> a0(a[0].re, a[0].im);
+    const bool scale = a0.real() != 1.0f;
+    for (uint32_t k = 0; k < nb; ++k) {
+        std::complex<float> v(b[k].re, b[k].im);
+        if (scale)
+            v = cdiv(v, a0);
+        h->b.push_back(make_float2(v.real(), v.imag()));
+    }
+    h->a.assign(h->p, make_float2(0.f, 0.f));
+    for (uint32_t k = 1; k < na; ++k) {
+        std::complex<float> v(a[k].re, a[k].im);
+        if (scale)
+            v = cdiv(v, a0);
+        h->a[k - 1] = make_float2(v.real(), v.imag());
+    }
+    for (const float2 &v : h->b)
+        h->real_coeffs &= v.y == 0.f;
+    for (const float2 &v : h->a)
+        h->real_coeffs &= v.y == 0.f;
+    int rc = h->d_b.ensure(nb * sizeof(float2));
+    if (!rc)
+        rc = h->d_xh.ensure((size_t)n_streams * kBankH * sizeof(float2));
+    if (!rc)
+        rc = h->d_yh.ensure((size_t)n_streams * kMaxFb * sizeof(float2));
+    if (!rc && (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipMemcpy(h->d_b.p, h->b.data(), nb * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(h->d_xh.p, 0, h->d_xh.bytes) != hipSuccess ||
+                hipMemset(h->d_yh.p, 0, h->d_yh.bytes) != hipSuccess))
+        rc = set_err(FDSP_EHIP, "iir_bank_create: HIP setup failed");
+    if (rc) {
+        fdsp_iir_bank_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir_bank_destroy(fdsp_iir_bank_t h)
+{
+    if (!h)
+        return FDSP_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    for (DevBuf *d : {&h->d_b, &h->d_xh, &h->d_yh, &h->d_in, &h->d_out})
+        d->release();
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FDSP_OK;
+}
+
+extern "C" int fdsp_iir_bank_reset(fdsp_iir_bank_t h)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    // launches on caller streams may still write the state being cleared
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemset(h->d_xh.p, 0, h->d_xh.bytes));
+    HIP_TRY(hipMemset(h->d_yh.p, 0, h->d_yh.bytes));
+    return FDSP_OK;
+}
+
+namespace {
+
+int bank_apply_device(fdsp_iir_bank_t h, uint32_t t, const void *d_in, size_t n, size_t in_stride, void *d_out,
+                      size_t out_stride, hipStream_t s)
+{
+    if (n == 0)
+        return FDSP_OK;  // nothing to filter; the state stays
+    if (!d_in || !d_out)
+        return set_err(FDSP_EINVAL, "iir_bank: null buffers");
+    if (in_stride < n || out_stride < n)
+        return set_err(FDSP_EINVAL, "iir_bank: row strides (%zu in, %zu out) below the %zu samples", in_stride,
+                       out_stride, n);
+    if (int rc = check_chunk(n, 1))
+        return rc;
+    BankLaunch g{};
+    g.in = d_in;
+    g.out = d_out;
+    g.in_stride = in_stride;
+    g.out_stride = out_stride;
+    g.n = (uint32_t)n;
+    g.n_streams = h->n_streams;
+    g.nb = h->nb;
+    g.b = h->d_b.as<float2>();
+    g.xh = h->d_xh.as<float2>();
+    g.yh = h->d_yh.as<float2>();
+    const hipError_t e = t == kSampleCf32 ? launch_bank<kSampleCf32>(g, h->p, h->a.data(), h->real_coeffs, s)
+                                          : launch_bank<kSampleCi16>(g, h->p, h->a.data(), h->real_coeffs, s);
+    HIP_TRY(e);
+    return FDSP_OK;
+}
+
+int bank_apply_host(fdsp_iir_bank_t h, uint32_t t, const void *in, size_t n, void *out)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    if (n == 0)
+        return FDSP_OK;
+    if (!in || !out)
+        return set_err(FDSP_EINVAL, "iir_bank: null buffers");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const size_t bytes = (size_t)h->n_streams * n * (t == kSampleCf32 ? 8 : 4);
+    if (int rc = h->d_in.ensure(bytes))
+        return rc;
+    if (int rc = h->d_out.ensure(bytes))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_in.p, in, bytes, hipMemcpyHostToDevice, h->stream));
+    if (int rc = bank_apply_device(h, t, h->d_in.p, n, n, h->d_out.p, n, h->stream))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, h->d_out.p, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FDSP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdsp_iir_bank_apply(fdsp_iir_bank_t h, const fdsp_cf32 *in, size_t n, fdsp_cf32 *out)
+{
+    return bank_apply_host(h, kSampleCf32, in, n, out);
+}
+extern "C" int fdsp_iir_bank_apply_ci16(fdsp_iir_bank_t h, const fdsp_ci16 *in, size_t n, fdsp_ci16 *out)
+{
+    return bank_apply_host(h, kSampleCi16, in, n, out);
+}
+extern "C" int fdsp_iir_bank_apply_device(fdsp_iir_bank_t h, const fdsp_cf32 *d_in, size_t n, size_t in_stride,
+                                          fdsp_cf32 *d_out, size_t out_stride, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    return bank_apply_device(h, kSampleCf32, d_in, n, in_stride, d_out, out_stride, pick(h->stream, stream));
+}
+extern "C" int fdsp_iir_bank_apply_ci16_device(fdsp_iir_bank_t h, const fdsp_ci16 *d_in, size_t n, size_t in_stride,
+                                               fdsp_ci16 *d_out, size_t out_stride, fdsp_stream_t stream)
+{
+    if (!h)
+        return set_err(FDSP_ENOTINIT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    return bank_apply_device(h, kSampleCi16, d_in, n, in_stride, d_out, out_stride, pick(h->stream, stream));
+}
