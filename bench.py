@@ -505,14 +505,26 @@ def bench_serving(fd):
         span = (max(r.done_ns for r in res) - min(r.submit_ns for r in res)) / 1e9
         n_in = sum(m for _, m in tickets)
         n_out = sum(r.n_out for r in res)
+        # where the time goes, per request (host steady clock): waiting for a
+        # batch, packing and issuing it, the GPU's copy-in + kernels + copy-out
+        # (until the worker sees it), delivery into the caller's buffer
+        stages = {"queue": [(r.pack_ns - r.submit_ns) / 1e6 for r in res],
+                  "pack_issue": [(r.issue_ns - r.pack_ns) / 1e6 for r in res],
+                  "gpu": [(r.gpu_ns - r.issue_ns) / 1e6 for r in res],
+                  "deliver": [(r.done_ns - r.gpu_ns) / 1e6 for r in res]}
+        worst = sorted(range(len(lat)), key=lambda i: lat[i])[-max(1, len(lat) // 100):]
         return {"p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
                 "in_msps": round(n_in / span / 1e6, 1), "out_msps": round(n_out / span / 1e6, 2),
                 "seconds": round(span, 3), "batches": info["batches"],
                 "chunks_per_batch": round(info["chunks"] / max(1, info["batches"]), 2),
-                "streams": len(sids)}
+                "streams": len(sids),
+                "stages_ms": {k: {"p50": round(pct(v, 0.5), 3), "p99": round(pct(v, 0.99), 3),
+                                  "slowest_1pct_mean": round(sum(v[i] for i in worst) / len(worst), 3)}
+                              for k, v in stages.items()}}
 
     paced = run(True)
     log(f"serving (paced): p50 {paced['p50_ms']} ms, p99 {paced['p99_ms']} ms, {paced['in_msps']} Msps in")
+    log(f"serving (paced) stages: {paced['stages_ms']}")
     sat = run(False)
     log(f"serving (all at once): p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms, {sat['in_msps']} Msps in")
     return {"workload": f"{len(reqs)} requests, Poisson {w.rate_hz:g}/s over {w.max_streams} streams at "

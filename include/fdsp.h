@@ -537,7 +537,9 @@ typedef struct fdsp_serve_config {
     uint32_t max_streams; /* live streams at most */
     uint32_t max_chunk;   /* samples per request at most */
     uint32_t max_batch;   /* input (and output) samples per launch at most; >= max_chunk */
-    uint64_t hbm_budget;  /* device bytes for batch buffers plus stream state (0: no limit) */
+    uint64_t hbm_budget;  /* device bytes for batch buffers plus the stream state pool (0: no limit) */
+    uint64_t state_bytes; /* the stream state pool (0: what hbm_budget leaves after the batch buffers, at
+                             most 1 GiB; 64 KiB per stream when hbm_budget is 0) */
 } fdsp_serve_config;
 typedef struct fdsp_serve_result {
     size_t n_out;       /* samples written: fdsp_cf32, or float angles for a discriminator tail */
@@ -545,21 +547,27 @@ typedef struct fdsp_serve_result {
     uint64_t submit_ns; /* steady-clock time of submission */
     uint64_t done_ns;   /* steady-clock time the result was in `out` */
     uint64_t batch;     /* launch sequence number */
+    uint64_t pack_ns;   /* steady-clock time the worker took it into a batch */
+    uint64_t issue_ns;  /* ... the batch's copy-in was issued */
+    uint64_t gpu_ns;    /* ... the worker saw the batch's kernels and copy-out done */
 } fdsp_serve_result;
 typedef struct fdsp_serve_stats {
-    uint64_t hbm_used, hbm_budget; /* device bytes held: batch buffers + live streams */
+    uint64_t hbm_used, hbm_budget; /* device bytes in use: batch buffers + live streams' state */
     uint32_t live_streams;
     size_t queued;                 /* requests waiting or running */
     uint64_t batches, chunks, in_samples, out_samples;
+    uint64_t state_pool;           /* bytes of the stream state pool */
 } fdsp_serve_stats;
 
-/* Allocates the batch buffers (two staging slots) up front; FDSP_ENOMEM when
- * they alone exceed the budget. */
+/* Allocates the batch buffers (two staging slots) and the stream state pool
+ * up front; FDSP_ENOMEM when they exceed the budget. Copies in run on one
+ * HIP stream and kernels plus copies out on another, ordered by events. */
 int fdsp_serve_create(const fdsp_serve_config *cfg, fdsp_serve_t *out);
 /* Finishes every queued request, then frees everything. */
 int fdsp_serve_destroy(fdsp_serve_t s);
-/* Admit a stream: FDSP_EAGAIN when max_streams are live or its state would
- * exceed the budget (retry after a release). */
+/* Admit a stream: its state takes a slice of the pool (no device
+ * allocation); FDSP_EAGAIN when max_streams are live or no slice of its size
+ * is free (retry after a release). */
 int fdsp_serve_admit(fdsp_serve_t s, const fdsp_channel_desc *chan, uint32_t *stream);
 /* Output capacity a chunk of n samples may need (samples). */
 int fdsp_serve_max_out(fdsp_serve_t s, uint32_t stream, size_t n, size_t *count);

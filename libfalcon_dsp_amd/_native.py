@@ -66,6 +66,7 @@ class ServeConfig(C.Structure):
         ("max_chunk", C.c_uint32),
         ("max_batch", C.c_uint32),
         ("hbm_budget", C.c_uint64),
+        ("state_bytes", C.c_uint64),
     ]
 
 
@@ -77,6 +78,9 @@ class ServeResult(C.Structure):
         ("submit_ns", C.c_uint64),
         ("done_ns", C.c_uint64),
         ("batch", C.c_uint64),
+        ("pack_ns", C.c_uint64),
+        ("issue_ns", C.c_uint64),
+        ("gpu_ns", C.c_uint64),
     ]
 
 
@@ -91,6 +95,7 @@ class ServeStats(C.Structure):
         ("chunks", C.c_uint64),
         ("in_samples", C.c_uint64),
         ("out_samples", C.c_uint64),
+        ("state_pool", C.c_uint64),
     ]
 
 

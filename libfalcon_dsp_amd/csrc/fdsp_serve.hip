@@ -10,13 +10,15 @@
 // closed form: inputs consumed p plus the last hist_len inputs in device
 // memory, so a chunk's work depends only on (p, history, chunk).
 //
-// Runtime: clients submit chunks from any thread; one worker thread forms
-// batches (at most one chunk per stream per batch, streams taken round-robin,
-// bounded by max_batch input and output samples), packs them into pinned
-// staging, and issues copy-in, kernels and copy-out on the server's HIP
-// stream. Two staging slots alternate: while batch k runs on the GPU, batch
-// k+1 is packed and queued behind it, then batch k's results are delivered.
-// Requests that arrive while a batch runs join the next one.
+// Runtime: clients submit chunks from any thread; the scheduler
+// (fdsp_serve_sched.h, free of HIP) queues them and one worker thread forms
+// batches, packs each into pinned staging and issues it: the copy-in on one
+// HIP stream, then (ordered by an event) the kernels and the copy-out on a
+// second, so batch k+1's copy-in overlaps batch k's kernels. Two staging
+// slots alternate: while batch k runs, batch k+1 is packed and issued, then
+// batch k's results are delivered. Stream state lives in slices of one
+// device pool allocated at create, so admission allocates nothing on the
+// device.
 //
 // Kernels: one block per (chunk, 256-output tile); thread j of the tile
 // computes output j as y = e^{j2pi nu k(A)} sum_t g_t x[A - cpp + 1 + t]
@@ -31,15 +33,16 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
-#include <deque>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "fdsp_host.h"
 #include "fdsp_internal.h"
+#include "fdsp_serve_sched.h"
 
 using namespace fdsp;
 
@@ -47,7 +50,7 @@ namespace {
 
 constexpr uint32_t kServeBlock = 256;  // outputs per tile
 constexpr size_t kAlign = 256;
-constexpr uint32_t kMaxQueued = 1u << 16;  // requests queued or running at once
+constexpr size_t kMaxQueued = 1u << 16;  // requests queued or running at once
 
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -151,61 +154,26 @@ __global__ __launch_bounds__(kServeBlock) void serve_discriminator_kernel(const 
     a.angles[a.has_prev ? i : i - 1] = atan2f(z.y, z.x);
 }
 
-using Clock = std::chrono::steady_clock;
-uint64_t now_ns()
-{
-    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
-}
-
 struct ServeStream {
     bool live = false;
     uint32_t up = 1, down = 1, cpp = 1, hist_len = 1, tail = 0;
     double nu = 0, inv_r = 1;
     uint32_t r = 1;
     float2 wrap{1.f, 0.f};
-    void *dev = nullptr;  // taps | hist[2] | prev[2]
-    size_t dev_bytes = 0;
+    char *dev = nullptr;  // the stream's pool slice: taps | hist[2] | prev[2]
+    size_t dev_off = 0, dev_bytes = 0;
     size_t taps_off = 0, hist_off[2] = {0, 0}, prev_off[2] = {0, 0};
-    // host view of the stream position (advanced when a chunk joins a batch)
-    uint64_t p = 0;
-    uint32_t parity = 0, prev_parity = 0;
-    bool has_prev = false;
-    std::deque<uint64_t> queue;  // tickets waiting, oldest first
-    bool busy = false;           // a chunk of this stream is in a launched batch
-    template <class T> T *at(size_t off) const { return reinterpret_cast<T *>(static_cast<char *>(dev) + off); }
-};
-
-struct ServeReq {
-    uint32_t stream;
-    const fdsp_cf32 *in;
-    size_t n;
-    void *out;
-    size_t cap;
-    uint64_t submit_ns = 0, done_ns = 0, batch = 0;
-    size_t n_out = 0;
-    int status = FDSP_OK;
-    bool done = false;
-};
-
-// what the worker needs of one chunk of a packed batch
-struct BatchEntry {
-    uint64_t ticket;
-    uint32_t stream;
-    const fdsp_cf32 *in;
-    size_t n;
-    void *out;
-    uint32_t count;    // complex outputs computed
-    size_t emit;       // samples delivered (count, or count - 1 on a tail's first chunk)
-    size_t elem;       // bytes per delivered sample
-    size_t out_off;    // bytes into the output region
+    template <class T> T *at(size_t off) const { return reinterpret_cast<T *>(dev + off); }
 };
 
 struct Slot {
     char *h_buf = nullptr;  // pinned: items | table | tail table | inputs
     char *h_out = nullptr;  // pinned: packed outputs
     DevBuf d_buf, d_out, d_scratch;
-    hipEvent_t ev = nullptr;
-    std::vector<BatchEntry> entries;
+    hipEvent_t ev_in = nullptr;    // the copy-in has landed
+    hipEvent_t ev_done = nullptr;  // kernels and copy-out are done
+    std::vector<SchedEntry> entries;
+    std::vector<size_t> out_off;  // bytes into the output region, per entry
     uint64_t batch = 0;
 };
 
@@ -214,25 +182,22 @@ struct Slot {
 struct fdsp_serve_s {
     fdsp_serve_config cfg{};
     int device = 0;
-    hipStream_t stream = nullptr;
-    size_t table_cap = 0;   // tiles per batch at most
+    hipStream_t s_in = nullptr;     // copies in
+    hipStream_t s_run = nullptr;    // kernels, then copies out
+    hipStream_t s_admin = nullptr;  // stream state uploads at admission
+    size_t table_cap = 0;           // tiles per batch at most
     size_t buf_bytes = 0, out_bytes = 0;
-    uint64_t fixed_bytes = 0, used_bytes = 0;
+    uint64_t fixed_bytes = 0;  // batch buffers
     Slot slot[2];
-
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::vector<ServeStream> streams;
+    std::unique_ptr<ServeScheduler> sched;
+    // stream state pool: one allocation, slices handed out first-fit
+    std::mutex pool_mu;
+    DevBuf d_pool;
+    std::map<size_t, size_t> free_blocks;  // offset -> bytes, coalesced
+    uint64_t state_used = 0;
+    std::vector<ServeStream> streams;  // max_streams entries, never resized
     std::vector<uint32_t> free_ids;
     uint32_t live = 0;
-    std::unordered_map<uint64_t, ServeReq> reqs;
-    uint64_t next_ticket = 1;
-    size_t queued = 0;
-    uint32_t rr = 0;
-    uint64_t batches = 0, chunks = 0, in_samples = 0, out_samples = 0;
-    int fatal = FDSP_OK;
-    std::string fatal_msg;
-    bool stop = false;
     std::thread worker;
 };
 
@@ -255,129 +220,98 @@ void stream_layout(ServeStream &st)
     st.dev_bytes = off;
 }
 
-bool eligible(const fdsp_serve_s *s)
+// first fit in the pool; false when no free block holds `bytes`
+bool pool_take(fdsp_serve_s *s, size_t bytes, size_t *off)
 {
-    for (const ServeStream &st : s->streams)
-        if (st.live && !st.busy && !st.queue.empty())
-            return true;
+    for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
+        if (it->second < bytes)
+            continue;
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t at = it->first + bytes;
+        s->free_blocks.erase(it);
+        if (rest)
+            s->free_blocks.emplace(at, rest);
+        return true;
+    }
     return false;
 }
 
-void fail_all(fdsp_serve_s *s, int code, const std::string &msg)
+void pool_give(fdsp_serve_s *s, size_t off, size_t bytes)
 {
-    // a HIP failure: every waiting and running request ends with it
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->fatal = code;
-    s->fatal_msg = msg;
-    for (auto &kv : s->reqs)
-        if (!kv.second.done) {
-            kv.second.done = true;
-            kv.second.status = code;
-            kv.second.done_ns = now_ns();
-        }
-    for (ServeStream &st : s->streams) {
-        st.queue.clear();
-        st.busy = false;
+    auto it = s->free_blocks.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != s->free_blocks.end() && it->first + it->second == next->first) {
+        it->second += next->second;
+        s->free_blocks.erase(next);
     }
-    s->queued = 0;
-    s->cv_done.notify_all();
+    if (it != s->free_blocks.begin()) {
+        auto prev = std::prev(it);
+        if (prev->first + prev->second == it->first) {
+            prev->second += it->second;
+            s->free_blocks.erase(it);
+        }
+    }
 }
 
-// Take the next batch into slot `si` and queue its work on the GPU. Returns
-// false when no stream has a chunk ready.
+// Take the next batch into slot `si` and issue its work. *launched is false
+// when no stream has a chunk ready.
 int form_and_launch(fdsp_serve_s *s, int si, bool *launched)
 {
     Slot &sl = s->slot[si];
     *launched = false;
-    sl.entries.clear();
-    std::vector<ServeItem> items;
-    std::vector<uint32_t> table, dtable;
-    size_t total_in = 0, total_out = 0, total_scratch = 0, total_bytes_out = 0;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->fatal)
-            return FDSP_OK;
-        const uint32_t S = (uint32_t)s->streams.size();
-        for (uint32_t k = 0; k < S; ++k) {
-            const uint32_t id = (s->rr + k) % S;
-            ServeStream &st = s->streams[id];
-            if (!st.live || st.busy || st.queue.empty())
-                continue;
-            const uint64_t ticket = st.queue.front();
-            ServeReq &rq = s->reqs.at(ticket);
-            const uint32_t count = chunk_count(st.p, rq.n, st.up, st.down);
-            if (total_in + rq.n > s->cfg.max_batch || total_out + count > s->cfg.max_batch)
-                continue;  // does not fit this batch; it leads the next one (admission keeps one chunk within)
-            st.queue.pop_front();
-            st.busy = true;
-            BatchEntry e{};
-            e.ticket = ticket;
-            e.stream = id;
-            e.in = rq.in;
-            e.n = rq.n;
-            e.out = rq.out;
-            e.count = count;
-            e.elem = st.tail ? sizeof(float) : sizeof(float2);
-            e.emit = (st.tail && count && !st.has_prev) ? count - 1 : count;
-            e.out_off = total_bytes_out;
-            total_bytes_out += (e.emit * e.elem + 7) / 8 * 8;
-            ServeItem it{};
-            it.in = nullptr;  // set after packing (device address of the slot)
-            it.hist = st.at<float2>(st.hist_off[st.parity]);
-            it.hist_next = st.at<float2>(st.hist_off[st.parity ^ 1]);
-            it.taps = st.at<float2>(st.taps_off);
-            it.nu = st.nu;
-            it.inv_r = st.inv_r;
-            it.inv_up = 1.0 / (double)st.up;
-            it.p = st.p;
-            it.n0 = ceil_div(st.p * st.up, st.down);
-            it.wrap = st.wrap;
-            it.n = (uint32_t)rq.n;
-            it.count = count;
-            it.r = st.r;
-            it.up = st.up;
-            it.down = st.down;
-            it.cpp = st.cpp;
-            it.hist_len = st.hist_len;
-            it.has_prev = st.has_prev ? 1u : 0u;
-            if (st.tail) {
-                it.prev = st.at<float2>(st.prev_off[st.prev_parity]);
-                it.prev_next = st.at<float2>(st.prev_off[st.prev_parity ^ 1]);
-                // complex outputs go to scratch; `out` and `angles` are
-                // patched with device addresses below
-                it.out = reinterpret_cast<float2 *>(total_scratch);
-                total_scratch += count;
-            }
-            const uint32_t idx = (uint32_t)items.size();
-            items.push_back(it);
-            const uint32_t tiles = std::max<uint32_t>(1, (count + kServeBlock - 1) / kServeBlock);
-            for (uint32_t t = 0; t < tiles; ++t) {
-                table.push_back(idx);
-                table.push_back(t);
-            }
-            if (st.tail)
-                for (uint32_t t = 0; t * kServeBlock < count; ++t) {
-                    dtable.push_back(idx);
-                    dtable.push_back(t);
-                }
-            // advance the host view of the stream
-            st.p += rq.n;
-            st.parity ^= 1;
-            if (st.tail && count) {
-                st.has_prev = true;
-                st.prev_parity ^= 1;
-            }
-            total_in += rq.n;
-            total_out += count;
-            sl.entries.push_back(e);
-        }
-        if (!sl.entries.empty()) {
-            s->rr = (s->rr + 1) % std::max<uint32_t>(1, S);
-            sl.batch = ++s->batches;
-        }
-    }
+    s->sched->form_batch(&sl.entries, &sl.batch);
     if (sl.entries.empty())
         return FDSP_OK;
+    std::vector<ServeItem> items;
+    std::vector<uint32_t> table, dtable;
+    size_t total_in = 0, total_scratch = 0, total_bytes_out = 0;
+    sl.out_off.clear();
+    for (const SchedEntry &e : sl.entries) {
+        // the stream's slice stays put while it has a chunk in flight
+        // (release refuses busy streams)
+        const ServeStream &st = s->streams[e.stream];
+        sl.out_off.push_back(total_bytes_out);
+        total_bytes_out += (e.emit * e.elem + 7) / 8 * 8;
+        ServeItem it{};
+        it.hist = st.at<float2>(st.hist_off[e.parity]);
+        it.hist_next = st.at<float2>(st.hist_off[e.parity ^ 1]);
+        it.taps = st.at<float2>(st.taps_off);
+        it.nu = st.nu;
+        it.inv_r = st.inv_r;
+        it.inv_up = 1.0 / (double)st.up;
+        it.p = e.p;
+        it.n0 = ceil_div(e.p * st.up, st.down);
+        it.wrap = st.wrap;
+        it.n = (uint32_t)e.n;
+        it.count = e.count;
+        it.r = st.r;
+        it.up = st.up;
+        it.down = st.down;
+        it.cpp = st.cpp;
+        it.hist_len = st.hist_len;
+        it.has_prev = e.has_prev ? 1u : 0u;
+        if (e.tail) {
+            it.prev = st.at<float2>(st.prev_off[e.prev_parity]);
+            it.prev_next = st.at<float2>(st.prev_off[e.prev_parity ^ 1]);
+            // complex outputs go to scratch at this offset (patched below)
+            it.out = reinterpret_cast<float2 *>(total_scratch);
+            total_scratch += e.count;
+        }
+        const uint32_t idx = (uint32_t)items.size();
+        items.push_back(it);
+        const uint32_t tiles = std::max<uint32_t>(1, (e.count + kServeBlock - 1) / kServeBlock);
+        for (uint32_t t = 0; t < tiles; ++t) {
+            table.push_back(idx);
+            table.push_back(t);
+        }
+        if (e.tail)
+            for (uint32_t t = 0; t * kServeBlock < e.count; ++t) {
+                dtable.push_back(idx);
+                dtable.push_back(t);
+            }
+        total_in += e.n;
+    }
     // pack: items | table | tail table | inputs, each 256-byte aligned
     const size_t items_off = 0;
     const size_t table_off = align_up(items_off + items.size() * sizeof(ServeItem));
@@ -389,36 +323,39 @@ int form_and_launch(fdsp_serve_s *s, int si, bool *launched)
     float2 *scratch = sl.d_scratch.as<float2>();
     size_t in_pos = 0;
     for (size_t i = 0; i < items.size(); ++i) {
-        const BatchEntry &e = sl.entries[i];
+        const SchedEntry &e = sl.entries[i];
         ServeItem &it = items[i];
         std::memcpy(sl.h_buf + in_off + in_pos * sizeof(float2), e.in, e.n * sizeof(float2));
         it.in = reinterpret_cast<const float2 *>(dbase + in_off) + in_pos;
         in_pos += e.n;
         if (it.prev) {
             it.out = scratch + reinterpret_cast<size_t>(it.out);
-            it.angles = reinterpret_cast<float *>(obase + e.out_off);
+            it.angles = reinterpret_cast<float *>(obase + sl.out_off[i]);
         } else {
-            it.out = reinterpret_cast<float2 *>(obase + e.out_off);
+            it.out = reinterpret_cast<float2 *>(obase + sl.out_off[i]);
         }
     }
     std::memcpy(sl.h_buf + items_off, items.data(), items.size() * sizeof(ServeItem));
     std::memcpy(sl.h_buf + table_off, table.data(), table.size() * sizeof(uint32_t));
     if (!dtable.empty())
         std::memcpy(sl.h_buf + dtable_off, dtable.data(), dtable.size() * sizeof(uint32_t));
-    HIP_TRY(hipMemcpyAsync(dbase, sl.h_buf, used, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(serve_chan_kernel, dim3((uint32_t)(table.size() / 2)), dim3(kServeBlock), 0, s->stream,
+    s->sched->mark_issued(sl.entries);
+    HIP_TRY(hipMemcpyAsync(dbase, sl.h_buf, used, hipMemcpyHostToDevice, s->s_in));
+    HIP_TRY(hipEventRecord(sl.ev_in, s->s_in));
+    HIP_TRY(hipStreamWaitEvent(s->s_run, sl.ev_in, 0));
+    hipLaunchKernelGGL(serve_chan_kernel, dim3((uint32_t)(table.size() / 2)), dim3(kServeBlock), 0, s->s_run,
                        reinterpret_cast<const ServeItem *>(dbase + items_off),
                        reinterpret_cast<const uint32_t *>(dbase + table_off));
     HIP_TRY(hipGetLastError());
     if (!dtable.empty()) {
         hipLaunchKernelGGL(serve_discriminator_kernel, dim3((uint32_t)(dtable.size() / 2)), dim3(kServeBlock), 0,
-                           s->stream, reinterpret_cast<const ServeItem *>(dbase + items_off),
+                           s->s_run, reinterpret_cast<const ServeItem *>(dbase + items_off),
                            reinterpret_cast<const uint32_t *>(dbase + dtable_off));
         HIP_TRY(hipGetLastError());
     }
     if (total_bytes_out)
-        HIP_TRY(hipMemcpyAsync(sl.h_out, obase, total_bytes_out, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipEventRecord(sl.ev, s->stream));
+        HIP_TRY(hipMemcpyAsync(sl.h_out, obase, total_bytes_out, hipMemcpyDeviceToHost, s->s_run));
+    HIP_TRY(hipEventRecord(sl.ev_done, s->s_run));
     *launched = true;
     return FDSP_OK;
 }
@@ -427,34 +364,22 @@ int form_and_launch(fdsp_serve_s *s, int si, bool *launched)
 int complete(fdsp_serve_s *s, int si)
 {
     Slot &sl = s->slot[si];
-    HIP_TRY(hipEventSynchronize(sl.ev));
-    for (const BatchEntry &e : sl.entries)
+    HIP_TRY(hipEventSynchronize(sl.ev_done));
+    const uint64_t gpu_ns = sched_now_ns();
+    for (size_t i = 0; i < sl.entries.size(); ++i) {
+        const SchedEntry &e = sl.entries[i];
         if (e.emit)
-            std::memcpy(e.out, sl.h_out + e.out_off, e.emit * e.elem);
-    const uint64_t t = now_ns();
-    std::lock_guard<std::mutex> lk(s->mu);
-    for (const BatchEntry &e : sl.entries) {
-        ServeReq &rq = s->reqs.at(e.ticket);
-        rq.done = true;
-        rq.status = FDSP_OK;
-        rq.n_out = e.emit;
-        rq.done_ns = t;
-        rq.batch = sl.batch;
-        s->streams[e.stream].busy = false;
-        --s->queued;
-        ++s->chunks;
-        s->in_samples += e.n;
-        s->out_samples += e.emit;
+            std::memcpy(e.out, sl.h_out + sl.out_off[i], e.emit * e.elem);
     }
+    s->sched->complete(sl.entries, gpu_ns);
     sl.entries.clear();
-    s->cv_done.notify_all();
     return FDSP_OK;
 }
 
 void worker_main(fdsp_serve_s *s)
 {
     if (hipSetDevice(s->device) != hipSuccess) {
-        fail_all(s, FDSP_EHIP, "serve worker: hipSetDevice failed");
+        s->sched->fail_all(FDSP_EHIP, "serve worker: hipSetDevice failed");
         return;
     }
     int cur = 0;
@@ -470,12 +395,12 @@ void worker_main(fdsp_serve_s *s)
             running[other] = false;
         }
         if (rc != FDSP_OK) {
-            fail_all(s, rc, g_err);
-            // drain what the GPU still holds before going idle
-            (void)hipStreamSynchronize(s->stream);
-            running[0] = running[1] = false;
-            std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop; });
+            s->sched->fail_all(rc, g_err);
+            // drain what the GPU still holds, then wait for the end
+            (void)hipStreamSynchronize(s->s_in);
+            (void)hipStreamSynchronize(s->s_run);
+            while (s->sched->wait_work()) {
+            }
             return;
         }
         if (launched) {
@@ -485,15 +410,13 @@ void worker_main(fdsp_serve_s *s)
         if (running[cur]) {
             // nothing new could be formed: deliver the batch in flight
             if (complete(s, cur) != FDSP_OK) {
-                fail_all(s, FDSP_EHIP, g_err);
+                s->sched->fail_all(FDSP_EHIP, g_err);
                 return;
             }
             running[cur] = false;
             continue;
         }
-        std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_work.wait(lk, [&] { return s->stop || s->fatal || eligible(s); });
-        if (s->fatal || (s->stop && !eligible(s)))
+        if (!s->sched->wait_work())
             return;
     }
 }
@@ -508,9 +431,11 @@ void release_slot(Slot &sl)
     sl.d_buf.release();
     sl.d_out.release();
     sl.d_scratch.release();
-    if (sl.ev)
-        (void)hipEventDestroy(sl.ev);
-    sl.ev = nullptr;
+    for (hipEvent_t *e : {&sl.ev_in, &sl.ev_done})
+        if (*e) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
 }
 
 }  // namespace
@@ -532,7 +457,9 @@ extern "C" int fdsp_serve_create(const fdsp_serve_config *cfg, fdsp_serve_t *out
     s->cfg = *cfg;
     int rc = FDSP_OK;
     if (hipGetDevice(&s->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        hipStreamCreateWithFlags(&s->s_in, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->s_run, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->s_admin, hipStreamNonBlocking) != hipSuccess)
         rc = set_err(FDSP_EHIP, "serve_create: HIP stream setup failed");
     // per slot: items, tables and inputs; outputs (<= 8 B per output sample);
     // complex scratch of the discriminator tails
@@ -544,6 +471,22 @@ extern "C" int fdsp_serve_create(const fdsp_serve_config *cfg, fdsp_serve_t *out
     if (!rc && cfg->hbm_budget && s->fixed_bytes > cfg->hbm_budget)
         rc = set_err(FDSP_ENOMEM, "serve_create: batch buffers need %llu bytes, above the %llu-byte budget",
                      (unsigned long long)s->fixed_bytes, (unsigned long long)cfg->hbm_budget);
+    // the stream state pool: state_bytes, else what the budget leaves (at most 1 GiB), else 64 KiB a stream
+    uint64_t pool = cfg->state_bytes;
+    if (!pool)
+        pool = cfg->hbm_budget ? std::min<uint64_t>(cfg->hbm_budget - std::min(cfg->hbm_budget, s->fixed_bytes),
+                                                    (uint64_t)1 << 30)
+                               : (uint64_t)cfg->max_streams << 16;
+    pool = pool / kAlign * kAlign;
+    if (!rc && cfg->hbm_budget && s->fixed_bytes + pool > cfg->hbm_budget)
+        rc = set_err(FDSP_ENOMEM, "serve_create: batch buffers and a %llu-byte state pool exceed the %llu-byte budget",
+                     (unsigned long long)pool, (unsigned long long)cfg->hbm_budget);
+    if (!rc && pool == 0)
+        rc = set_err(FDSP_ENOMEM, "serve_create: no device memory left for stream state");
+    if (!rc)
+        rc = s->d_pool.ensure(pool);
+    if (!rc)
+        s->free_blocks.emplace(0, (size_t)pool);
     for (int i = 0; i < 2 && !rc; ++i) {
         Slot &sl = s->slot[i];
         rc = sl.d_buf.ensure(s->buf_bytes);
@@ -553,15 +496,43 @@ extern "C" int fdsp_serve_create(const fdsp_serve_config *cfg, fdsp_serve_t *out
             rc = sl.d_scratch.ensure((size_t)cfg->max_batch * sizeof(float2));
         if (!rc && (hipHostMalloc((void **)&sl.h_buf, s->buf_bytes, hipHostMallocDefault) != hipSuccess ||
                     hipHostMalloc((void **)&sl.h_out, s->out_bytes, hipHostMallocDefault) != hipSuccess ||
-                    hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess))
+                    hipEventCreateWithFlags(&sl.ev_in, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) != hipSuccess))
             rc = set_err(FDSP_ENOMEM, "serve_create: pinned staging allocation failed");
     }
-    s->used_bytes = s->fixed_bytes;
+    // Warm-up, so the first batches do not pay one-time costs (measured at
+    // several ms each): touch every page of the pinned staging (first-touch
+    // faults), copy it both ways once (the device's mappings of it), and
+    // launch both kernels once on an empty item (their code objects load).
+    for (int i = 0; i < 2 && !rc; ++i) {
+        Slot &sl = s->slot[i];
+        std::memset(sl.h_buf, 0, s->buf_bytes);
+        std::memset(sl.h_out, 0, s->out_bytes);
+        const uint32_t table[2] = {0, 0};
+        std::memcpy(sl.h_buf + align_up(sizeof(ServeItem)), table, sizeof table);
+        const ServeItem *items = sl.d_buf.as<ServeItem>();  // item 0: all zero, count 0
+        const uint32_t *dtab = reinterpret_cast<const uint32_t *>(sl.d_buf.as<char>() + align_up(sizeof(ServeItem)));
+        if (hipMemcpyAsync(sl.d_buf.p, sl.h_buf, s->buf_bytes, hipMemcpyHostToDevice, s->s_in) != hipSuccess ||
+            hipStreamSynchronize(s->s_in) != hipSuccess)
+            rc = set_err(FDSP_EHIP, "serve_create: staging warm-up failed");
+        if (!rc) {
+            hipLaunchKernelGGL(serve_chan_kernel, dim3(1), dim3(kServeBlock), 0, s->s_run, items, dtab);
+            hipLaunchKernelGGL(serve_discriminator_kernel, dim3(1), dim3(kServeBlock), 0, s->s_run, items, dtab);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(sl.h_out, sl.d_out.p, s->out_bytes, hipMemcpyDeviceToHost, s->s_run) != hipSuccess ||
+                hipStreamSynchronize(s->s_run) != hipSuccess)
+                rc = set_err(FDSP_EHIP, "serve_create: kernel warm-up failed");
+        }
+    }
     if (!rc) {
         try {
+            s->streams.resize(cfg->max_streams);
+            for (uint32_t i = cfg->max_streams; i > 0; --i)
+                s->free_ids.push_back(i - 1);
+            s->sched.reset(new ServeScheduler(cfg->max_batch, kMaxQueued));
             s->worker = std::thread(worker_main, s);
         } catch (...) {
-            rc = set_err(FDSP_ENOMEM, "serve_create: worker thread could not start");
+            rc = set_err(FDSP_ENOMEM, "serve_create: host setup failed");
         }
     }
     if (rc) {
@@ -577,22 +548,18 @@ extern "C" int fdsp_serve_destroy(fdsp_serve_t s)
     if (!s)
         return FDSP_OK;
     if (s->worker.joinable()) {
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->stop = true;
-        }
-        s->cv_work.notify_all();
+        s->sched->stop();
         s->worker.join();
     }
-    if (s->stream)
-        (void)hipStreamSynchronize(s->stream);
+    for (hipStream_t st : {s->s_in, s->s_run, s->s_admin})
+        if (st)
+            (void)hipStreamSynchronize(st);
     for (auto &sl : s->slot)
         release_slot(sl);
-    for (auto &st : s->streams)
-        if (st.dev)
-            (void)hipFree(st.dev);
-    if (s->stream)
-        (void)hipStreamDestroy(s->stream);
+    s->d_pool.release();
+    for (hipStream_t st : {s->s_in, s->s_run, s->s_admin})
+        if (st)
+            (void)hipStreamDestroy(st);
     delete s;
     return FDSP_OK;
 }
@@ -621,48 +588,36 @@ extern "C" int fdsp_serve_admit(fdsp_serve_t s, const fdsp_channel_desc *chan, u
     if (int rc = check_chunk(s->cfg.max_chunk, st.up))
         return rc;
     stream_layout(st);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->fatal)
-            return set_err(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
-        if (s->live >= s->cfg.max_streams)
-            return set_err(FDSP_EAGAIN, "serve_admit: %u live streams, the limit", s->live);
-        if (s->cfg.hbm_budget && s->used_bytes + st.dev_bytes > s->cfg.hbm_budget)
-            return set_err(FDSP_EAGAIN, "serve_admit: %zu more bytes exceed the budget (%llu of %llu in use)",
-                           st.dev_bytes, (unsigned long long)s->used_bytes,
-                           (unsigned long long)s->cfg.hbm_budget);
-        s->used_bytes += st.dev_bytes;
-        ++s->live;
-    }
+    std::string why;
+    if (int f = s->sched->fatal(&why))
+        return set_err(f, "server stopped: %s", why.c_str());
     std::vector<float2> taps;
     append_modulated_taps(bank, st.up, st.cpp, st.nu, &taps);
-    int rc = FDSP_OK;
-    if (hipMalloc(&st.dev, st.dev_bytes) != hipSuccess) {
-        st.dev = nullptr;
-        rc = set_err(FDSP_ENOMEM, "serve_admit: hipMalloc(%zu) failed", st.dev_bytes);
-    } else if (hipMemset(st.dev, 0, st.dev_bytes) != hipSuccess ||
-               hipMemcpy(st.at<float2>(st.taps_off), taps.data(), taps.size() * sizeof(float2),
-                         hipMemcpyHostToDevice) != hipSuccess) {
-        rc = set_err(FDSP_EHIP, "serve_admit: state upload failed");
+    // a slice of the pool, zeroed, with the modulated taps: the upload runs
+    // on the admission stream and finishes before the stream is live
+    std::lock_guard<std::mutex> lk(s->pool_mu);
+    if (s->live >= s->cfg.max_streams || s->free_ids.empty())
+        return set_err(FDSP_EAGAIN, "serve_admit: %u live streams, the limit", s->live);
+    size_t off = 0;
+    if (!pool_take(s, st.dev_bytes, &off))
+        return set_err(FDSP_EAGAIN, "serve_admit: no %zu-byte slice free in the %zu-byte state pool (%llu in use)",
+                       st.dev_bytes, s->d_pool.bytes, (unsigned long long)s->state_used);
+    st.dev = static_cast<char *>(s->d_pool.p) + off;
+    st.dev_off = off;
+    if (hipMemsetAsync(st.dev, 0, st.dev_bytes, s->s_admin) != hipSuccess ||
+        hipMemcpyAsync(st.at<float2>(st.taps_off), taps.data(), taps.size() * sizeof(float2),
+                       hipMemcpyHostToDevice, s->s_admin) != hipSuccess ||
+        hipStreamSynchronize(s->s_admin) != hipSuccess) {
+        pool_give(s, off, st.dev_bytes);
+        return set_err(FDSP_EHIP, "serve_admit: state upload failed");
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (rc) {
-        if (st.dev)
-            (void)hipFree(st.dev);
-        s->used_bytes -= st.dev_bytes;
-        --s->live;
-        return rc;
-    }
+    const uint32_t id = s->free_ids.back();
+    s->free_ids.pop_back();
     st.live = true;
-    uint32_t id;
-    if (!s->free_ids.empty()) {
-        id = s->free_ids.back();
-        s->free_ids.pop_back();
-        s->streams[id] = std::move(st);
-    } else {
-        id = (uint32_t)s->streams.size();
-        s->streams.push_back(std::move(st));
-    }
+    s->streams[id] = st;
+    s->state_used += st.dev_bytes;
+    ++s->live;
+    s->sched->add_stream(id, st.up, st.down, st.tail != 0);
     *stream = id;
     return FDSP_OK;
 }
@@ -673,7 +628,7 @@ extern "C" int fdsp_serve_max_out(fdsp_serve_t s, uint32_t stream, size_t n, siz
         return set_err(FDSP_ENOTINIT, "null server");
     if (!count)
         return set_err(FDSP_EINVAL, "null count");
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::lock_guard<std::mutex> lk(s->pool_mu);
     if (stream >= s->streams.size() || !s->streams[stream].live)
         return set_err(FDSP_EINVAL, "stream %u is not live", stream);
     const ServeStream &st = s->streams[stream];
@@ -690,31 +645,9 @@ extern "C" int fdsp_serve_submit(fdsp_serve_t s, uint32_t stream, const fdsp_cf3
         return set_err(FDSP_EINVAL, "serve_submit: need input, output and ticket pointers and n > 0");
     if (n > s->cfg.max_chunk)
         return set_err(FDSP_ERANGE, "serve_submit: chunk of %zu samples above max_chunk (%u)", n, s->cfg.max_chunk);
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->fatal)
-        return set_err(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
-    if (stream >= s->streams.size() || !s->streams[stream].live)
-        return set_err(FDSP_EINVAL, "stream %u is not live", stream);
-    ServeStream &st = s->streams[stream];
-    const size_t bound = (size_t)((uint64_t)n * st.up / st.down + 1);
-    if (cap < bound)
-        return set_err(FDSP_ERANGE, "serve_submit: output capacity %zu below the %zu samples a chunk may emit", cap,
-                       bound);
-    if (s->queued >= kMaxQueued)
-        return set_err(FDSP_EAGAIN, "serve_submit: %zu requests in flight, the limit", s->queued);
-    ServeReq rq;
-    rq.stream = stream;
-    rq.in = in;
-    rq.n = n;
-    rq.out = out;
-    rq.cap = cap;
-    rq.submit_ns = now_ns();
-    const uint64_t t = s->next_ticket++;
-    s->reqs.emplace(t, rq);
-    st.queue.push_back(t);
-    ++s->queued;
-    *ticket = t;
-    s->cv_work.notify_one();
+    std::string why;
+    if (int rc = s->sched->submit(stream, in, n, out, cap, ticket, &why))
+        return set_err(rc, "serve_submit: %s", why.c_str());
     return FDSP_OK;
 }
 
@@ -722,31 +655,25 @@ extern "C" int fdsp_serve_poll(fdsp_serve_t s, uint64_t ticket, int wait_ms, fds
 {
     if (!s)
         return set_err(FDSP_ENOTINIT, "null server");
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end())
-        return set_err(FDSP_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
-    if (!it->second.done && wait_ms != 0) {
-        auto ready = [&] { return s->reqs.at(ticket).done; };
-        if (wait_ms < 0)
-            s->cv_done.wait(lk, ready);
-        else
-            s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), ready);
-        it = s->reqs.find(ticket);
-    }
-    if (!it->second.done)
+    SchedReq rq;
+    std::string why;
+    const int rc = s->sched->poll(ticket, wait_ms, &rq, &why);
+    if (rc == FDSP_EAGAIN)
         return FDSP_EAGAIN;
-    const ServeReq rq = it->second;
-    s->reqs.erase(it);
+    if (rc == FDSP_EINVAL && why.rfind("unknown", 0) == 0)
+        return set_err(rc, "%s", why.c_str());
     if (res) {
         res->n_out = rq.n_out;
         res->status = rq.status;
         res->submit_ns = rq.submit_ns;
         res->done_ns = rq.done_ns;
         res->batch = rq.batch;
+        res->pack_ns = rq.pack_ns;
+        res->issue_ns = rq.issue_ns;
+        res->gpu_ns = rq.gpu_ns;
     }
-    if (rq.status != FDSP_OK)
-        return set_err(rq.status, "request failed: %s", s->fatal_msg.c_str());
+    if (rc != FDSP_OK)
+        return set_err(rc, "%s", why.c_str());
     return FDSP_OK;
 }
 
@@ -754,22 +681,17 @@ extern "C" int fdsp_serve_release(fdsp_serve_t s, uint32_t stream)
 {
     if (!s)
         return set_err(FDSP_ENOTINIT, "null server");
-    void *dev = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (stream >= s->streams.size() || !s->streams[stream].live)
-            return set_err(FDSP_EINVAL, "stream %u is not live", stream);
-        ServeStream &st = s->streams[stream];
-        if (st.busy || !st.queue.empty())
-            return set_err(FDSP_EBUSY, "stream %u has chunks queued or running", stream);
-        dev = st.dev;
-        s->used_bytes -= st.dev_bytes;
-        --s->live;
-        st = ServeStream{};
-        s->free_ids.push_back(stream);
-    }
-    if (dev)
-        HIP_TRY(hipFree(dev));
+    std::lock_guard<std::mutex> lk(s->pool_mu);
+    if (stream >= s->streams.size() || !s->streams[stream].live)
+        return set_err(FDSP_EINVAL, "stream %u is not live", stream);
+    if (int rc = s->sched->remove_stream(stream))
+        return set_err(rc, "stream %u has chunks queued or running", stream);
+    ServeStream &st = s->streams[stream];
+    pool_give(s, st.dev_off, st.dev_bytes);
+    s->state_used -= st.dev_bytes;
+    --s->live;
+    st = ServeStream{};
+    s->free_ids.push_back(stream);
     return FDSP_OK;
 }
 
@@ -779,14 +701,13 @@ extern "C" int fdsp_serve_info(fdsp_serve_t s, fdsp_serve_stats *st)
         return set_err(FDSP_ENOTINIT, "null server");
     if (!st)
         return set_err(FDSP_EINVAL, "null stats");
-    std::lock_guard<std::mutex> lk(s->mu);
-    st->hbm_used = s->used_bytes;
-    st->hbm_budget = s->cfg.hbm_budget;
-    st->live_streams = s->live;
-    st->queued = s->queued;
-    st->batches = s->batches;
-    st->chunks = s->chunks;
-    st->in_samples = s->in_samples;
-    st->out_samples = s->out_samples;
-    return s->fatal;
+    {
+        std::lock_guard<std::mutex> lk(s->pool_mu);
+        st->hbm_used = s->fixed_bytes + s->state_used;
+        st->hbm_budget = s->cfg.hbm_budget;
+        st->live_streams = s->live;
+        st->state_pool = s->d_pool.bytes;
+    }
+    s->sched->stats(&st->queued, &st->batches, &st->chunks, &st->in_samples, &st->out_samples);
+    return s->sched->fatal(nullptr);
 }

@@ -828,9 +828,10 @@ class Server(_Handle):
     _destroy = "fdsp_serve_destroy"
 
     def __init__(self, fs: int, max_streams: int = 200, max_chunk: int = 1 << 18, max_batch: int = 1 << 22,
-                 hbm_budget: int = 64 << 30):
+                 hbm_budget: int = 64 << 30, state_bytes: int = 0):
         super().__init__()
-        cfg = N.ServeConfig(int(fs), int(max_streams), int(max_chunk), int(max_batch), int(hbm_budget))
+        cfg = N.ServeConfig(int(fs), int(max_streams), int(max_chunk), int(max_batch), int(hbm_budget),
+                            int(state_bytes))
         check(lib().fdsp_serve_create(C.byref(cfg), C.byref(self.h)))
         self.fs = int(fs)
         self._tail = {}

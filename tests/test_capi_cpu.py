@@ -87,7 +87,7 @@ def test_serve_argument_checks_before_any_device_work():
     assert L.fdsp_serve_release(None, 0) == N.FDSP_ENOTINIT
     assert L.fdsp_serve_destroy(None) == N.FDSP_OK
     # the ctypes mirrors of the header's structs
-    assert C.sizeof(N.ServeConfig) == 24 and C.sizeof(N.ServeResult) == 40 and C.sizeof(N.ServeStats) == 64
+    assert C.sizeof(N.ServeConfig) == 32 and C.sizeof(N.ServeResult) == 64 and C.sizeof(N.ServeStats) == 72
 
 
 def test_python_rate_helpers():

@@ -228,3 +228,26 @@ def test_serving_workload_slice():
             checked += 1
     assert checked > 12
     srv.close()
+
+
+def test_request_timestamps_and_state_pool():
+    # every result carries its stages in order (submit, taken into a batch,
+    # copy-in issued, GPU work seen done, delivered); admission takes slices
+    # of the pool allocated at create, and a released slice is reused
+    ch = channel(FS, 1_000_000, 250_000)
+    srv = fd.Server(FS, max_streams=4, max_chunk=1 << 16, max_batch=1 << 17, state_bytes=1 << 20)
+    info = srv.info()
+    assert info["state_pool"] == 1 << 20
+    sids = [srv.admit(ch) for _ in range(3)]
+    used = srv.info()["hbm_used"]
+    x = cwgn(9, 50_000)
+    ts = [srv.submit(s, x) for s in sids for _ in range(3)]
+    for t in ts:
+        y, r = srv.result(t)
+        assert r.status == 0
+        assert r.submit_ns <= r.pack_ns <= r.issue_ns <= r.gpu_ns <= r.done_ns
+    srv.release(sids[0])
+    assert srv.info()["hbm_used"] < used
+    assert srv.admit(ch) is not None
+    assert srv.info()["hbm_used"] == used
+    srv.close()
