@@ -1232,6 +1232,9 @@ __global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
     const uint32_t rows = min(64u, g.n_streams - s0);
     const uint32_t nt = (g.n + kBankT - 1) / kBankT;
     const auto tile_cnt = [&](uint32_t q) { return min(kBankT, g.n - q * kBankT); };
+    __shared__ float2 bsh[kBankMaxB];  // the b taps, read by all lanes at once
+    if (tid < g.nb)
+        bsh[tid] = g.b[tid];
     float2 st[P];  // wave 0: the stream's carried outputs, newest first
     float2 pf[kBankPer];  // waves 1-3: the next tile's inputs in flight
     if (tid < 64) {
@@ -1283,20 +1286,30 @@ __global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
                     load_tile(q + 1);
             }
             if (q >= 1 && q - 1 < nt) {
-                // b-parts of tile q - 1: ((0 + b0 x[n]) + b1 x[n-1]) + ...
+                // b-parts of tile q - 1: ((0 + b0 x[n]) + b1 x[n-1]) + ...,
+                // the lane's entries side by side, one tap at a time
                 const float2 *xq = xt + ((q - 1) % 2) * 64 * xrow;
                 float2 *vq = vt + ((q - 1) % 3) * 64 * kBankVRow;
                 const uint32_t cnt = tile_cnt(q - 1);
-#pragma unroll 2
+                float2 acc[kBankPer];
+                uint32_t base[kBankPer];
+#pragma unroll
+                for (uint32_t i = 0; i < kBankPer; ++i) {
+                    const uint32_t e = i * kBankProd + p, r = min(e / kBankT, 63u), c = e % kBankT;
+                    acc[i] = make_float2(0.f, 0.f);
+                    base[i] = r * xrow + h + c;
+                }
+                for (uint32_t k = 0; k < g.nb; ++k) {
+                    const float2 bk = bsh[k];
+#pragma unroll
+                    for (uint32_t i = 0; i < kBankPer; ++i)
+                        acc[i] = mac<true, RA>(acc[i], xq[base[i] - k], bk);
+                }
+#pragma unroll
                 for (uint32_t i = 0; i < kBankPer; ++i) {
                     const uint32_t e = i * kBankProd + p, r = e / kBankT, c = e % kBankT;
-                    if (r < rows && c < cnt) {
-                        const float2 *xr = xq + r * xrow + h + c;
-                        float2 acc = make_float2(0.f, 0.f);
-                        for (uint32_t k = 0; k < g.nb; ++k)
-                            acc = mac<true, RA>(acc, xr[-(int32_t)k], g.b[k]);
-                        vq[r * kBankVRow + c] = acc;
-                    }
+                    if (r < rows && c < cnt)
+                        vq[r * kBankVRow + c] = acc[i];
                 }
             }
             if (q >= 3 && q - 3 < nt) {

@@ -273,9 +273,10 @@ hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t u
         tile = span_cap * up / down;
         if (tile > 4096)
             tile = 4096;
-        // short launches: smaller tiles, so the grid reaches about 512 blocks
-        // (a chunk of 52k outputs otherwise ran on 13 CUs)
-        tile = std::min<uint64_t>(tile, std::max<uint64_t>(kBlock, (count + 511) / 512));
+        // smaller tiles, so the grid reaches about 2048 blocks (8 per CU): a
+        // chunk of 52k outputs otherwise ran on 13 CUs, and 838k outputs in
+        // 512 blocks of 1639 took 13 us a launch, in 2048 blocks of 410 about 9
+        tile = std::min<uint64_t>(tile, std::max<uint64_t>(kBlock, (count + 2047) / 2048));
     }
     const bool staged = tile >= kBlock;
     if (!staged) {
