@@ -121,8 +121,8 @@ inline NcoChan make_nco(uint32_t fs, int32_t f)
         c.r = r;
     }
     c.inv_r = 1.0 / (double)c.r;
-    for (int i = 0; i < 4; ++i)
-        c.step[i] = phasor64(c.nu * i);
+    for (int i = 0; i < kNcoSamples; ++i)
+        c.step[i] = phasor64(c.nu * kNcoOffsets[i]);
     c.omega = w;
     return c;
 }

@@ -76,12 +76,18 @@ __device__ __forceinline__ uint64_t mod_u64(uint64_t a, uint32_t m, double inv_m
 // ---------------------------------------------------------------------------
 
 // one NCO output of the multi-shift kernel
+// the offsets of an NCO thread's samples from its first: pairs 128 apart, so
+// a wave's 16-byte accesses each cover 1 KB of consecutive samples
+constexpr int kNcoSamples = 16;
+constexpr uint32_t kNcoOffsets[kNcoSamples] = {0, 1, 128, 129, 256, 257, 384, 385,
+                                               512, 513, 640, 641, 768, 769, 896, 897};
+
 struct NcoChan {
     double nu;        // f/fs, signed, revolutions per sample
     double inv_r;     // 1/R
     uint32_t r;       // rollover index R (UINT32_MAX for the identity shift)
     uint32_t tab;     // first entry of this shift's phasor table (kNoTable: none)
-    float2 step[4];   // e^{j 2 pi nu i}, i = 0..3
+    float2 step[kNcoSamples];  // e^{j 2 pi nu o_i}, o = kNcoOffsets: a thread's samples relative to its first
     double omega;     // 2 pi f / fs rad/sample, formed as the reference forms it
 };
 

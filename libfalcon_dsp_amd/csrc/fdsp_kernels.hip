@@ -21,48 +21,53 @@ __device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) { retur
 // NCO, one input -> n_chan shifted outputs
 // (reference kernels __freq_shift / __freq_shift_multi_chan,
 //  src/transform/falcon_dsp_freq_shift_cuda.cu:124-216)
-// Each thread owns 4 consecutive samples: one exact phasor per thread and
-// channel, the other three by a product with e^{j 2 pi nu i} (two roundings).
+// Each thread owns 16 samples (pairs 128 apart): one exact phasor per thread and
+// channel, the others by a product with e^{j 2 pi nu o_i} (two roundings).
 // Complex int16 output instead takes the reference's own arithmetic
 // (falcon_dsp_freq_shift.cc:146-151): per sample, angle = omega * s in FP64
 // with s the index modulo R, cos and sin in FP64 rounded to float (from the
 // host-built table where there is one, else sincos here), and the float
 // complex product, so the truncated outputs equal the reference's.
 // ---------------------------------------------------------------------------
-constexpr int kNcoL = 4;
+constexpr int kNcoL = kNcoSamples;
+// A thread's samples: pairs 128 apart (lane l: its wave's samples 2l, 2l+1,
+// 128+2l, 129+2l, ...), so each 16-byte load or store instruction of a wave
+// covers 1 KB of consecutive samples; a wave covers 64 kNcoL samples
+constexpr const uint32_t (&kNcoOff)[kNcoL] = kNcoOffsets;
 
-// four consecutive samples as whole 16-byte accesses (p 16-byte aligned)
-__device__ __forceinline__ void load4(const float2 *p, float2 (&x)[4])
+// a thread's samples as whole aligned accesses, one per pair
+__device__ __forceinline__ void load_pairs(const float2 *p, float2 (&x)[kNcoL])
 {
-    const float4 a = reinterpret_cast<const float4 *>(p)[0];
-    const float4 b = reinterpret_cast<const float4 *>(p)[1];
-    x[0] = make_float2(a.x, a.y);
-    x[1] = make_float2(a.z, a.w);
-    x[2] = make_float2(b.x, b.y);
-    x[3] = make_float2(b.z, b.w);
-}
-struct alignas(16) Ci16x4 {
-    short2 v[4];
-};
-__device__ __forceinline__ void load4(const short2 *p, float2 (&x)[4])
-{
-    const Ci16x4 q = *reinterpret_cast<const Ci16x4 *>(p);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        x[i] = sample_f32(q.v[i]);
+    for (int i = 0; i < kNcoL; i += 2) {
+        const float4 a = *reinterpret_cast<const float4 *>(p + kNcoOff[i]);
+        x[i] = make_float2(a.x, a.y);
+        x[i + 1] = make_float2(a.z, a.w);
+    }
 }
-__device__ __forceinline__ void store4(float2 *p, const float2 (&y)[4])
+__device__ __forceinline__ void load_pairs(const short2 *p, float2 (&x)[kNcoL])
 {
-    reinterpret_cast<float4 *>(p)[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
-    reinterpret_cast<float4 *>(p)[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
-}
-__device__ __forceinline__ void store4(short2 *p, const float2 (&y)[4])
-{
-    Ci16x4 q;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        q.v[i] = make_short2(i16_trunc(y[i].x), i16_trunc(y[i].y));
-    *reinterpret_cast<Ci16x4 *>(p) = q;
+    for (int i = 0; i < kNcoL; i += 2) {
+        const uint2 a = *reinterpret_cast<const uint2 *>(p + kNcoOff[i]);
+        x[i] = make_float2((float)(short)(a.x & 0xffffu), (float)(short)(a.x >> 16));
+        x[i + 1] = make_float2((float)(short)(a.y & 0xffffu), (float)(short)(a.y >> 16));
+    }
+}
+__device__ __forceinline__ void store_pairs(float2 *p, const float2 (&y)[kNcoL])
+{
+#pragma unroll
+    for (int i = 0; i < kNcoL; i += 2)
+        *reinterpret_cast<float4 *>(p + kNcoOff[i]) = make_float4(y[i].x, y[i].y, y[i + 1].x, y[i + 1].y);
+}
+__device__ __forceinline__ void store_pairs(short2 *p, const float2 (&y)[kNcoL])
+{
+    const auto word = [](float2 v) {
+        return (uint32_t)(uint16_t)i16_trunc(v.x) | ((uint32_t)(uint16_t)i16_trunc(v.y) << 16);
+    };
+#pragma unroll
+    for (int i = 0; i < kNcoL; i += 2)
+        *reinterpret_cast<uint2 *>(p + kNcoOff[i]) = make_uint2(word(y[i]), word(y[i + 1]));
 }
 
 template <bool kVec, typename TI, typename TO>
@@ -74,17 +79,18 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ i
     // blockIdx.y picks the shifts c0 .. c1-1 (cpb per block): small chunks
     // spread their shifts over more blocks, re-reading the input from L2
     const uint32_t c0 = blockIdx.y * cpb, c1 = min(n_chan, c0 + cpb);
-    const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kNcoL;
+    // the wave's 256 samples, and this lane's first pair in them
+    const uint32_t i0 = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) * kNcoL + 2 * (threadIdx.x & 63u);
     if (i0 >= n)
         return;
-    const bool full = (i0 + kNcoL) <= n;
+    const bool full = i0 + kNcoOff[kNcoL - 1] < n;
     float2 x[kNcoL];
     if (kVec && full) {
-        load4(in + i0, x);
+        load_pairs(in + i0, x);
     } else {
 #pragma unroll
         for (int i = 0; i < kNcoL; ++i)
-            x[i] = (i0 + i < n) ? sample_f32(in[i0 + i]) : make_float2(0.f, 0.f);
+            x[i] = (i0 + kNcoOff[i] < n) ? sample_f32(in[i0 + kNcoOff[i]]) : make_float2(0.f, 0.f);
     }
     for (uint32_t c = c0; c < c1; ++c) {
         const double nu = chans[c].nu;
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ i
             const uint32_t t0 = chans[c].tab;
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i) {
-                const uint64_t k = k0 + i >= r ? k0 + i - r : k0 + i;
+                const uint64_t k = k0 + kNcoOff[i] >= r ? k0 + kNcoOff[i] - r : k0 + kNcoOff[i];
                 float2 ph;
                 if (tab && t0 != kNoTable) {
                     ph = tab[t0 + k];
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ i
                 }
                 y[i] = cmul_ref(x[i], ph);
             }
-        } else if (k0 + (kNcoL - 1) < r) {
+        } else if (k0 + kNcoOff[kNcoL - 1] < r) {
             const float2 base = nco_phasor((int64_t)k0, nu);
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i)
@@ -118,16 +124,16 @@ __global__ __launch_bounds__(256) void nco_multi_kernel(const TI *__restrict__ i
             const float2 basew = nco_phasor((int64_t)k0 - (int64_t)r, nu);
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i)
-                y[i] = cmul(x[i], cmul((k0 + i < r) ? base : basew, chans[c].step[i]));
+                y[i] = cmul(x[i], cmul((k0 + kNcoOff[i] < r) ? base : basew, chans[c].step[i]));
         }
         TO *o = out + (uint64_t)c * out_stride;
         if (kVec && full) {
-            store4(o + i0, y);
+            store_pairs(o + i0, y);
         } else {
 #pragma unroll
             for (int i = 0; i < kNcoL; ++i)
-                if (i0 + i < n)
-                    sample_put(o + i0 + i, y[i]);
+                if (i0 + kNcoOff[i] < n)
+                    sample_put(o + i0 + kNcoOff[i], y[i]);
         }
     }
 }
@@ -148,7 +154,7 @@ hipError_t launch_nco_multi(const TI *in, uint32_t n, uint64_t start, const NcoC
     const uint32_t groups = min(n_chan, max(1u, target / blocks));
     const uint32_t cpb = (n_chan + groups - 1) / groups;
     const dim3 grid(blocks, (n_chan + cpb - 1) / cpb);
-    // 16-byte accesses of 4 samples: every channel's output must start 16-byte aligned
+    // 16-byte accesses of sample pairs: every channel's output must start 16-byte aligned
     const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16 == 0) &&
                      (out_stride * sizeof(TO)) % 16 == 0;
     if (vec)
