@@ -56,6 +56,7 @@ template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p) { ret
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
+constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer descriptor word 3: 32-bit data format
 
 // rows of the transposed span (odd), fixed per segment length
 __host__ __device__ constexpr uint32_t seg_rows(uint32_t seg) { return ((seg + kSegMaxHalo) / 4 + 1) | 1u; }
@@ -459,12 +460,15 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
         float4 v[kRounds];
         if (base >= 0 && seg_lo + kSeg <= a.n && in16) {
             // interior segment: every pair lies in the chunk, 16-byte aligned
-            // (base and the input are even / 16-byte aligned)
-            const float4 *src = reinterpret_cast<const float4 *>(a.in + base);
+            // (base and the input are even / 16-byte aligned). Buffer loads:
+            // 32-bit offsets on the segment's base, zero past the span.
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<float2 *>(a.in + base), 0, (int)(pairs * 16u), kBufferFlags);
 #pragma unroll
             for (uint32_t r = 0; r < kRounds; ++r) {
-                const uint32_t pr = tid + r * kBlock;
-                v[r] = pr < pairs ? src[pr] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const auto w = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((tid + r * kBlock) * 16u), 0, 0);
+                v[r] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
+                                   __uint_as_float(w[3]));
             }
         } else {
 #pragma unroll
@@ -497,14 +501,16 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
                 v[r] = w;
             }
         }
+        // pair pr = tid + r kBlock holds samples q = 2 pr and q + 1, at slots
+        // (q mod 4) kM + q/4 = 2 (tid mod 2) kM + tid/2 + r kBlock/2 and that
+        // plus kM: one base per thread, constant offsets per round
+        const uint32_t s0 = 2u * (tid & 1u) * kM + (tid >> 1);
+        static_assert(kBlock % 2 == 0, "pairs of one thread keep their residue");
 #pragma unroll
         for (uint32_t r = 0; r < kRounds; ++r) {
-            const uint32_t pr = tid + r * kBlock;
-            if (pr < pairs) {
-                // q even: samples q and q+1 share a transposed column
-                const uint32_t q = 2 * pr;
-                xs[slot<kM>(q)] = make_float2(v[r].x, v[r].y);
-                xs[slot<kM>(q) + kM] = make_float2(v[r].z, v[r].w);
+            if (tid + r * kBlock < pairs) {
+                xs[s0 + r * (kBlock / 2)] = make_float2(v[r].x, v[r].y);
+                xs[s0 + r * (kBlock / 2) + kM] = make_float2(v[r].z, v[r].w);
             }
         }
     }
