@@ -91,7 +91,18 @@ struct ChanRun {
     uint32_t pad[3];
 };
 
-__device__ __forceinline__ ChanRun chan_run(const ChanStatic &c, uint64_t P, uint32_t lo, uint32_t hi, uint32_t n)
+// the ChanStatic fields chan_run reads
+struct RunParams {
+    double inv_down, inv_r;
+    uint32_t up, down, r;
+};
+
+__device__ __forceinline__ RunParams run_params(const ChanStatic &c)
+{
+    return RunParams{c.inv_down, c.inv_r, c.up, c.down, c.r};
+}
+
+__device__ __forceinline__ ChanRun chan_run(const RunParams &c, uint64_t P, uint32_t lo, uint32_t hi, uint32_t n)
 {
     const uint64_t up = c.up, down = c.down;
     const uint64_t uP = up * P;
@@ -449,14 +460,53 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
     const uint32_t span_lo = seg_lo - H;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uniform(tid >> 6);
 
-    // 1. stage the span (chunk-local inputs seg_lo - H .. seg_hi - 1, the part
-    // before the chunk from the history), two samples per thread and load,
-    // into LDS transposed by 4
-    {
+    if (wave == 0) {
+        // 1. bookkeeping, wave 0 (while waves 1.. stage the span): one lane
+        // per channel, then the groups' work item counts (from the channels'
+        // runs this wave has just written) scanned across the wave. Lanes
+        // past the last channel / group repeat it.
+        const RunParams rp = run_params(a.chans[min(lane, a.n_chan - 1)]);
+        uint32_t g_chan, g_kind, g_tail;
+        {
+            const SegGroup &gr = a.groups[min(lane, a.n_groups - 1)];
+            g_chan = gr.chan[0];
+            g_kind = gr.kind;
+            g_tail = gr.tail;
+        }
+        if (lane < a.n_chan)
+            runs[lane] = chan_run(rp, P, seg_lo, seg_hi, a.n);
+        for (uint32_t c = lane + 64; c < a.n_chan; c += 64)
+            runs[c] = chan_run(run_params(a.chans[c]), P, seg_lo, seg_hi, a.n);
+        // the lanes' LDS writes above precede the reads below (one wave)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t cnt = 0;
+        if (lane < a.n_groups) {
+            const ChanRun &r = runs[g_chan];
+            const uint32_t rows = r.j_hi - r.j_lo;
+            cnt = g_kind == kSegMfma ? (rows + 15) / 16 : g_tail ? (rows + 62) / 63 : (rows + 63) / 64;
+        }
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(cnt, d);
+            if (lane >= d)
+                cnt += o;
+        }
+        if (lane < a.n_groups)
+            item0[lane + 1] = cnt;
+        if (lane == 0)
+            item0[0] = 0;
+    } else {
+        // 2. stage the span (chunk-local inputs seg_lo - H .. seg_hi - 1, the
+        // part before the chunk from the history), waves 1..: two samples per
+        // thread and load, into LDS transposed by 4
+        constexpr uint32_t kStage = kBlock - 64;
+        const uint32_t st = tid - 64;
         const int64_t base = (int64_t)seg_lo - (int64_t)H;
         const uint32_t pairs = (seg_hi - seg_lo + H + 1) / 2;
         const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15) == 0;
-        constexpr uint32_t kRounds = (kSeg + kSegMaxHalo) / 2 / kBlock + 1;
+        constexpr uint32_t kRounds = ((kSeg + kSegMaxHalo) / 2 + kStage - 1) / kStage;
         float4 v[kRounds];
         if (base >= 0 && seg_lo + kSeg <= a.n && in16) {
             // interior segment: every pair lies in the chunk, 16-byte aligned
@@ -466,14 +516,14 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
                 const_cast<float2 *>(a.in + base), 0, (int)(pairs * 16u), kBufferFlags);
 #pragma unroll
             for (uint32_t r = 0; r < kRounds; ++r) {
-                const auto w = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((tid + r * kBlock) * 16u), 0, 0);
+                const auto w = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((st + r * kStage) * 16u), 0, 0);
                 v[r] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
                                    __uint_as_float(w[3]));
             }
         } else {
 #pragma unroll
             for (uint32_t r = 0; r < kRounds; ++r) {
-                const uint32_t pr = tid + r * kBlock;
+                const uint32_t pr = st + r * kStage;
                 const int64_t i = base + 2 * (int64_t)pr;
                 float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (pr < pairs) {
@@ -501,42 +551,18 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
                 v[r] = w;
             }
         }
-        // pair pr = tid + r kBlock holds samples q = 2 pr and q + 1, at slots
-        // (q mod 4) kM + q/4 = 2 (tid mod 2) kM + tid/2 + r kBlock/2 and that
+        // pair pr = st + r kStage holds samples q = 2 pr and q + 1, at slots
+        // (q mod 4) kM + q/4 = 2 (st mod 2) kM + st/2 + r kStage/2 and that
         // plus kM: one base per thread, constant offsets per round
-        const uint32_t s0 = 2u * (tid & 1u) * kM + (tid >> 1);
-        static_assert(kBlock % 2 == 0, "pairs of one thread keep their residue");
+        const uint32_t s0 = 2u * (st & 1u) * kM + (st >> 1);
+        static_assert(kStage % 2 == 0, "pairs of one thread keep their residue");
 #pragma unroll
         for (uint32_t r = 0; r < kRounds; ++r) {
-            if (tid + r * kBlock < pairs) {
-                xs[s0 + r * (kBlock / 2)] = make_float2(v[r].x, v[r].y);
-                xs[s0 + r * (kBlock / 2) + kM] = make_float2(v[r].z, v[r].w);
+            if (st + r * kStage < pairs) {
+                xs[s0 + r * (kStage / 2)] = make_float2(v[r].x, v[r].y);
+                xs[s0 + r * (kStage / 2) + kM] = make_float2(v[r].z, v[r].w);
             }
         }
-    }
-
-    // 2. bookkeeping: one thread per channel, and the groups' work item
-    // counts scanned across wave 0
-    for (uint32_t c = tid; c < a.n_chan; c += kBlock)
-        runs[c] = chan_run(a.chans[c], P, seg_lo, seg_hi, a.n);
-    if (wave == 0) {
-        uint32_t cnt = 0;
-        if (lane < a.n_groups) {
-            const SegGroup &gr = a.groups[lane];
-            const ChanRun r = chan_run(a.chans[gr.chan[0]], P, seg_lo, seg_hi, a.n);
-            const uint32_t rows = r.j_hi - r.j_lo;
-            cnt = gr.kind == kSegMfma ? (rows + 15) / 16 : gr.tail ? (rows + 62) / 63 : (rows + 63) / 64;
-        }
-#pragma unroll
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(cnt, d);
-            if (lane >= d)
-                cnt += o;
-        }
-        if (lane < a.n_groups)
-            item0[lane + 1] = cnt;
-        if (lane == 0)
-            item0[0] = 0;
     }
     __syncthreads();
 
