@@ -1340,6 +1340,10 @@ void build_seg_plan(fdsp_channelizer_s *h, const std::vector<float2> &taps)
     }
     if (reach > kSegMaxHalo)
         return;
+    // the kernel's NCO index arithmetic is 32-bit signed: periods up to 2^30
+    for (uint32_t c = 0; c < C; ++c)
+        if (h->stat[c].r > kNoShiftPeriod)
+            return;
     // padded modulated taps
     std::vector<float2> ptaps;
     for (uint32_t c = 0; c < C; ++c) {
@@ -1950,6 +1954,13 @@ extern "C" int fdsp_channelizer_create(uint32_t fs, const fdsp_channel_desc *cha
             s.nu = k.nco.nu;
             s.r = k.nco.r;
             s.inv_r = k.nco.inv_r;
+            if (s.nu == 0.0) {
+                // no shift: the phase is 0 at every index, so any period
+                // gives the same outputs; a long one keeps the kernels'
+                // rollover paths (a window straddling an index restart) idle
+                s.r = kNoShiftPeriod;
+                s.inv_r = 1.0 / (double)kNoShiftPeriod;
+            }
             s.inv_up = 1.0 / (double)k.up;
             s.inv_down = 1.0 / (double)k.down;
             s.up = k.up;

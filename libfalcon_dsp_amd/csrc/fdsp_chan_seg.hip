@@ -188,21 +188,22 @@ __device__ __forceinline__ float2 finish_out(const SegDev &a, const LaneChan &c,
 // MFMA group: this wave's sub-tiles st, st + kWaves, ... < n_st (sub-tile s:
 // segment rows 16 s .. 16 s + 15 of every channel of the group)
 // ---------------------------------------------------------------------------
-// One sub-tile's A operand (eight consecutive slots from p) into two MFMA
-// chains (real-part and imaginary-part K steps); the caller adds them.
-__device__ __forceinline__ void mfma_tile(const float2 *p, const float (&bv)[16], floatx4 &c0, floatx4 &c1)
+// One sub-tile's A operand (eight consecutive slots from p) through one MFMA
+// chain: the K steps of the real and imaginary parts alternate (dependent
+// MFMAs issue back to back).
+__device__ __forceinline__ floatx4 mfma_tile(const float2 *p, const float (&bv)[16])
 {
     float2 x[8];
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j)
         x[j] = p[j];
-    c0 = floatx4{0.f, 0.f, 0.f, 0.f};
-    c1 = floatx4{0.f, 0.f, 0.f, 0.f};
+    floatx4 c = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) {
-        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].x, bv[2 * j], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].y, bv[2 * j + 1], c1, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].x, bv[2 * j], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].y, bv[2 * j + 1], c, 0, 0, 0);
     }
+    return c;
 }
 
 template <uint32_t kM>
@@ -239,9 +240,14 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
     // mod R per step, and its phasor ph = e^{j2pi nu k}, advanced by e^{j2pi
     // nu dk}; the second row is d1 = down mod R later. Where an index wraps
     // past R the phase jumps (the reference's rollover): that lane's phasor
-    // is evaluated afresh in FP64.
-    uint32_t k = (uint32_t)mod_u64((uint64_t)runs[ci].k0 + (uint64_t)(run.j_lo + 16 * st + ro) * down, R, lc.inv_r);
-    float2 ph = nco_phasor((int64_t)k, lc.nu);
+    // is evaluated afresh in FP64. The lane tracks kw = k - (win - 1) (signed):
+    // its rows need no rollover correction and its second row does not wrap
+    // while 0 <= kw < quiet = R - d1 - (win - 1).
+    const int32_t w1 = (int32_t)lc.win - 1;
+    int32_t kw = (int32_t)mod_u64((uint64_t)runs[ci].k0 + (uint64_t)(run.j_lo + 16 * st + ro) * down, R, lc.inv_r) - w1;
+    const int32_t r_w = (int32_t)R - w1;  // kw at which k wraps
+    const uint32_t quiet = (uint32_t)max((int32_t)(R - d1) - w1, 0);
+    float2 ph = nco_phasor((int64_t)(kw + w1), lc.nu);
     // A operand: row lane%16 reads window samples g + 4j, eight consecutive
     // slots; rows of a whole sub-tile are unclamped, so the next one's base is
     // (16 kWaves down)/4 slots on. A partial sub-tile clamps its rows to the
@@ -252,16 +258,43 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
         return slot<kM>((run.j_lo + row) * down + run.rho - 31 - span_lo + g);
     };
     const bool pair16 = a.out16 && (run.j_lo & 1) == 0;
-    // epilogue of sub-tile s from its two chains
-    const auto finish = [&](uint32_t s, const floatx4 &c0, const floatx4 &c1) {
-        const floatx4 acc = c0 + c1;
+    // the lane's two outputs of sub-tile s (rows ro, ro + 1), not yet phased
+    const auto rows_of = [&](const floatx4 &acc, float2 &y0, float2 &y1) {
         float oth[4];
 #pragma unroll
         for (uint32_t e = 0; e < 4; ++e)
             oth[e] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[e]), 0xB1, 0xF, 0xF, false));
-        float2 y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
-        float2 y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
+        y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
+        y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
+    };
+    const auto store = [&](uint32_t rr, float2 o0, float2 o1, bool two) {
+        const uint32_t j = run.j_lo + rr;
+        if (two) {
+            if (pair16)
+                ((gvec4_ptr)(lc.out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+            else {
+                lc.out[j] = fvec2{o0.x, o0.y};
+                lc.out[j + 1] = fvec2{o1.x, o1.y};
+            }
+        } else {
+            lc.out[j] = fvec2{o0.x, o0.y};
+        }
+    };
+    // epilogue of sub-tile s. Quiet (every row of the sub-tile exists and no
+    // lane is near a wrap): phase and store. Otherwise the general form:
+    // partial rows, a second row that wraps, rollover corrections.
+    const auto finish = [&](uint32_t s, const floatx4 &acc) {
+        float2 y0, y1;
+        rows_of(acc, y0, y1);
         const uint32_t rr = 16 * s + ro;  // segment row of y0
+        const bool quiet_here = (uint32_t)kw < quiet;
+        if (16 * (s + 1) <= n_rows && __builtin_amdgcn_ballot_w64(!quiet_here) == 0) {
+            const float2 ph1 = cmul(ph, e_d1);
+            if (live)
+                store(rr, cmul(ph, y0), cmul(ph1, y1), true);
+            return;
+        }
+        const uint32_t k = (uint32_t)(kw + w1);
         uint32_t k1 = k + d1;
         float2 ph1 = cmul(ph, e_d1);
         if (k1 >= R) {  // rare: the second row's index wraps
@@ -282,42 +315,27 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
             cmac(y0, lc.wrap_m1, p0);
             cmac(y1, lc.wrap_m1, p1);
         }
-        const float2 o0 = cmul(ph, y0), o1 = cmul(ph1, y1);
-        if (live && rr < n_rows) {
-            const uint32_t j = run.j_lo + rr;
-            if (rr + 1 < n_rows) {
-                if (pair16)
-                    ((gvec4_ptr)(lc.out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
-                else {
-                    lc.out[j] = fvec2{o0.x, o0.y};
-                    lc.out[j + 1] = fvec2{o1.x, o1.y};
-                }
-            } else {
-                lc.out[j] = fvec2{o0.x, o0.y};
-            }
-        }
+        if (live && rr < n_rows)
+            store(rr, cmul(ph, y0), cmul(ph1, y1), rr + 1 < n_rows);
     };
     // software pipeline: the next sub-tile's LDS reads and MFMAs are issued
     // before this one's epilogue, so no wave waits on its own products
     uint32_t ab = a_base(st);
-    floatx4 c0, c1;
-    mfma_tile(xs + ab, bv, c0, c1);
+    floatx4 c = mfma_tile(xs + ab, bv);
     for (uint32_t sn = st + kWaves; sn < n_st; sn += kWaves) {
         ab = 16 * (sn + 1) <= n_rows ? ab + 4 * kWaves * down : a_base(sn);
-        floatx4 n0, n1;
-        mfma_tile(xs + ab, bv, n0, n1);
-        finish(st, c0, c1);
-        c0 = n0;
-        c1 = n1;
+        const floatx4 n = mfma_tile(xs + ab, bv);
+        finish(st, c);
+        c = n;
         st = sn;
-        k += dk;
+        kw += (int32_t)dk;
         ph = cmul(ph, e_dk);
-        if (k >= R) {  // rare: the index wrapped
-            k -= R;
-            ph = nco_phasor((int64_t)k, lc.nu);
+        if (kw >= r_w) {  // rare: the index wrapped
+            kw -= (int32_t)R;
+            ph = nco_phasor((int64_t)(kw + w1), lc.nu);
         }
     }
-    finish(st, c0, c1);
+    finish(st, c);
 }
 
 // ---------------------------------------------------------------------------

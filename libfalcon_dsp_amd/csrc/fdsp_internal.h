@@ -259,6 +259,7 @@ constexpr uint32_t kSegMaxChan = 256;    // channels of one plan at most
 constexpr uint32_t kSegWaves = 4;        // waves per workgroup (256 threads)
 constexpr uint32_t kSegMaxGroups = 64;   // work groups of one plan at most
 constexpr uint32_t kSegMaxHalo = 256;    // staged history samples at most
+constexpr uint32_t kNoShiftPeriod = 1u << 30;  // NCO period used for a zero shift (any period is exact there)
 struct SegGroup {
     uint32_t kind;     // kSegMfma or kSegValu
     uint32_t m;        // channels
