@@ -158,6 +158,8 @@ struct SegDev {
     uint64_t p;
     uint32_t parity;
     uint32_t out16;      // every channel's output is 16-byte aligned
+    uint32_t n_big;      // workgroups 0 .. n_big-1 take segments of the full length,
+    uint32_t small;      // later ones (the chunk's end) segments of this length
 };
 
 // the chunk's position, from the device record in graph mode
@@ -469,8 +471,11 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
     const uint64_t P = pos.P;
     const uint32_t parity = pos.parity;
     const float2 *hist = parity ? a.hist1 : a.hist0;
-    const uint32_t seg_lo = blockIdx.x * kSeg;  // chunk-local first input of the segment
-    const uint32_t seg_hi = min(a.n, seg_lo + kSeg);
+    // chunk-local first input and nominal length of the segment
+    const uint32_t seg_lo =
+        blockIdx.x < a.n_big ? blockIdx.x * kSeg : a.n_big * kSeg + (blockIdx.x - a.n_big) * a.small;
+    const uint32_t seg_len = blockIdx.x < a.n_big ? kSeg : a.small;
+    const uint32_t seg_hi = min(a.n, seg_lo + seg_len);
     const uint32_t H = a.halo;
     // chunk-local index of span sample 0 (wraps below zero in the first
     // segment; window starts q = anchor - (w-1) - span_lo stay >= 0 in
@@ -526,7 +531,7 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
         const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15) == 0;
         constexpr uint32_t kRounds = ((kSeg + kSegMaxHalo) / 2 + kStage - 1) / kStage;
         float4 v[kRounds];
-        if (base >= 0 && seg_lo + kSeg <= a.n && in16) {
+        if (base >= 0 && seg_lo + seg_len <= a.n && in16) {
             // interior segment: every pair lies in the chunk, 16-byte aligned
             // (base and the input are even / 16-byte aligned). Buffer loads:
             // 32-bit offsets on the segment's base, zero past the span.
@@ -628,10 +633,21 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
     }
 }
 
+// The chunk's last 1/kTailDiv or so runs in segments kTailSplit times
+// shorter: the workgroups dispatched last are short, so the launch drains
+// sooner (every workgroup of the full length has to end before it does).
+// 16 and 2 measured best on the 32-channel workload (1/8 and 1/32, and
+// quarter-length segments, were slower).
+constexpr uint32_t kTailDiv = 16, kTailSplit = 2;
+
 template <uint32_t kSeg>
-hipError_t launch_seg(const SegDev &a, uint32_t seg, size_t lds, hipStream_t stream)
+hipError_t launch_seg(SegDev a, size_t lds, hipStream_t stream)
 {
-    const uint32_t blocks = (a.n + seg - 1) / seg;
+    static_assert(kSeg % (kTailSplit * 256) == 0, "short segments stay multiples of 256 inputs");
+    const uint32_t whole = a.n / kSeg, tail = (a.n / kTailDiv + kSeg - 1) / kSeg;
+    a.n_big = whole > tail ? whole - tail : 0;
+    a.small = kSeg / kTailSplit;
+    const uint32_t blocks = a.n_big + (a.n - a.n_big * kSeg + a.small - 1) / a.small;
     hipLaunchKernelGGL(chan_seg_kernel<kSeg>, dim3(blocks), dim3(kBlock), lds, stream, a);
     return hipGetLastError();
 }
@@ -670,11 +686,11 @@ hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
     const size_t lds = chan_seg_lds_bytes(s.seg, s.halo, s.n_chan);
     switch (s.seg) {
     case 1024:
-        return launch_seg<1024>(a, s.seg, lds, stream);
+        return launch_seg<1024>(a, lds, stream);
     case 2048:
-        return launch_seg<2048>(a, s.seg, lds, stream);
+        return launch_seg<2048>(a, lds, stream);
     case 4096:
-        return launch_seg<4096>(a, s.seg, lds, stream);
+        return launch_seg<4096>(a, lds, stream);
     default:
         return hipErrorInvalidValue;
     }
