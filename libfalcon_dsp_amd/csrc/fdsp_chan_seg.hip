@@ -220,10 +220,11 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
     asm volatile("" : "+v"(lane));
     float bv[16];
     {
-        const float *bt = a.btab + gr->btab;
+        // one lane pointer, the steps at immediate offsets
+        const float *bt = a.btab + gr->btab + lane;
 #pragma unroll
         for (uint32_t s = 0; s < 16; ++s)
-            bv[s] = bt[s * 64 + lane];
+            bv[s] = bt[s * 64];
     }
     // D[i][col] sits in lane col + 16*(i/4), element i%4: lanes 2c and 2c+1
     // hold channel c's real and imaginary parts of rows 4*(lane/16) .. +3.
