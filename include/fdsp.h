@@ -455,8 +455,8 @@ int fdsp_channelizer_get_tiling(fdsp_channelizer_t h, uint32_t *segment_len, uin
  *    segment stages it once and computes every channel's outputs in it
  *    (groups of 2..8 channels with up 1, equal down and <= 32 taps on FP32
  *    matrix cores, the others on FP32 vector ALUs). Its tiling takes
- *    segment_len 1024, 2048 or 4096 (256 threads); the autotuner times the
- *    three.
+ *    segment_len a multiple of 256 from 1024 to 4096 (256 threads; default
+ *    2560); the autotuner times 1024, 2048, 2560, 3072, 3584 and 4096.
  *  - FDSP_KERNEL_GROUPS_F32: the channel-group kernel in FP32 (windows
  *    above 64 taps), with the work splits described above.
  *  - FDSP_KERNEL_GROUPS_BF16X2: FDSP_CHAN_BATCHED_BF16X2.

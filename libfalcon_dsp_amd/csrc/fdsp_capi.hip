@@ -1105,7 +1105,7 @@ struct fdsp_channelizer_s {
     // whenever every channel fits it (windows of at most 64 taps)
     bool seg_ok = false;   // the segment plan fits this channel set
     bool use_seg = false;  // launches use it (fdsp_channelizer_set_kernel)
-    uint32_t seg_len_in = 2048, seg_halo = 0;
+    uint32_t seg_len_in = 2560, seg_halo = 0;
     std::vector<SegGroup> seg_groups;
     std::vector<float> seg_btab;
     DevBuf d_seg_groups, d_seg_btab, d_ptaps;
@@ -2430,8 +2430,8 @@ extern "C" int fdsp_channelizer_set_tiling(fdsp_channelizer_t h, uint32_t segmen
         return set_err(FDSP_EINVAL, "tiling applies to the batched mode");
     if (h->use_seg) {
         // segment kernel: the segment length is the only parameter
-        if (segment_len != 1024 && segment_len != 2048 && segment_len != 4096)
-            return set_err(FDSP_EINVAL, "the segment kernel takes segment_len 1024, 2048 or 4096");
+        if (!chan_seg_len_ok(segment_len))
+            return set_err(FDSP_EINVAL, "the segment kernel takes segment_len a multiple of 256 from 1024 to 4096");
         if (h->graph_exec)
             return set_err(FDSP_EBUSY, "destroy the graph before retiling");
         h->seg_len_in = segment_len;
@@ -2650,7 +2650,7 @@ static int autotune_run(fdsp_channelizer_t h, const fdsp_cf32 *d_in, size_t chun
             (void)hipEventDestroy(t0);
             return set_err(FDSP_EHIP, "event creation failed");
         }
-        static const uint32_t kSegCand[] = {2048, 1024, 4096};
+        static const uint32_t kSegCand[] = {2560, 3072, 2048, 3584, 4096, 1024};
         const uint32_t seg0 = h->seg_len_in;
         float best = -1.f;
         uint32_t best_seg = seg0;

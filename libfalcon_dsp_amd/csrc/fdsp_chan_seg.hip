@@ -58,11 +58,13 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
 constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer descriptor word 3: 32-bit data format
 
-// rows of the transposed span (odd), fixed per segment length
-__host__ __device__ constexpr uint32_t seg_rows(uint32_t seg) { return ((seg + kSegMaxHalo) / 4 + 1) | 1u; }
+// segment lengths: multiples of kSegSplit inputs from 1024 to kSegMax
+constexpr uint32_t kSegMax = 4096, kSegSplit = 256;
+// rows of the transposed span (odd) for a segment length and halo
+__host__ __device__ constexpr uint32_t seg_rows(uint32_t seg, uint32_t halo) { return ((seg + halo) / 4 + 1) | 1u; }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-template <uint32_t kM> __device__ __forceinline__ uint32_t slot(uint32_t q) { return (q & 3u) * kM + (q >> 2); }
+__device__ __forceinline__ uint32_t slot(uint32_t q, uint32_t km) { return (q & 3u) * km + (q >> 2); }
 
 // floor(a / d) for a < 2^52 from an FP64 quotient estimate and one exact
 // correction step (inv_d = 1/d rounded to double)
@@ -158,7 +160,9 @@ struct SegDev {
     uint64_t p;
     uint32_t parity;
     uint32_t out16;      // every channel's output is 16-byte aligned
-    uint32_t n_big;      // workgroups 0 .. n_big-1 take segments of the full length,
+    uint32_t seg;        // segment length (a multiple of kSegSplit, at most kSegMax)
+    uint32_t km;         // rows of the transposed span, seg_rows(seg, halo)
+    uint32_t n_big;      // workgroups 0 .. n_big-1 take segments of length seg,
     uint32_t small;      // later ones (the chunk's end) segments of this length
 };
 
@@ -172,7 +176,6 @@ struct SegPos {
 // correction when the window straddles a restart (window positions t <
 // win-1-k precede it; taps of phase ph; q: the window's first span sample),
 // then the phasor.
-template <uint32_t kM>
 __device__ __forceinline__ float2 finish_out(const SegDev &a, const LaneChan &c, uint32_t ph, const float2 *xs,
                                              uint32_t q, uint32_t k, float2 y)
 {
@@ -180,7 +183,7 @@ __device__ __forceinline__ float2 finish_out(const SegDev &a, const LaneChan &c,
         const float2 *g = a.ptaps + c.poff + ph * c.win;
         float2 part = make_float2(0.f, 0.f);
         for (uint32_t t = 0; t + 1 + k < c.win; ++t)
-            cmac(part, g[t], xs[slot<kM>(q + t)]);
+            cmac(part, g[t], xs[slot(q + t, a.km)]);
         cmac(y, c.wrap_m1, part);
     }
     return cmul(nco_phasor((int64_t)k, c.nu), y);
@@ -208,7 +211,6 @@ __device__ __forceinline__ floatx4 mfma_tile(const float2 *p, const float (&bv)[
     return c;
 }
 
-template <uint32_t kM>
 __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, const ChanRun &run,
                                            const ChanRun *runs, const float2 *xs, uint32_t span_lo, uint32_t st,
                                            uint32_t n_st, uint32_t lane)
@@ -258,7 +260,7 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
     const uint32_t i = lane & 15;
     const auto a_base = [&](uint32_t s) -> uint32_t {
         const uint32_t row = min(16 * s + i, n_rows - 1);
-        return slot<kM>((run.j_lo + row) * down + run.rho - 31 - span_lo + g);
+        return slot((run.j_lo + row) * down + run.rho - 31 - span_lo + g, a.km);
     };
     const bool pair16 = a.out16 && (run.j_lo & 1) == 0;
     // the lane's two outputs of sub-tile s (rows ro, ro + 1), not yet phased
@@ -312,9 +314,9 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
             const float2 *gt = a.ptaps + lc.poff;
             float2 p0 = make_float2(0.f, 0.f), p1 = make_float2(0.f, 0.f);
             for (uint32_t t = 0; roll0 && t + 1 + k < lc.win; ++t)
-                cmac(p0, gt[t], xs[slot<kM>(q0 + t)]);
+                cmac(p0, gt[t], xs[slot(q0 + t, a.km)]);
             for (uint32_t t = 0; roll1 && t + 1 + k1 < lc.win; ++t)
-                cmac(p1, gt[t], xs[slot<kM>(q0 + down + t)]);
+                cmac(p1, gt[t], xs[slot(q0 + down + t, a.km)]);
             cmac(y0, lc.wrap_m1, p0);
             cmac(y1, lc.wrap_m1, p1);
         }
@@ -354,7 +356,7 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
 // chunk's last output is carried to the next chunk.
 // The window's samples of residue u (window positions u, u+4, ...) are
 // consecutive slots from base u.
-template <uint32_t kW, uint32_t kM>
+template <uint32_t kW>
 __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool tail, const ChanRun &run,
                                            const SegPos &pos, const float2 *xs, uint32_t span_lo, uint32_t b,
                                            uint32_t n_b, uint32_t lane)
@@ -390,7 +392,7 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
         const float2 *base[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u)
-            base[u] = xs + slot<kM>(q + u);
+            base[u] = xs + slot(q + u, a.km);
         // four independent chains: even and odd taps, real and imaginary;
         // eight taps (two per residue) per step
         float2 acc0 = make_float2(0.f, 0.f), acc1 = make_float2(0.f, 0.f);
@@ -425,7 +427,7 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
             k += lc.r;
         if (k >= lc.r)
             k -= lc.r;
-        float2 y = finish_out<kM>(a, lc, ph, xs, q, k, acc);
+        float2 y = finish_out(a, lc, ph, xs, q, k, acc);
         if (!tail) {
             if (rr < n_rows)
                 lc.out[j] = fvec2{y.x, y.y};
@@ -454,14 +456,13 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
 // ---------------------------------------------------------------------------
 // the kernel: one segment per workgroup
 // ---------------------------------------------------------------------------
-// at most 5 waves per SIMD fit the LDS of a 2048-sample segment anyway
-template <uint32_t kSeg>
+// at least 5 waves per SIMD: 5 workgroups of 3840-sample segments fit a CU's LDS
 __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
 {
-    constexpr uint32_t kM = seg_rows(kSeg);
-    // dynamic LDS: the transposed span (4 kM samples), then one ChanRun per channel
+    const uint32_t km = a.km;
+    // dynamic LDS: the transposed span (4 km samples), then one ChanRun per channel
     extern __shared__ float2 xs[];
-    ChanRun *runs = reinterpret_cast<ChanRun *>(xs + 4 * kM);
+    ChanRun *runs = reinterpret_cast<ChanRun *>(xs + 4 * km);
     __shared__ uint32_t item0[kSegMaxGroups + 1];
 
     SegPos pos{a.p, a.parity};
@@ -474,8 +475,8 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
     const float2 *hist = parity ? a.hist1 : a.hist0;
     // chunk-local first input and nominal length of the segment
     const uint32_t seg_lo =
-        blockIdx.x < a.n_big ? blockIdx.x * kSeg : a.n_big * kSeg + (blockIdx.x - a.n_big) * a.small;
-    const uint32_t seg_len = blockIdx.x < a.n_big ? kSeg : a.small;
+        blockIdx.x < a.n_big ? blockIdx.x * a.seg : a.n_big * a.seg + (blockIdx.x - a.n_big) * a.small;
+    const uint32_t seg_len = blockIdx.x < a.n_big ? a.seg : a.small;
     const uint32_t seg_hi = min(a.n, seg_lo + seg_len);
     const uint32_t H = a.halo;
     // chunk-local index of span sample 0 (wraps below zero in the first
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
         const int64_t base = (int64_t)seg_lo - (int64_t)H;
         const uint32_t pairs = (seg_hi - seg_lo + H + 1) / 2;
         const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15) == 0;
-        constexpr uint32_t kRounds = ((kSeg + kSegMaxHalo) / 2 + kStage - 1) / kStage;
+        constexpr uint32_t kRounds = ((kSegMax + kSegMaxHalo) / 2 + kStage - 1) / kStage;
         float4 v[kRounds];
         if (base >= 0 && seg_lo + seg_len <= a.n && in16) {
             // interior segment: every pair lies in the chunk, 16-byte aligned
@@ -576,15 +577,15 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
             }
         }
         // pair pr = st + r kStage holds samples q = 2 pr and q + 1, at slots
-        // (q mod 4) kM + q/4 = 2 (st mod 2) kM + st/2 + r kStage/2 and that
-        // plus kM: one base per thread, constant offsets per round
-        const uint32_t s0 = 2u * (st & 1u) * kM + (st >> 1);
+        // (q mod 4) km + q/4 = 2 (st mod 2) km + st/2 + r kStage/2 and that
+        // plus km: two bases per thread, constant offsets per round
+        const uint32_t s0 = 2u * (st & 1u) * km + (st >> 1), s1 = s0 + km;
         static_assert(kStage % 2 == 0, "pairs of one thread keep their residue");
 #pragma unroll
         for (uint32_t r = 0; r < kRounds; ++r) {
             if (st + r * kStage < pairs) {
                 xs[s0 + r * (kStage / 2)] = make_float2(v[r].x, v[r].y);
-                xs[s0 + r * (kStage / 2) + kM] = make_float2(v[r].z, v[r].w);
+                xs[s1 + r * (kStage / 2)] = make_float2(v[r].z, v[r].w);
             }
         }
     }
@@ -603,14 +604,14 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
         const cptr<SegGroup> gr = groups + g;
         const ChanRun run = runs[gr->chan[0]];
         if (gr->kind == kSegMfma) {
-            mfma_group<kM>(a, gr, run, runs, xs, span_lo, i - i0, i1 - i0, lane);
+            mfma_group(a, gr, run, runs, xs, span_lo, i - i0, i1 - i0, lane);
         } else {
             const uint32_t ci = gr->chan[0];
             const bool tail = gr->tail != 0;
             if (gr->win == 32)
-                valu_group<32, kM>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
+                valu_group<32>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
             else
-                valu_group<64, kM>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
+                valu_group<64>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
         }
     }
 
@@ -641,24 +642,15 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
 // quarter-length segments, were slower).
 constexpr uint32_t kTailDiv = 16, kTailSplit = 2;
 
-template <uint32_t kSeg>
-hipError_t launch_seg(SegDev a, size_t lds, hipStream_t stream)
-{
-    static_assert(kSeg % (kTailSplit * 256) == 0, "short segments stay multiples of 256 inputs");
-    const uint32_t whole = a.n / kSeg, tail = (a.n / kTailDiv + kSeg - 1) / kSeg;
-    a.n_big = whole > tail ? whole - tail : 0;
-    a.small = kSeg / kTailSplit;
-    const uint32_t blocks = a.n_big + (a.n - a.n_big * kSeg + a.small - 1) / a.small;
-    hipLaunchKernelGGL(chan_seg_kernel<kSeg>, dim3(blocks), dim3(kBlock), lds, stream, a);
-    return hipGetLastError();
-}
+static_assert(kSegSplit % (kTailSplit * 16) == 0, "short segments stay multiples of 16 inputs (16-byte aligned spans)");
 
 }  // namespace
 
+bool chan_seg_len_ok(uint32_t seg) { return seg >= 1024 && seg <= kSegMax && seg % kSegSplit == 0; }
+
 size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo, uint32_t n_chan)
 {
-    (void)halo;  // the transposed span is sized for the largest halo
-    return (size_t)4 * seg_rows(seg) * sizeof(float2) + (size_t)n_chan * sizeof(ChanRun);
+    return (size_t)4 * seg_rows(seg, halo) * sizeof(float2) + (size_t)n_chan * sizeof(ChanRun);
 }
 
 hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
@@ -684,17 +676,17 @@ hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
     a.p = s.p;
     a.parity = s.parity;
     a.out16 = s.out16 ? 1u : 0u;
-    const size_t lds = chan_seg_lds_bytes(s.seg, s.halo, s.n_chan);
-    switch (s.seg) {
-    case 1024:
-        return launch_seg<1024>(a, lds, stream);
-    case 2048:
-        return launch_seg<2048>(a, lds, stream);
-    case 4096:
-        return launch_seg<4096>(a, lds, stream);
-    default:
+    if (!chan_seg_len_ok(s.seg) || s.halo > kSegMaxHalo)
         return hipErrorInvalidValue;
-    }
+    a.seg = s.seg;
+    a.km = seg_rows(s.seg, s.halo);
+    const uint32_t whole = a.n / a.seg, tail = (a.n / kTailDiv + a.seg - 1) / a.seg;
+    a.n_big = whole > tail ? whole - tail : 0;
+    a.small = a.seg / kTailSplit;
+    const uint32_t blocks = a.n_big + (a.n - a.n_big * a.seg + a.small - 1) / a.small;
+    const size_t lds = chan_seg_lds_bytes(s.seg, s.halo, s.n_chan);
+    hipLaunchKernelGGL(chan_seg_kernel, dim3(blocks), dim3(kBlock), lds, stream, a);
+    return hipGetLastError();
 }
 
 }  // namespace fdsp

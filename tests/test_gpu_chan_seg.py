@@ -89,7 +89,7 @@ def test_mixed_channel_set_against_oracle():
                 assert k == rk
 
 
-@pytest.mark.parametrize("seg", [1024, 4096])
+@pytest.mark.parametrize("seg", [1024, 1280, 3072, 3840, 4096])
 def test_segment_lengths_agree(seg):
     fs = 20_000_000
     chans = _mixed(fs)
@@ -99,6 +99,16 @@ def test_segment_lengths_agree(seg):
     for c in range(len(chans)):
         assert other[c].size == base[c].size
         assert_close_rms(other[c], base[c], rms(x), tol=F32_TOL, what=f"chan {c}")
+
+
+def test_segment_length_limits():
+    g = fd.MultiRateChannelizer(20_000_000, _mixed())
+    for bad in (512, 768, 1000, 2560 + 128, 4352):
+        with pytest.raises(fd.FdspError):
+            g.set_tiling(bad, 256)
+    assert g.get_tiling()[0] == 2560  # the default
+    g.set_tiling(1280, 256)
+    assert g.get_tiling()[0] == 1280
 
 
 def test_chunkings_agree():
