@@ -1478,6 +1478,9 @@ SegArgs seg_args(const fdsp_channelizer_s *h, const fdsp_cf32 *d_in, uint32_t n,
     a.out16 = true;
     for (const ChanStatic &st : h->stat)
         a.out16 = a.out16 && reinterpret_cast<uintptr_t>(st.out) % 16 == 0;
+    a.tails = false;
+    for (const SegGroup &g : h->seg_groups)
+        a.tails = a.tails || g.tail != 0;
     return a;
 }
 

@@ -236,23 +236,31 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
     const uint32_t col = lane & 15, g = lane >> 4;
     const bool odd = col & 1, live = (col >> 1) < m;
     const uint32_t ci = gr->chan[min(col >> 1, m - 1)];
-    const ChanStatic &cs = a.chans[ci];
-    const LaneChan lc = lane_chan(cs);
-    const uint32_t R = lc.r, dk = cs.dk, d1 = cs.d1;
-    const float2 e_dk = cs.e_dk, e_d1 = cs.e_d1;
+    // the constants of the quiet steps stay in registers; the rare paths
+    // (index wraps, rollover corrections) read the others from the record
+    const ChanStatic *csp = a.chans + ci;
+    const uint32_t R = csp->r, dk = csp->dk, d1 = csp->d1;
+    const float2 e_dk = csp->e_dk, e_d1 = csp->e_d1;
+    const gvec2_ptr out = (gvec2_ptr)csp->out;
+    const auto rec = [&]() {
+        const ChanStatic *p = csp;
+        asm volatile("" : "+v"(p));  // read at the use, not held across the loop
+        return p;
+    };
     const uint32_t ro = 4 * g + (odd ? 2u : 0u);  // sub-tile row of the lane's first output
     // NCO index k of the lane's first row, advanced by dk = (16 kWaves down)
     // mod R per step, and its phasor ph = e^{j2pi nu k}, advanced by e^{j2pi
     // nu dk}; the second row is d1 = down mod R later. Where an index wraps
     // past R the phase jumps (the reference's rollover): that lane's phasor
-    // is evaluated afresh in FP64. The lane tracks kw = k - (win - 1) (signed):
-    // its rows need no rollover correction and its second row does not wrap
-    // while 0 <= kw < quiet = R - d1 - (win - 1).
-    const int32_t w1 = (int32_t)lc.win - 1;
-    int32_t kw = (int32_t)mod_u64((uint64_t)runs[ci].k0 + (uint64_t)(run.j_lo + 16 * st + ro) * down, R, lc.inv_r) - w1;
+    // is evaluated afresh in FP64. The lane tracks kw = k - (win - 1) (signed;
+    // MFMA groups have 32-tap windows): its rows need no rollover correction
+    // and its second row does not wrap while 0 <= kw < quiet = R - d1 - 31.
+    constexpr int32_t w1 = 31;
+    int32_t kw =
+        (int32_t)mod_u64((uint64_t)runs[ci].k0 + (uint64_t)(run.j_lo + 16 * st + ro) * down, R, csp->inv_r) - w1;
     const int32_t r_w = (int32_t)R - w1;  // kw at which k wraps
     const uint32_t quiet = (uint32_t)max((int32_t)(R - d1) - w1, 0);
-    float2 ph = nco_phasor((int64_t)(kw + w1), lc.nu);
+    float2 ph = nco_phasor((int64_t)(kw + w1), csp->nu);
     // A operand: row lane%16 reads window samples g + 4j, eight consecutive
     // slots; rows of a whole sub-tile are unclamped, so the next one's base is
     // (16 kWaves down)/4 slots on. A partial sub-tile clamps its rows to the
@@ -276,13 +284,13 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
         const uint32_t j = run.j_lo + rr;
         if (two) {
             if (pair16)
-                ((gvec4_ptr)(lc.out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+                ((gvec4_ptr)(out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
             else {
-                lc.out[j] = fvec2{o0.x, o0.y};
-                lc.out[j + 1] = fvec2{o1.x, o1.y};
+                out[j] = fvec2{o0.x, o0.y};
+                out[j + 1] = fvec2{o1.x, o1.y};
             }
         } else {
-            lc.out[j] = fvec2{o0.x, o0.y};
+            out[j] = fvec2{o0.x, o0.y};
         }
     };
     // epilogue of sub-tile s. Quiet (every row of the sub-tile exists and no
@@ -304,21 +312,23 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
         float2 ph1 = cmul(ph, e_d1);
         if (k1 >= R) {  // rare: the second row's index wraps
             k1 -= R;
-            ph1 = nco_phasor((int64_t)k1, lc.nu);
+            ph1 = nco_phasor((int64_t)k1, rec()->nu);
         }
-        const bool roll0 = rr < n_rows && k + 1 < lc.win, roll1 = rr + 1 < n_rows && k1 + 1 < lc.win;
+        const bool roll0 = rr < n_rows && k < (uint32_t)w1, roll1 = rr + 1 < n_rows && k1 < (uint32_t)w1;
         if (roll0 || roll1) {
             // rare: a window of a row that exists straddles a rollover; its
             // samples before it carry e^{j2pi nu R}
+            const ChanStatic *c = rec();
             const uint32_t q0 = (run.j_lo + rr) * down + run.rho - 31 - span_lo;
-            const float2 *gt = a.ptaps + lc.poff;
+            const float2 *gt = a.ptaps + c->poff;
+            const float2 wrap_m1 = make_float2(c->wrap.x - 1.f, c->wrap.y);
             float2 p0 = make_float2(0.f, 0.f), p1 = make_float2(0.f, 0.f);
-            for (uint32_t t = 0; roll0 && t + 1 + k < lc.win; ++t)
+            for (uint32_t t = 0; roll0 && t + k < (uint32_t)w1; ++t)
                 cmac(p0, gt[t], xs[slot(q0 + t, a.km)]);
-            for (uint32_t t = 0; roll1 && t + 1 + k1 < lc.win; ++t)
+            for (uint32_t t = 0; roll1 && t + k1 < (uint32_t)w1; ++t)
                 cmac(p1, gt[t], xs[slot(q0 + down + t, a.km)]);
-            cmac(y0, lc.wrap_m1, p0);
-            cmac(y1, lc.wrap_m1, p1);
+            cmac(y0, wrap_m1, p0);
+            cmac(y1, wrap_m1, p1);
         }
         if (live && rr < n_rows)
             store(rr, cmul(ph, y0), cmul(ph1, y1), rr + 1 < n_rows);
@@ -337,7 +347,7 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
         ph = cmul(ph, e_dk);
         if (kw >= r_w) {  // rare: the index wrapped
             kw -= (int32_t)R;
-            ph = nco_phasor((int64_t)(kw + w1), lc.nu);
+            ph = nco_phasor((int64_t)(kw + w1), rec()->nu);
         }
     }
     finish(st, c);
@@ -408,14 +418,16 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
                 }
             }
         } else {
+            // per-lane taps (the lane's filter phase): four per step, so the
+            // step's operands stay within the registers of the up 1 path
             const float2 *g = a.ptaps + cs->poff + ph * kW;
 #pragma unroll 1
-            for (uint32_t m = 0; m < kW / 4; m += 2) {
+            for (uint32_t m = 0; m < kW / 4; ++m) {
 #pragma unroll
-                for (uint32_t e = 0; e < 8; e += 2) {
+                for (uint32_t e = 0; e < 4; e += 2) {
                     const uint32_t t = 4 * m + e;
-                    cmac(acc0, g[t], base[e & 3][m + (e >> 2)]);
-                    cmac(acc1, g[t + 1], base[(e + 1) & 3][m + (e >> 2)]);
+                    cmac(acc0, g[t], base[e][m]);
+                    cmac(acc1, g[t + 1], base[e + 1][m]);
                 }
             }
         }
@@ -456,7 +468,11 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
 // ---------------------------------------------------------------------------
 // the kernel: one segment per workgroup
 // ---------------------------------------------------------------------------
-// at least 5 waves per SIMD: 5 workgroups of 3840-sample segments fit a CU's LDS
+// kTails: the plan has discriminator tails. Their code (the angle, the
+// neighbour exchange, the carried sample) sets the kernel's register peak, so
+// plans without them run an instance that leaves it out (fewer VGPRs, more
+// workgroups per CU).
+template <bool kTails>
 __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
 {
     const uint32_t km = a.km;
@@ -607,7 +623,7 @@ __global__ __launch_bounds__(kBlock, 5) void chan_seg_kernel(SegDev a)
             mfma_group(a, gr, run, runs, xs, span_lo, i - i0, i1 - i0, lane);
         } else {
             const uint32_t ci = gr->chan[0];
-            const bool tail = gr->tail != 0;
+            const bool tail = kTails && gr->tail != 0;
             if (gr->win == 32)
                 valu_group<32>(a, ci, tail, run, pos, xs, span_lo, i - i0, i1 - i0, lane);
             else
@@ -685,7 +701,10 @@ hipError_t launch_chan_seg(const SegArgs &s, hipStream_t stream)
     a.small = a.seg / kTailSplit;
     const uint32_t blocks = a.n_big + (a.n - a.n_big * a.seg + a.small - 1) / a.small;
     const size_t lds = chan_seg_lds_bytes(s.seg, s.halo, s.n_chan);
-    hipLaunchKernelGGL(chan_seg_kernel, dim3(blocks), dim3(kBlock), lds, stream, a);
+    if (s.tails)
+        hipLaunchKernelGGL(chan_seg_kernel<true>, dim3(blocks), dim3(kBlock), lds, stream, a);
+    else
+        hipLaunchKernelGGL(chan_seg_kernel<false>, dim3(blocks), dim3(kBlock), lds, stream, a);
     return hipGetLastError();
 }
 

@@ -285,6 +285,7 @@ struct SegArgs {
     uint64_t p;
     uint32_t parity;
     bool out16;
+    bool tails;  // some group ends in the polar discriminator
 };
 hipError_t launch_chan_seg(const SegArgs &a, hipStream_t s);
 size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo, uint32_t n_chan);
