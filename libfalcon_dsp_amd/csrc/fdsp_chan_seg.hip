@@ -82,26 +82,37 @@ __device__ __forceinline__ uint64_t udiv64(uint64_t a, uint32_t d, double inv_d)
 // Stream bookkeeping of one channel for this chunk (global input position
 // P), the same for every block but j_lo / j_hi: the chunk's outputs start at
 // stream output n0 = ceil(P*up/down); its output j has chunk-local anchor
-// floor((j*down + rho)/up) with rho = n0*down - up*P in [0, down); k0 is the
-// NCO index of output 0's anchor; outputs j_lo .. j_hi-1 have their anchors in
-// this block's segment, and the chunk emits j_end outputs in all.
+// floor((j*down + rho)/up) with rho = n0*down - up*P in [0, down); outputs
+// j_lo .. j_hi-1 have their anchors in this block's segment, the first of
+// them at chunk-local input a_lo, whose NCO index is k_lo = (P + a_lo) mod R;
+// the chunk emits j_end outputs in all.
 struct ChanRun {
     uint32_t rho;
-    uint32_t k0;
     uint32_t j_lo, j_hi;
     uint32_t j_end;
-    uint32_t pad[3];
+    uint32_t a_lo, k_lo;
+    uint32_t pad[2];
 };
 
 // the ChanStatic fields chan_run reads
 struct RunParams {
-    double inv_down, inv_r;
+    double inv_down, inv_r, inv_up;
     uint32_t up, down, r;
 };
 
 __device__ __forceinline__ RunParams run_params(const ChanStatic &c)
 {
-    return RunParams{c.inv_down, c.inv_r, c.up, c.down, c.r};
+    return RunParams{c.inv_down, c.inv_r, c.inv_up, c.up, c.down, c.r};
+}
+
+// a mod m for a below about 2m (one subtraction), else the FP64 form
+__device__ __forceinline__ uint32_t mod_near(uint64_t a, uint32_t m, double inv_m)
+{
+    if (a < m)
+        return (uint32_t)a;
+    if (a - m < m)
+        return (uint32_t)(a - m);
+    return (uint32_t)mod_u64(a, m, inv_m);
 }
 
 __device__ __forceinline__ ChanRun chan_run(const RunParams &c, uint64_t P, uint32_t lo, uint32_t hi, uint32_t n)
@@ -118,7 +129,9 @@ __device__ __forceinline__ ChanRun chan_run(const RunParams &c, uint64_t P, uint
     r.j_lo = first_at(lo);
     r.j_hi = first_at(hi);
     r.j_end = first_at(n);
-    r.k0 = (uint32_t)mod_u64(P + r.rho / c.up, c.r, c.inv_r);
+    const uint64_t num = (uint64_t)r.j_lo * down + r.rho;
+    r.a_lo = (uint32_t)(c.up == 1 ? num : udiv64(num, c.up, c.inv_up));
+    r.k_lo = (uint32_t)mod_u64(P + r.a_lo, c.r, c.inv_r);
     return r;
 }
 
@@ -257,7 +270,7 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
     // and its second row does not wrap while 0 <= kw < quiet = R - d1 - 31.
     constexpr int32_t w1 = 31;
     int32_t kw =
-        (int32_t)mod_u64((uint64_t)runs[ci].k0 + (uint64_t)(run.j_lo + 16 * st + ro) * down, R, csp->inv_r) - w1;
+        (int32_t)mod_near((uint64_t)runs[ci].k_lo + (uint64_t)(16 * st + ro) * down, R, csp->inv_r) - w1;
     const int32_t r_w = (int32_t)R - w1;  // kw at which k wraps
     const uint32_t quiet = (uint32_t)max((int32_t)(R - d1) - w1, 0);
     float2 ph = nco_phasor((int64_t)(kw + w1), csp->nu);
@@ -380,8 +393,7 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
     // NCO index of the segment's first output's anchor (output 0's anchor is
     // floor(rho/up)); later anchors of the segment are fewer than R inputs on
     // (the host keeps channels with ceil(down/up) + segment >= R off this path)
-    const uint32_t a_lo = (uint32_t)(((uint64_t)run.j_lo * down + run.rho) / up);
-    const uint32_t k_lo = (uint32_t)mod_u64((uint64_t)run.k0 + (a_lo - run.rho / up), lc.r, lc.inv_r);
+    const uint32_t a_lo = run.a_lo, k_lo = run.k_lo;
     for (; b < n_b; b += kWaves) {
         const int32_t rr = tail ? (int32_t)(b * 63 + lane) - 1 : (int32_t)(b * 64 + lane);
         // the row computed: rows past the segment's last take the last; row -1
