@@ -293,17 +293,18 @@ __device__ __forceinline__ void mfma_group(const SegDev &a, cptr<SegGroup> gr, c
         y0 = odd ? make_float2(oth[2], acc[2]) : make_float2(acc[0], oth[0]);
         y1 = odd ? make_float2(oth[3], acc[3]) : make_float2(acc[1], oth[1]);
     };
+    // outputs are streamed (non-temporal): nothing in this launch reads them
     const auto store = [&](uint32_t rr, float2 o0, float2 o1, bool two) {
         const uint32_t j = run.j_lo + rr;
         if (two) {
             if (pair16)
-                ((gvec4_ptr)(out + j))[0] = fvec4{o0.x, o0.y, o1.x, o1.y};
+                __builtin_nontemporal_store(fvec4{o0.x, o0.y, o1.x, o1.y}, (gvec4_ptr)(out + j));
             else {
-                out[j] = fvec2{o0.x, o0.y};
-                out[j + 1] = fvec2{o1.x, o1.y};
+                __builtin_nontemporal_store(fvec2{o0.x, o0.y}, out + j);
+                __builtin_nontemporal_store(fvec2{o1.x, o1.y}, out + j + 1);
             }
         } else {
-            out[j] = fvec2{o0.x, o0.y};
+            __builtin_nontemporal_store(fvec2{o0.x, o0.y}, out + j);
         }
     };
     // epilogue of sub-tile s. Quiet (every row of the sub-tile exists and no
@@ -454,7 +455,7 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
         float2 y = finish_out(a, lc, ph, xs, q, k, acc);
         if (!tail) {
             if (rr < n_rows)
-                lc.out[j] = fvec2{y.x, y.y};
+                __builtin_nontemporal_store(fvec2{y.x, y.y}, lc.out + j);
             continue;
         }
         // discriminator tail (see above)
@@ -473,7 +474,7 @@ __device__ __forceinline__ void valu_group(const SegDev &a, uint32_t ci, bool ta
         if (j < drop)
             continue;
         const float2 z = cmul(y, make_float2(yp.x, -yp.y));
-        ((__attribute__((address_space(1))) float *)lc.out)[j - drop] = atan2f(z.y, z.x);
+        __builtin_nontemporal_store(atan2f(z.y, z.x), (__attribute__((address_space(1))) float *)lc.out + (j - drop));
     }
 }
 
