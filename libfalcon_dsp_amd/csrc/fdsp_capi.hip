@@ -2471,14 +2471,15 @@ extern "C" int fdsp_channelizer_plan_info(fdsp_channelizer_t h, size_t chunk_len
     if (h->mode != FDSP_CHAN_BATCHED)
         return set_err(FDSP_EINVAL, "plan info applies to the batched mode");
     if (h->use_seg) {
+        const SegGeom geo = chan_seg_geometry((uint32_t)chunk_len, h->seg_len_in, h->seg_halo, (uint32_t)h->ch.size());
         if (blocks)
-            *blocks = (uint32_t)ceil_div(chunk_len, h->seg_len_in);
+            *blocks = chunk_len ? geo.blocks : 0u;
         if (capacity)
             *capacity = h->capacity;
         if (groups)
             *groups = (uint32_t)h->seg_groups.size();
         if (lds_bytes)
-            *lds_bytes = (uint32_t)chan_seg_lds_bytes(h->seg_len_in, h->seg_halo, (uint32_t)h->ch.size());
+            *lds_bytes = (uint32_t)geo.lds;
         return FDSP_OK;
     }
     std::vector<uint32_t> t;

@@ -290,6 +290,14 @@ struct SegArgs {
 hipError_t launch_chan_seg(const SegArgs &a, hipStream_t s);
 size_t chan_seg_lds_bytes(uint32_t seg, uint32_t halo, uint32_t n_chan);
 bool chan_seg_len_ok(uint32_t seg);  // a segment length the segment kernel takes
+// what a launch of the segment kernel over n inputs runs: the segment length
+// used, the workgroups of full length and the shorter length after them, the
+// grid and the LDS per workgroup
+struct SegGeom {
+    uint32_t seg, n_big, small, blocks;
+    size_t lds;
+};
+SegGeom chan_seg_geometry(uint32_t n, uint32_t seg, uint32_t halo, uint32_t n_chan);
 // resident blocks of the fused kernel per CU for a block size, build and LDS size
 int chan_groups_occupancy(uint32_t block, bool wide, size_t lds_bytes);
 // advance the device position by n inputs and flip the history parity
