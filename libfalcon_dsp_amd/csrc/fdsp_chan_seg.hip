@@ -33,10 +33,11 @@
 //         bank row of its filter phase (n*down) mod up.
 //
 // LDS layout of the staged span: transposed by 4, span sample q at slot
-// (q mod 4)*kM + q/4 (kM odd). Samples 4 apart are adjacent, so a lane's
-// eight MFMA samples (or a VALU window's samples of one residue) are eight
-// consecutive slots, read from one base address with immediate offsets; the
-// 16 rows of a down-8 sub-tile read slots 2 apart, on distinct banks.
+// (q mod 4)*km + q/4, km odd and sized for the segment and the plan's halo.
+// Samples 4 apart are adjacent, so a lane's eight MFMA samples (or a VALU
+// window's samples of one residue) are eight consecutive slots, read from one
+// base address with immediate offsets; the 16 rows of a down-8 sub-tile read
+// slots 2 apart, on distinct banks.
 #include <algorithm>
 
 #include "fdsp_internal.h"
