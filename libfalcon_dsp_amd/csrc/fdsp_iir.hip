@@ -1236,14 +1236,16 @@ __global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
     if (tid < g.nb)
         bsh[tid] = g.b[tid];
     float2 st[P];  // wave 0: the stream's carried outputs, newest first
-    float2 pf[kBankPer];  // waves 1-3: the next tile's inputs in flight
+    // waves 1-3: the inputs of the next two tiles in flight (tile q in
+    // pf[q % 2]), so each load has two phases to land
+    float2 pf0[kBankPer], pf1[kBankPer];
     if (tid < 64) {
 #pragma unroll
         for (int k = 0; k < P; ++k)
             st[k] = tid < rows ? g.yh[(size_t)(s0 + tid) * kMaxFb + k] : make_float2(0.f, 0.f);
     }
     const uint32_t p = tid - 64;  // producer index (waves 1-3)
-    const auto load_tile = [&](uint32_t q) {
+    const auto load_tile = [&](uint32_t q, float2 (&pf)[kBankPer]) {
         const uint32_t t0 = q * kBankT, cnt = tile_cnt(q);
 #pragma unroll
         for (uint32_t i = 0; i < kBankPer; ++i) {
@@ -1253,8 +1255,10 @@ __global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
         }
     };
     if (tid >= 64 && nt > 0)
-        load_tile(0);
-    for (uint32_t q = 0; q < nt + 3; ++q) {
+        load_tile(0, pf0);
+    if (tid >= 64 && nt > 1)
+        load_tile(1, pf1);
+    const auto phase = [&](uint32_t q, float2 (&pf)[kBankPer]) {
         if (tid < 64) {
             // the recurrence of tile q - 2 along the lane's stream
             if (q >= 2 && q - 2 < nt && tid < rows) {
@@ -1282,8 +1286,8 @@ __global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
                         xq[r * xrow + k] = q == 0 ? g.xh[(size_t)(s0 + r) * kBankH + kBankH - h + k]
                                                   : xt[((q - 1) % 2) * 64 * xrow + r * xrow + tile_cnt(q - 1) + k];
                 }
-                if (q + 1 < nt)
-                    load_tile(q + 1);
+                if (q + 2 < nt)
+                    load_tile(q + 2, pf);
             }
             if (q >= 1 && q - 1 < nt) {
                 // b-parts of tile q - 1: ((0 + b0 x[n]) + b1 x[n-1]) + ...,
@@ -1325,6 +1329,11 @@ __global__ __launch_bounds__(256) void iir_bank_kernel(BankLaunch g, Fb<P> fb)
             }
         }
         __syncthreads();
+    };
+    for (uint32_t q = 0; q < nt + 3; q += 2) {
+        phase(q, pf0);
+        if (q + 1 < nt + 3)
+            phase(q + 1, pf1);
     }
     // carried state: the last h inputs of (carried inputs, last tile), and the outputs
     if (tid < 64 && tid < rows) {
