@@ -177,9 +177,67 @@ template hipError_t launch_nco_multi<short2, short2>(const short2 *, uint32_t, u
 //  per-output (x_idx, phase) pairs are computed on the device in closed form)
 // A block owns a tile of consecutive outputs. Dense tiles stage their input
 // span in LDS and read the real taps as scalars; sparse tiles (long
-// decimation) read each window straight from global memory.
+// decimation) read each window straight from global memory. kCpp > 0 is a
+// build for exactly kCpp taps per phase: the phase's taps are loaded into
+// scalar registers once and the window reads issued in groups (a scalar load
+// and an LDS read share one wait counter, so the loop over a runtime tap
+// count waits for both every four taps).
 // ---------------------------------------------------------------------------
-template <int kBlock, bool kStaged>
+typedef const __attribute__((address_space(4))) float *ctaps_ptr;
+typedef float rs_fv2 __attribute__((ext_vector_type(2)));
+
+// one output: the window xw[0 .. cpp-1] (LDS) against the phase's taps g
+// (wavefront-uniform), accumulated in the reference's tap order
+template <int kCpp>
+__device__ __forceinline__ float2 resampler_dot(const float2 *xw, ctaps_ptr g, uint32_t cpp)
+{
+    float2 acc = make_float2(0.f, 0.f);
+    if constexpr (kCpp > 0) {
+        // all taps into scalar registers, then the window eight reads at a time
+        static_assert(kCpp % 8 == 0, "unrolled builds take multiples of 8 taps per phase");
+        float gr[kCpp];
+#pragma unroll
+        for (int t = 0; t < kCpp; ++t)
+            gr[t] = g[t];
+#pragma unroll
+        for (int t0 = 0; t0 < kCpp; t0 += 8) {
+            float2 xr[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                xr[t] = xw[t0 + t];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                acc.x = fmaf(xr[t].x, gr[t0 + t], acc.x);
+                acc.y = fmaf(xr[t].y, gr[t0 + t], acc.y);
+            }
+        }
+    } else {
+        // four taps per step (scalar operands)
+        uint32_t t = 0;
+        for (; t + 4 <= cpp; t += 4) {
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const float2 x = xw[t + e];
+                acc.x = fmaf(x.x, g[t + e], acc.x);
+                acc.y = fmaf(x.y, g[t + e], acc.y);
+            }
+        }
+        for (; t < cpp; ++t) {
+            const float2 x = xw[t];
+            acc.x = fmaf(x.x, g[t], acc.x);
+            acc.y = fmaf(x.y, g[t], acc.y);
+        }
+    }
+    return acc;
+}
+
+// outputs are written once and not read back by the launch: streaming stores
+__device__ __forceinline__ void store_stream(float2 *dst, float2 v)
+{
+    __builtin_nontemporal_store(rs_fv2{v.x, v.y}, (__attribute__((address_space(1))) rs_fv2 *)dst);
+}
+
+template <int kBlock, bool kStaged, int kCpp = 0>
 __global__ __launch_bounds__(kBlock) void resampler_kernel(const float2 *__restrict__ in, uint32_t n, uint64_t n0,
                                                            uint64_t p, uint32_t up, uint32_t down, uint32_t cpp,
                                                            const float *__restrict__ taps,
@@ -211,9 +269,23 @@ __global__ __launch_bounds__(kBlock) void resampler_kernel(const float2 *__restr
         const int64_t lo = a_first - hlen;
         const uint32_t span = (uint32_t)(a_last - lo + 1);
         float2 *xs = reinterpret_cast<float2 *>(smem);
-        for (uint32_t t = threadIdx.x; t < span; t += kBlock) {
-            const int64_t i = lo + t;
-            xs[t] = i >= 0 ? in[i] : hist_in[hlen + i];
+        // four loads in flight a thread: unconditional, with the index
+        // clamped into the span and the source picked by pointer, so the
+        // one wait is at the first LDS write
+        for (uint32_t t0 = 0; t0 < span; t0 += 4 * kBlock) {
+            float2 v[4];
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const int64_t i = lo + min(t0 + threadIdx.x + e * kBlock, span - 1);
+                const float2 *src = i >= 0 ? in + i : hist_in + (hlen + i);
+                v[e] = *src;
+            }
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint32_t t = t0 + threadIdx.x + e * kBlock;
+                if (t < span)
+                    xs[t] = v[e];
+            }
         }
         // Outputs q = c (mod up) share the filter phase (r0 + c down) mod up:
         // a wave takes 64 outputs of one class (q = c + up (64 i + lane)), so
@@ -221,7 +293,6 @@ __global__ __launch_bounds__(kBlock) void resampler_kernel(const float2 *__restr
         // samples come from LDS. Work item k: class k mod up, slice k / up.
         __syncthreads();
         constexpr uint32_t kWaves = kBlock / 64;
-        typedef const __attribute__((address_space(4))) float *ctaps;
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
         const uint32_t per_class = (nt + up - 1) / up, slices = (per_class + 63) / 64;
         for (uint32_t k = wave; k < up * slices; k += kWaves) {
@@ -230,29 +301,12 @@ __global__ __launch_bounds__(kBlock) void resampler_kernel(const float2 *__restr
             // first anchor (uniform) plus down per output of the class
             const uint64_t nc = (uint64_t)r0 + (uint64_t)c * down;
             const uint32_t ph = (uint32_t)(nc % up), ac = (uint32_t)(nc / up);
-            const ctaps g = (ctaps)taps + ph * cpp;
             const uint32_t oc = 64 * i + lane;  // the output's index within its class
             const uint32_t q = c + up * oc;
             if (q >= nt)
                 continue;
             const float2 *xw = xs + ac + down * oc;  // window start, relative to lo
-            // taps in the reference's order, four per step (scalar operands)
-            float2 acc = make_float2(0.f, 0.f);
-            uint32_t t = 0;
-            for (; t + 4 <= cpp; t += 4) {
-#pragma unroll
-                for (uint32_t e = 0; e < 4; ++e) {
-                    const float2 x = xw[t + e];
-                    acc.x = fmaf(x.x, g[t + e], acc.x);
-                    acc.y = fmaf(x.y, g[t + e], acc.y);
-                }
-            }
-            for (; t < cpp; ++t) {
-                const float2 x = xw[t];
-                acc.x = fmaf(x.x, g[t], acc.x);
-                acc.y = fmaf(x.y, g[t], acc.y);
-            }
-            out[j0 + q] = acc;
+            store_stream(out + j0 + q, resampler_dot<kCpp>(xw, (ctaps_ptr)taps + ph * cpp, cpp));
         }
     } else {
         for (uint32_t q = threadIdx.x; q < nt; q += kBlock) {
@@ -321,8 +375,17 @@ hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t u
     if (staged) {
         const uint32_t span_max = (uint32_t)(((tile - 1) * down + up) / up + cpp + 1);
         const size_t lds = (size_t)span_max * sizeof(float2);
-        hipLaunchKernelGGL((resampler_kernel<kBlock, true>), dim3(blocks), dim3(kBlock), lds, s, in, n, n0, p, up,
-                           down, cpp, taps, hist_in, hist_out, out, count, (uint32_t)tile);
+        // unrolled builds for common filter lengths per phase
+#define FDSP_RS_LAUNCH(C)                                                                                          \
+    hipLaunchKernelGGL((resampler_kernel<kBlock, true, C>), dim3(blocks), dim3(kBlock), lds, s, in, n, n0, p, up, \
+                       down, cpp, taps, hist_in, hist_out, out, count, (uint32_t)tile)
+        switch (cpp) {
+        case 16: FDSP_RS_LAUNCH(16); break;
+        case 24: FDSP_RS_LAUNCH(24); break;
+        case 32: FDSP_RS_LAUNCH(32); break;
+        default: FDSP_RS_LAUNCH(0); break;
+        }
+#undef FDSP_RS_LAUNCH
     } else {
         hipLaunchKernelGGL((resampler_kernel<kBlock, false>), dim3(blocks), dim3(kBlock), 0, s, in, n, n0, p, up,
                            down, cpp, taps, hist_in, hist_out, out, count, (uint32_t)tile);
