@@ -383,6 +383,7 @@ hipError_t launch_resampler(const float2 *in, uint32_t n, uint64_t p, uint32_t u
         case 16: FDSP_RS_LAUNCH(16); break;
         case 24: FDSP_RS_LAUNCH(24); break;
         case 32: FDSP_RS_LAUNCH(32); break;
+        case 64: FDSP_RS_LAUNCH(64); break;
         default: FDSP_RS_LAUNCH(0); break;
         }
 #undef FDSP_RS_LAUNCH
