@@ -83,10 +83,12 @@ def test_random_chunking_matches_oracle(up, down, ntaps):
     _stream_compare(up, down, taps, x, [1, 2, 3, 5, 997, 4096, 13, 1])
 
 
-@pytest.mark.parametrize("up,down,ntaps", [(4, 5, 96), (1, 2, 32), (2, 3, 31), (3, 2, 30), (5, 1, 17)])
+@pytest.mark.parametrize("up,down,ntaps", [(4, 5, 96), (1, 2, 32), (2, 3, 31), (3, 2, 30), (5, 1, 17), (5, 2, 80),
+                                           (2, 9, 48)])
 def test_long_chunks_match_oracle(up, down, ntaps):
-    # chunks of a million outputs and more: the launch runs several tiles per
-    # block, with ragged last tiles and the history carried in between
+    # long chunks in many tiles, ragged last tiles and the history carried in
+    # between; 16, 24 and 32 taps per phase take the unrolled builds, and
+    # (5, 2, 80) and (2, 9, 48) add an odd up and a decimation by 9 to them
     taps = kaiser_sinc(ntaps, 0.5 / max(up, down), 8.0, gain=up).astype(np.complex64)
     x = cwgn(17 * up + down, 1_100_001)
     _stream_compare(up, down, taps, x, [600_001, 500_000])
