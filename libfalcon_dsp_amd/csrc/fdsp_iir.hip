@@ -44,6 +44,10 @@ namespace {
 constexpr uint32_t kLanes = 64;                  // one wave per block (lockstep in-place LDS use)
 constexpr uint32_t kMaxSeg = 64;                 // outputs per lane: 8 to 64, by chunk length
 constexpr uint32_t kMinWarm = 32, kMaxWarm = 2048;
+// added to the shortest warm-up that merges on the host's trial starts: real
+// signals miss more often than the trial noise (FM angles: 0.5 us of repairs a
+// chunk without it)
+constexpr uint32_t kWarmMargin = 32;
 constexpr uint32_t kMaxB = 256, kMaxFb = 16;
 
 template <bool C> using Val = typename std::conditional<C, float2, float>::type;
@@ -832,7 +836,7 @@ extern "C" int fdsp_iir_create(const fdsp_cf32 *b, uint32_t nb, const fdsp_cf32 
     h->warm_fast = warm_for(h->a, h->p, &unused, 0x1p-26);
     h->merges = segments_merge(h->b, h->a, h->p, h->warm);
     if (h->merges && h->decays)
-        h->warm = shortest_merging_warm(h->b, h->a, h->p, h->warm);
+        h->warm = std::min(h->warm, shortest_merging_warm(h->b, h->a, h->p, h->warm) + kWarmMargin);
     int rc = h->d_b.ensure(nb * sizeof(float2));
     if (!rc)
         rc = h->d_xh[0].ensure(std::max<uint32_t>(nb - 1, 1) * sizeof(float2));
