@@ -11,7 +11,8 @@
 // from the carried state). For a stable filter two float trajectories draw
 // together and, once one step's products round alike, are identical from then
 // on; `warm` is sized on the host from the filter's own decay (the shortest
-// length between the 2^-30 and 2^-44 decays at which trial segments meet).
+// length between the 2^-30 and 2^-44 decays at which trial segments meet,
+// plus a margin).
 //
 // Nothing rests on that argument: every lane records its state when its
 // segment starts and when it ends. A segment whose start state equals its
