@@ -71,15 +71,28 @@ __global__ __launch_bounds__(256) void rescale_kernel(const float2 *__restrict__
 // second pass re-reads the chunk from L2)
 constexpr uint32_t kPeakThreads = 1024, kPeakOneBlock = 1u << 16;
 
+// kPer > 0: at most kPer samples a thread, kept in registers between the
+// two passes (one read of the chunk); kPer = 0: any n up to kPeakOneBlock
+template <int kPer>
 __global__ __launch_bounds__(kPeakThreads) void peak_rescale_kernel(const float2 *__restrict__ v, uint32_t n,
                                                                    int16_t *__restrict__ out,
                                                                    float *__restrict__ audio)
 {
     __shared__ float wm[kPeakThreads / 64];
     float m = 0.f;
+    float a[kPer > 0 ? kPer : 1];
+    if constexpr (kPer > 0) {
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const uint32_t i = threadIdx.x + e * kPeakThreads;
+            a[e] = i < n ? v[i].x : 0.f;
+            m = fmaxf(m, fabsf(a[e]));
+        }
+    } else {
 #pragma unroll 8
-    for (uint32_t i = threadIdx.x; i < n; i += kPeakThreads)
-        m = fmaxf(m, fabsf(v[i].x));
+        for (uint32_t i = threadIdx.x; i < n; i += kPeakThreads)
+            m = fmaxf(m, fabsf(v[i].x));
+    }
     for (int o = 32; o > 0; o >>= 1)
         m = fmaxf(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0)
@@ -90,12 +103,24 @@ __global__ __launch_bounds__(kPeakThreads) void peak_rescale_kernel(const float2
     for (uint32_t w = 1; w < kPeakThreads / 64; ++w)
         m = fmaxf(m, wm[w]);
     const float s = peak_scale(m);
+    if constexpr (kPer > 0) {
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const uint32_t i = threadIdx.x + e * kPeakThreads;
+            if (i < n) {
+                out[i] = i16_trunc(a[e] * s);
+                if (audio)
+                    audio[i] = a[e];
+            }
+        }
+    } else {
 #pragma unroll 8
-    for (uint32_t i = threadIdx.x; i < n; i += kPeakThreads) {
-        const float a = v[i].x;
-        out[i] = i16_trunc(a * s);
-        if (audio)
-            audio[i] = a;
+        for (uint32_t i = threadIdx.x; i < n; i += kPeakThreads) {
+            const float x = v[i].x;
+            out[i] = i16_trunc(x * s);
+            if (audio)
+                audio[i] = x;
+        }
     }
 }
 
@@ -251,8 +276,11 @@ extern "C" int fdsp_fm_demod_apply_device(fdsp_fm_demod_t h, const fdsp_cf32 *d_
     if (n3 == 0)
         return FDSP_OK;
     // rescale to a peak of 10000 (fm_demod.cc:251-266)
-    if (n3 <= kPeakOneBlock) {
-        hipLaunchKernelGGL(peak_rescale_kernel, dim3(1), dim3(kPeakThreads), 0, s, h->d_mono.as<float2>(),
+    if (n3 <= 16 * kPeakThreads) {
+        hipLaunchKernelGGL(peak_rescale_kernel<16>, dim3(1), dim3(kPeakThreads), 0, s, h->d_mono.as<float2>(),
+                           (uint32_t)n3, d_out, d_audio);
+    } else if (n3 <= kPeakOneBlock) {
+        hipLaunchKernelGGL(peak_rescale_kernel<0>, dim3(1), dim3(kPeakThreads), 0, s, h->d_mono.as<float2>(),
                            (uint32_t)n3, d_out, d_audio);
     } else {
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, h->d_max.as<uint32_t>());

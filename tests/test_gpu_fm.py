@@ -81,6 +81,26 @@ def test_fm_demod_vs_oracle(tau):
     assert differ <= 0.05 * total, f"{differ} of {total} int16 samples one step apart"
 
 
+def test_fm_demod_long_chunks():
+    # the rescale's three forms: up to 16 audio samples a thread in registers
+    # (the chunks above), one block re-reading the chunk (20,000 audio
+    # samples), and the multi-block max then rescale (68,000)
+    iso, mono = fm_taps()
+    offset = -200_000
+    x = fm_input(21, 2_200_000, -offset)
+    g, o = fd.FmDemodulator(FS, offset, iso, mono), O.Fm(FS, offset, 75e-6, iso, mono)
+    for k, (lo, hi) in enumerate([(0, 500_000), (500_000, 2_200_000)]):
+        gi, ga = g.demod_mono(x[lo:hi], audio=True)
+        oi, oa = o.demod(x[lo:hi])
+        assert gi.size == oi.size, f"chunk {k}: {gi.size} vs {oi.size}"
+        assert gi.size > (16_384, 65_536)[k]  # past the register form, then past one block
+        r = np.sqrt(np.mean(oa.astype(np.float64) ** 2))
+        assert np.max(np.abs(ga - oa)) <= 1e-4 * r, f"chunk {k}: audio error {np.max(np.abs(ga - oa)):.3e}"
+        d = np.abs(gi.astype(np.int32) - oi)
+        assert d.max() <= 1, f"chunk {k}: int16 max diff {d.max()}"
+        assert np.count_nonzero(d) <= 0.05 * d.size
+
+
 def test_fm_demod_device_and_errors():
     iso, mono = fm_taps()
     x = fm_input(3, 200_000, 50_000)
